@@ -1,0 +1,305 @@
+/*
+ * nricp_hip.h -- C ABI of the MI355X (gfx950) implementation of the NRICP
+ * inner loop: closest point on the target (LBVH), correspondence filters and
+ * IRLS weights, block Gauss-Newton assembly of the 6-unknowns-per-node system,
+ * block-Jacobi PCG, Horn local/global rotation fits, and the coarse-to-fine
+ * schedule that drives them.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Points are row-major (n, 3) float64,
+ *     triangles row-major (n, 3) int32, rotations row-major (n, 3, 3).
+ *   - Host arrays are borrowed for the duration of a call; nothing is retained
+ *     except by a session or graph handle, which copies what it needs.
+ *   - Every function returns 0 on success, NRICP_HIP_INVALID_ARGUMENT for a
+ *     caller error (the Python layer raises ValueError) or
+ *     NRICP_HIP_DEVICE_ERROR for a device/runtime failure (RuntimeError); the
+ *     message is then available from nricp_hip_last_error() on the same thread.
+ *     No C++ exception crosses this boundary.
+ *   - A handle is used by one thread at a time; different handles may be used
+ *     from different threads (each session owns its own HIP stream).
+ *
+ * Reference interfaces replaced (paths in the reference repository):
+ * each declaration names the reference function or binding it stands in for.
+ */
+#ifndef NRICP_HIP_H
+#define NRICP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRICP_HIP_OK 0
+#define NRICP_HIP_INVALID_ARGUMENT 1
+#define NRICP_HIP_DEVICE_ERROR 2
+
+/* Thread-local message of the last failing call (empty when none).
+ * Replaces throw_on_cuda_error -> std::runtime_error (src/cuda/cuda_error.h:14-19). */
+const char* nricp_hip_last_error(void);
+
+/* device_info() -> {available, device} (src/python/bindings.cpp:1614-1622,
+ * device_available / device_description in src/cuda/gpu_solver.cuh:59-60). */
+int nricp_hip_device_info(int32_t* available, char* description, size_t capacity);
+
+/* Waits for all work on the current device (no reference counterpart: the
+ * benchmark brackets its timed region with it). */
+int nricp_hip_device_synchronize(void);
+
+/* build_info() values (src/core/build_info.h:10-15): keys "version",
+ * "build_type", "cxx_compiler", "hip_runtime", "offload_arch",
+ * "accelerated_precision". Returns "" for an unknown key. */
+const char* nricp_hip_build_value(const char* key);
+
+/* ------------------------------------------------------------ operators ---- */
+
+/* Area-weighted unit vertex normals (src/core/mesh.cpp:71-88,
+ * binding vertex_normals, src/python/bindings.cpp:1009-1019). */
+int nricp_hip_vertex_normals(const double* points, int64_t vertex_count, const int32_t* faces,
+                             int64_t face_count, double* out_normals);
+
+/* Closest point, interpolated normal, distance and triangle per query
+ * (TargetQueries ctor + query, src/geometry/closest_point.cpp:123-279;
+ * binding closest_points, src/python/bindings.cpp:966-1007).
+ * out_vertex_normals (optional) receives the target's vertex normals. */
+int nricp_hip_closest_points(const double* target_points, int64_t target_vertex_count,
+                             const int32_t* target_faces, int64_t target_face_count,
+                             const double* queries, int64_t query_count, double* out_points,
+                             double* out_normals, double* out_distances, int32_t* out_triangles,
+                             double* out_vertex_normals);
+
+/* Correspondences with the mask / distance / normal / slope filters and the
+ * rejection breakdown (find_correspondences, src/solver/correspondence.cpp:18-96;
+ * binding find_correspondences, src/python/bindings.cpp:1201-1251).
+ * mask: NULL or query_count bytes (nonzero = participates).
+ * out_counts[6] = total, accepted, rejected_distance, rejected_normal,
+ * rejected_slope, rejected_masked; out_stats[3] = mean, p95, max. */
+int nricp_hip_find_correspondences(const double* target_points, int64_t target_vertex_count,
+                                   const int32_t* target_faces, int64_t target_face_count,
+                                   const double* queries, const double* query_normals,
+                                   int64_t query_count, double distance_threshold,
+                                   double min_normal_cosine, double min_slope, const uint8_t* mask,
+                                   double* out_points, double* out_normals, double* out_distances,
+                                   int32_t* out_triangles, int32_t* out_rejection,
+                                   int64_t* out_counts, double* out_stats);
+
+/* IRLS weights (robust_weight, src/energy/robust.cpp:30-58): kernel 0 none,
+ * 1 huber, 2 geman_mcclure. */
+int nricp_hip_robust_weights(int32_t kernel, const double* squared_residuals, int64_t count,
+                             double threshold, double* out_weights);
+
+/* Farthest-point seeds, first maximum wins (src/core/sampling.cpp:44-82). */
+int nricp_hip_farthest_point_seeds(const double* points, int64_t count, int32_t seeds,
+                                   int32_t first_index, int32_t* out_indices);
+
+/* Vertex binding to the nearest nodes, (1 - d/dmax)^2 weights
+ * (src/core/sampling.cpp:84-141). Outputs are (count, 4). */
+int nricp_hip_bind_vertices(const double* points, int64_t count, const double* nodes,
+                            int64_t node_count, int32_t max_nodes, int32_t* out_nodes,
+                            double* out_weights, int32_t* out_counts);
+
+/* kNN union of node pairs, sorted unique (min, max) (src/core/graph.cpp:11-39).
+ * out_edges has room for `capacity` pairs; *out_count receives the count. */
+int nricp_hip_build_node_edges(const double* nodes, int64_t node_count, int32_t neighbours,
+                               int32_t* out_edges, int64_t capacity, int64_t* out_count);
+
+/* v' = sum w (R (v - g) + g + t) (evaluate, src/core/deformation.cpp:54-84). */
+int nricp_hip_evaluate_deformation(const double* points, int64_t count, const int32_t* bind_nodes,
+                                   const double* bind_weights, const int32_t* bind_counts,
+                                   const double* rotations, const double* translations,
+                                   const double* node_positions, int64_t node_count, double* out);
+
+/* R <- exp([w]) R, t <- t + dt (retract, src/core/deformation.cpp:19-32). */
+int nricp_hip_retract(const double* rotations, const double* translations, const double* delta,
+                      int64_t node_count, double* out_rotations, double* out_translations);
+
+/* Level warm start (transfer_level, src/core/deformation.cpp:141-203). */
+int nricp_hip_transfer_level(const double* from_nodes, int64_t from_count, const double* rotations,
+                             const double* translations, const double* to_nodes, int64_t to_count,
+                             int32_t max_nodes, double* out_rotations, double* out_translations);
+
+/* Energy weights of one level (EnergyWeights, src/energy/energy.h:18-24). */
+typedef struct nricp_hip_weights {
+    double point_to_point, point_to_plane, smoothness, anchor, landmark;
+} nricp_hip_weights;
+
+/* Inputs of one energy / assembly evaluation (EnergyInputs,
+ * src/energy/energy.h:55-63, plus the deformation). accepted: nonzero = the
+ * correspondence survived the filters; mask: NULL or one byte per vertex. */
+typedef struct nricp_hip_energy_inputs {
+    const double* template_points;
+    int64_t vertex_count;
+    const int32_t* bind_nodes;
+    const double* bind_weights;
+    const int32_t* bind_counts;
+    const double* rotations;
+    const double* translations;
+    const double* node_positions;
+    int64_t node_count;
+    const double* correspondence_points;
+    const double* correspondence_normals;
+    const uint8_t* correspondence_accepted;
+    const double* correspondence_robust;
+    const int32_t* edges;
+    int64_t edge_count;
+    const int32_t* landmark_vertices;
+    const double* landmark_targets;
+    int64_t landmark_count;
+    const uint8_t* mask;
+    nricp_hip_weights weights;
+} nricp_hip_energy_inputs;
+
+/* Fused system and energies (assemble_and_energy, src/solver/fused_assembly.cpp:159-300;
+ * binding fused_system, src/python/bindings.cpp:1311-1353). Writes diagonal
+ * (N, 6, 6), rhs (N, 6) = -J^T r, the touched pairs (a < b, ascending) with
+ * their blocks J_a^T J_b, and energies[6] (five kinds, total). precision: 0
+ * FP64 blocks, 1 FP32 blocks (mixed mode). */
+int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precision,
+                           double* out_diagonal, double* out_rhs, int32_t* out_pairs,
+                           double* out_pair_blocks, int64_t capacity, int64_t* out_pair_count,
+                           double* out_energies);
+
+/* Damped solve of the block system (damped + solve_direct,
+ * src/solver/normal_equations.cpp:81-96, src/solver/direct_solve.cpp:10-64;
+ * binding solve_normal_equations, src/python/bindings.cpp:1355-1380), by
+ * block-Jacobi PCG to `tolerance` (relative residual). out_info[4] =
+ * iterations, relative residual, converged, breakdown. */
+int nricp_hip_solve_system(const double* diagonal, int64_t node_count, const int32_t* pairs,
+                           const double* pair_blocks, int64_t pair_count, const double* rhs,
+                           double damping, int32_t diagonal_scaling, int32_t precision,
+                           double tolerance, int32_t max_iterations, double* out_delta,
+                           double* out_info);
+
+/* ------------------------------------------------------- template graph ---- */
+
+/* Template graph on the device (build_template_graph, src/core/graph.cpp:41-73;
+ * build_graph_on_device, src/cuda/graph_build.cu:191-342). points are the
+ * template in its normalized frame. */
+typedef struct nricp_hip_graph nricp_hip_graph;
+int nricp_hip_graph_build(const double* points, int64_t vertex_count, int32_t levels,
+                          int32_t nodes_initial, int32_t growth, int32_t max_nodes_per_vertex,
+                          int32_t neighbours, nricp_hip_graph** out_graph);
+int nricp_hip_graph_level_count(const nricp_hip_graph* graph, int32_t* out_levels);
+int nricp_hip_graph_level_sizes(const nricp_hip_graph* graph, int32_t level, int64_t* out_nodes,
+                                int64_t* out_edges);
+/* Copies one level: node_indices (N), node_positions (N, 3), binding nodes /
+ * weights (V, 4), binding counts (V), edges (E, 2). Any output may be NULL. */
+int nricp_hip_graph_level_copy(const nricp_hip_graph* graph, int32_t level, int32_t* node_indices,
+                               double* node_positions, int32_t* bind_nodes, double* bind_weights,
+                               int32_t* bind_counts, int32_t* edges);
+void nricp_hip_graph_destroy(nricp_hip_graph* graph);
+
+/* ------------------------------------------------------------- session ---- */
+
+/* LevelSettings (src/solver/route_a.h:17-26, to_level_settings,
+ * src/python/bindings.cpp:181-196). Distances in the normalized frame. */
+typedef struct nricp_hip_level_settings {
+    nricp_hip_weights weights;
+    int32_t robust_kernel;
+    double robust_threshold;
+    int32_t max_outer_iterations;
+    int32_t max_inner_iterations;
+    double outer_tolerance;
+    double inner_tolerance;
+} nricp_hip_level_settings;
+
+#define NRICP_HIP_ROUTE_A 0  /* FP64 Gauss-Newton (route A semantics) */
+#define NRICP_HIP_ROUTE_B 1  /* device Gauss-Newton (route B) */
+#define NRICP_HIP_ROUTE_C 2  /* local/global: Horn fits + translation solve (route C) */
+#define NRICP_HIP_PRECISION_DOUBLE 0
+#define NRICP_HIP_PRECISION_MIXED 1
+
+/* RouteSettings + GraphSettings (src/solver/route_a.h:28-43, src/core/graph.h:11-16). */
+typedef struct nricp_hip_settings {
+    const nricp_hip_level_settings* levels;
+    int32_t level_settings_count;
+    double distance_threshold;
+    double min_normal_cosine;
+    double min_slope;
+    double initial_damping;
+    double damping_decrease;
+    double damping_increase;
+    int32_t max_damping_attempts;
+    int32_t diagonal_damping;
+    int32_t graph_levels;
+    int32_t nodes_initial;
+    int32_t growth;
+    int32_t max_nodes_per_vertex;
+    int32_t graph_neighbours;
+    double pcg_tolerance;
+    int32_t pcg_max_iterations;
+    int32_t precision;
+    int32_t route;
+} nricp_hip_settings;
+
+/* The wrap problem, both meshes already in the template graph's normalized
+ * frame (prepare_solve, src/python/bindings.cpp:584-672). */
+typedef struct nricp_hip_problem {
+    const double* template_points;
+    int64_t template_vertex_count;
+    const int32_t* template_faces;
+    int64_t template_face_count;
+    const double* target_points;
+    int64_t target_vertex_count;
+    const int32_t* target_faces;
+    int64_t target_face_count;
+    const uint8_t* mask;
+    const int32_t* landmark_vertices;
+    const double* landmark_targets;
+    int64_t landmark_count;
+} nricp_hip_problem;
+
+/* WrapProgress (src/cuda/gpu_solver.cuh:63-72). */
+typedef struct nricp_hip_progress {
+    int32_t level, outer, inner, finished, level_converged;
+    double energy;
+    double correspondence_mean;
+} nricp_hip_progress;
+
+/* RouteBSession (src/cuda/gpu_solver.cuh:86-126, session.cu:391-608) and the
+ * batch routes solve_route_a/b/c (src/solver/route_a.cpp:67, gpu_solver.cu:14-39).
+ * The session builds the template graph and the target hierarchy on the device
+ * (or reuses `graph` when given) and keeps every buffer resident until destroyed. */
+typedef struct nricp_hip_session nricp_hip_session;
+int nricp_hip_session_create(const nricp_hip_problem* problem, const nricp_hip_settings* settings,
+                             const nricp_hip_graph* graph, nricp_hip_session** out_session);
+int nricp_hip_session_step(nricp_hip_session* session, nricp_hip_progress* out_progress);
+/* Rewind to the start of the wrap keeping the uploaded inputs (no reference
+ * counterpart: lets a caller re-run a resident job, e.g. to benchmark it). */
+int nricp_hip_session_reset(nricp_hip_session* session);
+/* Work counters since creation, in this order: PCG launches, PCG iterations,
+ * PCG algorithmic bytes, assembly launches, assembly bytes, energy launches,
+ * energy bytes, query launches, query rows, Gauss-Newton solves. Returns at
+ * most `capacity` values; *out_count receives the number available. */
+int nricp_hip_session_counters(const nricp_hip_session* session, double* out, int32_t capacity,
+                               int32_t* out_count);
+int nricp_hip_session_run(nricp_hip_session* session);
+int nricp_hip_session_finished(const nricp_hip_session* session, int32_t* out_finished);
+/* Fitted template in the normalized frame (V, 3). */
+int nricp_hip_session_vertices(nricp_hip_session* session, double* out_vertices);
+/* Node count of the level whose state the session holds, and the level count. */
+int nricp_hip_session_sizes(const nricp_hip_session* session, int64_t* out_nodes, int32_t* out_levels);
+int nricp_hip_session_level_nodes(const nricp_hip_session* session, int64_t* out_counts);
+/* Current deformation: rotations (N, 3, 3), translations (N, 3), rest positions (N, 3). */
+int nricp_hip_session_state(nricp_hip_session* session, double* out_rotations,
+                            double* out_translations, double* out_node_positions);
+/* Run report as JSON (to_json, src/core/report.cpp:46-105). */
+int nricp_hip_session_report(nricp_hip_session* session, char* buffer, size_t capacity,
+                             size_t* out_length);
+/* Energy at the start of every inner iteration so far (decision trace). */
+int nricp_hip_session_trace(const nricp_hip_session* session, double* out, int64_t capacity,
+                            int64_t* out_count);
+/* Device-time accounting with HIP events on the session's stream (off by
+ * default): enable, then read the total milliseconds and launch count of a
+ * stage -- "closest_points", "assemble", "pcg", "energy", "layout". */
+int nricp_hip_session_profile(nricp_hip_session* session, int32_t enable);
+int nricp_hip_session_kernel_time(const nricp_hip_session* session, const char* stage, double* total_ms,
+                                  int64_t* launches);
+void nricp_hip_session_destroy(nricp_hip_session* session);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NRICP_HIP_H */

@@ -1,0 +1,474 @@
+// Block Gauss-Newton assembly of the 6-DOF-per-node normal equations and the
+// five block energies, atomics-free.
+//
+// Reference behaviour: the residual rows and Jacobians of
+// src/energy/energy.cpp:124-271 accumulated as in
+// src/solver/fused_assembly.cpp:159-300 (J^T J per node, -J^T r, J_a^T J_b per
+// interacting pair). The per-row sums are taken in closed form: for a row
+// direction e_k with arm a = R (v - g) and scale c the rows of a 3-row offset
+// block give
+//     sum_k J_k J_k^T = c^2 [ |a|^2 I - a a^T   [a]x ]
+//                           [ [a]x^T             I   ]
+//     -sum_k J_k r_k  = -c [ a x r ; r ]
+// which is the same quantity with a fraction of the arithmetic.
+//
+// Passes: (1) one thread per vertex resolves its rows (deformed point,
+// residuals, per-slot arms, scales) and their energies; (2) one thread per
+// edge/node adds smoothness and anchor energies; (3) one wave per node gathers
+// its diagonal block and right-hand side from its vertex rows, edges and
+// anchor; (4) one thread per interacting pair gathers the off-diagonal block.
+// Energies are reduced in a fixed order, so the energy of a state is the same
+// bits whether it comes from an assembly or an energy-only pass.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace nricp_hip {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline d3 apply_node(const double* R, const double* T, const double* G, int64_t j, d3 p) {
+    const d3 g = ld3(G, j);
+    return (mul3(R + 9 * j, p - g) + g) + ld3(T, j);
+}
+
+// log through the (w >= 0) unit quaternion, as src/core/rotation.h:50-62.
+__device__ inline d3 log_so3(const double* r) {
+    double w, x, y, z;
+    const double tr = r[0] + r[4] + r[8];
+    if (tr > 0.0) {
+        const double s = sqrt(tr + 1.0) * 2.0;
+        w = 0.25 * s; x = (r[7] - r[5]) / s; y = (r[2] - r[6]) / s; z = (r[3] - r[1]) / s;
+    } else if (r[0] > r[4] && r[0] > r[8]) {
+        const double s = sqrt(1.0 + r[0] - r[4] - r[8]) * 2.0;
+        w = (r[7] - r[5]) / s; x = 0.25 * s; y = (r[1] + r[3]) / s; z = (r[2] + r[6]) / s;
+    } else if (r[4] > r[8]) {
+        const double s = sqrt(1.0 + r[4] - r[0] - r[8]) * 2.0;
+        w = (r[2] - r[6]) / s; x = (r[1] + r[3]) / s; y = 0.25 * s; z = (r[5] + r[7]) / s;
+    } else {
+        const double s = sqrt(1.0 + r[8] - r[0] - r[4]) * 2.0;
+        w = (r[3] - r[1]) / s; x = (r[2] + r[6]) / s; y = (r[5] + r[7]) / s; z = 0.25 * s;
+    }
+    const double l = sqrt(w * w + x * x + y * y + z * z);
+    if (l > 0.0) {
+        w /= l; x /= l; y /= l; z /= l;
+    } else {
+        w = 1.0; x = y = z = 0.0;
+    }
+    if (w < 0.0) {
+        w = -w; x = -x; y = -y; z = -z;
+    }
+    const double len = sqrt(x * x + y * y + z * z);
+    if (len < 1e-12) return d3{0.0, 0.0, 0.0};
+    const double angle = 2.0 * atan2(len, w);
+    return d3{x, y, z} * (angle / len);
+}
+
+// J_l^{-1}(w) = I - [w]/2 + c [w]^2 (src/core/rotation.h:69-103).
+__device__ inline void left_jacobian_inverse(d3 w, double* J) {
+    const double t2 = dot(w, w);
+    double c;
+    if (t2 < 1e-12) {
+        c = 1.0 / 12.0;
+    } else {
+        const double t = sqrt(t2);
+        c = t > 3.14159265358979323846 - 1e-6 ? 0.0 : 1.0 / t2 - (1.0 + cos(t)) / (2.0 * t * sin(t));
+    }
+    const double k[9] = {0.0, -w.z, w.y, w.z, 0.0, -w.x, -w.y, w.x, 0.0};
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 3; ++col) {
+            double k2 = 0.0;
+            for (int m = 0; m < 3; ++m) k2 += k[r * 3 + m] * k[m * 3 + col];
+            J[r * 3 + col] = (r == col ? 1.0 : 0.0) - 0.5 * k[r * 3 + col] + c * k2;
+        }
+}
+
+// ------------------------------------------------------------ pass 1 -------
+
+template <bool Records>
+__global__ void __launch_bounds__(kThreads) vertex_rows(AssemblyInputs in, double* __restrict__ scale,
+                                                        double* __restrict__ resid, double* __restrict__ arm,
+                                                        double* __restrict__ partials) {
+    __shared__ double lds[3 * (kThreads / kWave)];
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    double e[3] = {0.0, 0.0, 0.0};
+    if (v < in.vertex_count) {
+        const d3 x = ld3(in.points, v);
+        d3 def{0.0, 0.0, 0.0};
+        d3 arms[4];
+        bool use[4];
+        int usable = 0;
+        const int count = in.bind_counts[v];
+        for (int s = 0; s < 4; ++s) {
+            const int32_t j = in.bind_nodes[4 * v + s];
+            const double w = in.bind_weights[4 * v + s];
+            use[s] = s < count && j >= 0 && j < in.node_count && w != 0.0;
+            arms[s] = d3{0.0, 0.0, 0.0};
+            if (!use[s]) continue;
+            ++usable;
+            const d3 g = ld3(in.rest, j);
+            arms[s] = mul3(in.rotations + 9 * static_cast<int64_t>(j), x - g);
+            def = def + ((arms[s] + g) + ld3(in.translations, j)) * w;
+        }
+        double sp = -1.0, sl = -1.0, sm = -1.0;
+        double r[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (usable > 0) {
+            if (in.active ? in.active[v] != 0 : in.rejection[v] == 0) {
+                const double u = fmax(0.0, in.robust[v]);
+                const d3 y = ld3(in.corr_points, v);
+                if (in.w.p2p > 0.0) {
+                    sp = sqrt(in.w.p2p * u);
+                    const d3 off = (def - y) * sp;
+                    r[0] = off.x; r[1] = off.y; r[2] = off.z;
+                    e[0] = (off.x * off.x + off.y * off.y) + off.z * off.z;
+                }
+                if (in.w.plane > 0.0) {
+                    sl = sqrt(in.w.plane * u);
+                    r[3] = sl * dot(ld3(in.corr_normals, v), def - y);
+                    e[1] = r[3] * r[3];
+                }
+            }
+            if (in.landmark_flag[v] && in.w.landmark > 0.0) {
+                sm = sqrt(in.w.landmark);
+                const d3 off = (def - ld3(in.landmark_target, v)) * sm;
+                r[4] = off.x; r[5] = off.y; r[6] = off.z;
+                e[2] = (off.x * off.x + off.y * off.y) + off.z * off.z;
+            }
+        }
+        if (Records) {
+            scale[3 * v] = sp;
+            scale[3 * v + 1] = sl;
+            scale[3 * v + 2] = sm;
+            for (int k = 0; k < 7; ++k) resid[7 * v + k] = r[k];
+            for (int s = 0; s < 4; ++s) st3(arm, 4 * v + s, arms[s]);
+        }
+    }
+    double out[3];
+    block_sum<3, kThreads>(e, lds, out);
+    if (threadIdx.x == 0) {
+        double* p = partials + 5 * static_cast<int64_t>(blockIdx.x);
+        p[0] = out[0];
+        p[1] = out[1];
+        p[2] = 0.0;
+        p[3] = 0.0;
+        p[4] = out[2];
+    }
+}
+
+// ------------------------------------------------------------ pass 2 -------
+
+__global__ void __launch_bounds__(kThreads) graph_energy(AssemblyInputs in, double* __restrict__ partials) {
+    __shared__ double lds[2 * (kThreads / kWave)];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    double e[2] = {0.0, 0.0};
+    if (i < in.edge_count && in.w.smooth > 0.0) {
+        const int64_t a = in.edges[2 * i], b = in.edges[2 * i + 1];
+        if (a >= 0 && b >= 0 && a < in.node_count && b < in.node_count) {
+            const double sc = sqrt(in.w.smooth);
+            const d3 gb = ld3(in.rest, b);
+            const d3 diff = (apply_node(in.rotations, in.translations, in.rest, a, gb) -
+                             apply_node(in.rotations, in.translations, in.rest, b, gb)) * sc;
+            e[0] = (diff.x * diff.x + diff.y * diff.y) + diff.z * diff.z;
+        }
+    }
+    if (i < in.node_count && in.w.anchor > 0.0) {
+        const double sc = sqrt(in.w.anchor);
+        const d3 w = log_so3(in.rotations + 9 * i) * sc;
+        const d3 t = ld3(in.translations, i) * sc;
+        e[1] = (((((w.x * w.x + w.y * w.y) + w.z * w.z) + t.x * t.x) + t.y * t.y) + t.z * t.z);
+    }
+    double out[2];
+    block_sum<2, kThreads>(e, lds, out);
+    if (threadIdx.x == 0) {
+        double* p = partials + 5 * static_cast<int64_t>(blockIdx.x);
+        p[0] = 0.0;
+        p[1] = 0.0;
+        p[2] = out[0];
+        p[3] = out[1];
+        p[4] = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) reduce_partials(const double* __restrict__ partials, int blocks,
+                                                            double* __restrict__ out) {
+    __shared__ double lds[5 * (kThreads / kWave)];
+    double e[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < blocks; b += kThreads)
+        for (int k = 0; k < 5; ++k) e[k] += partials[5 * b + k];
+    double s[5];
+    block_sum<5, kThreads>(e, lds, s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) out[k] = s[k];
+}
+
+// ------------------------------------------------------------ pass 3 -------
+
+// Upper triangle of a symmetric 6x6 in row order, plus 6 rhs entries.
+__device__ constexpr int U(int r, int c) { return r * 6 - (r * (r - 1)) / 2 + (c - r); }
+constexpr int kUpper = 21;
+
+__device__ inline void add_offset_rows(double* acc, double c, d3 a, const double* r) {
+    const double c2 = c * c;
+    const double aa = dot(a, a);
+    const double av[3] = {a.x, a.y, a.z};
+    for (int x = 0; x < 3; ++x)
+        for (int y = x; y < 3; ++y) acc[U(x, y)] += c2 * ((x == y ? aa : 0.0) - av[x] * av[y]);
+    // [a]x = [[0,-az,ay],[az,0,-ax],[-ay,ax,0]]
+    acc[U(0, 4)] += c2 * -a.z;
+    acc[U(0, 5)] += c2 * a.y;
+    acc[U(1, 3)] += c2 * a.z;
+    acc[U(1, 5)] += c2 * -a.x;
+    acc[U(2, 3)] += c2 * -a.y;
+    acc[U(2, 4)] += c2 * a.x;
+    acc[U(3, 3)] += c2;
+    acc[U(4, 4)] += c2;
+    acc[U(5, 5)] += c2;
+    const d3 rv{r[0], r[1], r[2]};
+    const d3 m = cross(a, rv);
+    acc[kUpper + 0] -= c * m.x;
+    acc[kUpper + 1] -= c * m.y;
+    acc[kUpper + 2] -= c * m.z;
+    acc[kUpper + 3] -= c * rv.x;
+    acc[kUpper + 4] -= c * rv.y;
+    acc[kUpper + 5] -= c * rv.z;
+}
+
+__device__ inline void add_plane_row(double* acc, double c, d3 a, d3 n, double value) {
+    const d3 an = cross(a, n);
+    const double q[6] = {c * an.x, c * an.y, c * an.z, c * n.x, c * n.y, c * n.z};
+    for (int x = 0; x < 6; ++x) {
+        for (int y = x; y < 6; ++y) acc[U(x, y)] += q[x] * q[y];
+        acc[kUpper + x] -= q[x] * value;
+    }
+}
+
+template <typename Scalar>
+__global__ void __launch_bounds__(kThreads)
+node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
+            const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+            const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
+            Scalar* __restrict__ bsr, double* __restrict__ rhs) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t node = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (node >= in.node_count) return;
+    double acc[kUpper + 6];
+    for (int k = 0; k < kUpper + 6; ++k) acc[k] = 0.0;
+    for (int32_t k = goff[node] + lane; k < goff[node + 1]; k += kWave) {
+        const int32_t ent = gent[k];
+        const int64_t v = ent >> 2;
+        const int s = ent & 3;
+        const double w = in.bind_weights[4 * v + s];
+        const d3 a = ld3(arm, 4 * v + s);
+        const double sp = scale[3 * v], sl = scale[3 * v + 1], sm = scale[3 * v + 2];
+        if (sp >= 0.0) add_offset_rows(acc, sp * w, a, resid + 7 * v);
+        if (sl >= 0.0) add_plane_row(acc, sl * w, a, ld3(in.corr_normals, v), resid[7 * v + 3]);
+        if (sm >= 0.0) add_offset_rows(acc, sm * w, a, resid + 7 * v + 4);
+    }
+    if (in.w.smooth > 0.0) {
+        const double sc = sqrt(in.w.smooth);
+        for (int32_t k = eoff[node] + lane; k < eoff[node + 1]; k += kWave) {
+            const int32_t ent = eent[k];
+            const int64_t e = ent >> 1;
+            const int64_t a = in.edges[2 * e], b = in.edges[2 * e + 1];
+            if (a < 0 || b < 0 || a >= in.node_count || b >= in.node_count) continue;
+            const d3 ga = ld3(in.rest, a), gb = ld3(in.rest, b);
+            const d3 diff = (apply_node(in.rotations, in.translations, in.rest, a, gb) -
+                             apply_node(in.rotations, in.translations, in.rest, b, gb)) * sc;
+            const double r[3] = {diff.x, diff.y, diff.z};
+            if ((ent & 1) == 0) {
+                add_offset_rows(acc, sc, mul3(in.rotations + 9 * a, gb - ga), r);
+            } else {
+                const double s2 = sc * sc;
+                acc[U(3, 3)] += s2;
+                acc[U(4, 4)] += s2;
+                acc[U(5, 5)] += s2;
+                acc[kUpper + 3] += sc * r[0];
+                acc[kUpper + 4] += sc * r[1];
+                acc[kUpper + 5] += sc * r[2];
+            }
+        }
+    }
+    if (lane == 0 && in.w.anchor > 0.0) {
+        const double sc2 = in.w.anchor;
+        const double* R = in.rotations + 9 * node;
+        const d3 w = log_so3(R);
+        double J[9];
+        left_jacobian_inverse(w, J);
+        const double wv[3] = {w.x, w.y, w.z};
+        for (int x = 0; x < 3; ++x) {
+            for (int y = x; y < 3; ++y) {
+                double s = 0.0;
+                for (int r = 0; r < 3; ++r) s += J[r * 3 + x] * J[r * 3 + y];
+                acc[U(x, y)] += sc2 * s;
+            }
+            double g = 0.0;
+            for (int r = 0; r < 3; ++r) g += J[r * 3 + x] * wv[r];
+            acc[kUpper + x] -= sc2 * g;
+        }
+        const d3 t = ld3(in.translations, node);
+        acc[U(3, 3)] += sc2;
+        acc[U(4, 4)] += sc2;
+        acc[U(5, 5)] += sc2;
+        acc[kUpper + 3] -= sc2 * t.x;
+        acc[kUpper + 4] -= sc2 * t.y;
+        acc[kUpper + 5] -= sc2 * t.z;
+    }
+    for (int k = 0; k < kUpper + 6; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+        Scalar* out = bsr + 36 * static_cast<int64_t>(diag_pos[node]);
+        for (int x = 0; x < 6; ++x)
+            for (int y = 0; y < 6; ++y) out[6 * x + y] = static_cast<Scalar>(x <= y ? acc[U(x, y)] : acc[U(y, x)]);
+        for (int x = 0; x < 6; ++x) rhs[6 * node + x] = acc[kUpper + x];
+    }
+}
+
+// ------------------------------------------------------------ pass 4 -------
+
+__device__ inline void add_pair_offset(double* acc, double clo, d3 alo, double chi, d3 ahi) {
+    const double cc = clo * chi;
+    const double lo[3] = {alo.x, alo.y, alo.z}, hi[3] = {ahi.x, ahi.y, ahi.z};
+    const double d = dot(alo, ahi);
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) acc[6 * x + y] += cc * ((x == y ? d : 0.0) - hi[x] * lo[y]);
+    // top right [alo]x, bottom left [ahi]x^T
+    acc[6 * 0 + 4] += cc * -alo.z;
+    acc[6 * 0 + 5] += cc * alo.y;
+    acc[6 * 1 + 3] += cc * alo.z;
+    acc[6 * 1 + 5] += cc * -alo.x;
+    acc[6 * 2 + 3] += cc * -alo.y;
+    acc[6 * 2 + 4] += cc * alo.x;
+    acc[6 * 4 + 0] += cc * -ahi.z;
+    acc[6 * 5 + 0] += cc * ahi.y;
+    acc[6 * 3 + 1] += cc * ahi.z;
+    acc[6 * 5 + 1] += cc * -ahi.x;
+    acc[6 * 3 + 2] += cc * -ahi.y;
+    acc[6 * 4 + 2] += cc * ahi.x;
+    acc[6 * 3 + 3] += cc;
+    acc[6 * 4 + 4] += cc;
+    acc[6 * 5 + 5] += cc;
+}
+
+template <typename Scalar>
+__global__ void __launch_bounds__(kThreads)
+pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
+            const double* __restrict__ arm, int64_t groups, const int32_t* __restrict__ goff,
+            const int32_t* __restrict__ gent, const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
+            int32_t* __restrict__ touched) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (g >= groups) return;
+    double acc[36];
+    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+    int hit = 0;
+    for (int32_t k = goff[g]; k < goff[g + 1]; ++k) {
+        const int32_t ent = gent[k];
+        if (ent >= 0) {
+            const int64_t v = ent >> 4;
+            const int slo = (ent >> 2) & 3, shi = ent & 3;
+            const double wlo = in.bind_weights[4 * v + slo], whi = in.bind_weights[4 * v + shi];
+            const d3 alo = ld3(arm, 4 * v + slo), ahi = ld3(arm, 4 * v + shi);
+            const double sp = scale[3 * v], sl = scale[3 * v + 1], sm = scale[3 * v + 2];
+            if (sp >= 0.0) {
+                hit = 1;
+                add_pair_offset(acc, sp * wlo, alo, sp * whi, ahi);
+            }
+            if (sl >= 0.0) {
+                hit = 1;
+                const d3 n = ld3(in.corr_normals, v);
+                const d3 nlo = cross(alo, n), nhi = cross(ahi, n);
+                const double cl = sl * wlo, ch = sl * whi;
+                const double ql[6] = {cl * nlo.x, cl * nlo.y, cl * nlo.z, cl * n.x, cl * n.y, cl * n.z};
+                const double qh[6] = {ch * nhi.x, ch * nhi.y, ch * nhi.z, ch * n.x, ch * n.y, ch * n.z};
+                for (int x = 0; x < 6; ++x)
+                    for (int y = 0; y < 6; ++y) acc[6 * x + y] += ql[x] * qh[y];
+            }
+            if (sm >= 0.0) {
+                hit = 1;
+                add_pair_offset(acc, sm * wlo, alo, sm * whi, ahi);
+            }
+        } else if (in.w.smooth > 0.0) {
+            const int64_t e = ~ent;
+            const int64_t a = in.edges[2 * e], b = in.edges[2 * e + 1];
+            hit = 1;
+            const double s2 = in.w.smooth;
+            const d3 rot = mul3(in.rotations + 9 * a, ld3(in.rest, b) - ld3(in.rest, a));
+            // M = J_a J_b^T: columns 3..5 hold -s^2 [ [rot]x ; I ].
+            double m[36];
+            for (int k = 0; k < 36; ++k) m[k] = 0.0;
+            m[6 * 0 + 4] = -s2 * -rot.z;
+            m[6 * 0 + 5] = -s2 * rot.y;
+            m[6 * 1 + 3] = -s2 * rot.z;
+            m[6 * 1 + 5] = -s2 * -rot.x;
+            m[6 * 2 + 3] = -s2 * -rot.y;
+            m[6 * 2 + 4] = -s2 * rot.x;
+            m[6 * 3 + 3] = -s2;
+            m[6 * 4 + 4] = -s2;
+            m[6 * 5 + 5] = -s2;
+            if (a < b) {
+                for (int k = 0; k < 36; ++k) acc[k] += m[k];
+            } else {
+                for (int x = 0; x < 6; ++x)
+                    for (int y = 0; y < 6; ++y) acc[6 * x + y] += m[6 * y + x];
+            }
+        }
+    }
+    Scalar* ab = bsr + 36 * static_cast<int64_t>(pos[2 * g]);
+    Scalar* ba = bsr + 36 * static_cast<int64_t>(pos[2 * g + 1]);
+    for (int x = 0; x < 6; ++x)
+        for (int y = 0; y < 6; ++y) {
+            ab[6 * x + y] = static_cast<Scalar>(acc[6 * x + y]);
+            ba[6 * y + x] = static_cast<Scalar>(acc[6 * x + y]);
+        }
+    if (touched) touched[g] = hit;
+}
+
+void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream) {
+    const int vblocks = std::max(1, blocks_for(in.vertex_count, kThreads));
+    const int gblocks = std::max(1, blocks_for(std::max(in.edge_count, in.node_count), kThreads));
+    s.partials.resize(5 * static_cast<int64_t>(vblocks + gblocks));
+    if (records) {
+        s.vertex_scale.resize(3 * std::max<int64_t>(in.vertex_count, 1));
+        s.vertex_resid.resize(7 * std::max<int64_t>(in.vertex_count, 1));
+        s.vertex_arm.resize(12 * std::max<int64_t>(in.vertex_count, 1));
+        vertex_rows<true><<<vblocks, kThreads, 0, stream>>>(in, s.vertex_scale.get(), s.vertex_resid.get(),
+                                                             s.vertex_arm.get(), s.partials.get());
+    } else {
+        vertex_rows<false><<<vblocks, kThreads, 0, stream>>>(in, nullptr, nullptr, nullptr, s.partials.get());
+    }
+    NRICP_LAUNCH_CHECK("vertex_rows");
+    graph_energy<<<gblocks, kThreads, 0, stream>>>(in, s.partials.get() + 5 * static_cast<int64_t>(vblocks));
+    NRICP_LAUNCH_CHECK("graph_energy");
+    reduce_partials<<<1, kThreads, 0, stream>>>(s.partials.get(), vblocks + gblocks, out);
+    NRICP_LAUNCH_CHECK("reduce_partials");
+}
+
+}  // namespace
+
+void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream) {
+    launch_energies(in, s, false, out, stream);
+}
+
+template <typename Scalar>
+void assemble(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s, Scalar* bsr, double* rhs,
+              double* energies_out, hipStream_t stream) {
+    launch_energies(in, s, true, energies_out, stream);
+    const int per_block = kThreads / kWave;
+    node_blocks<Scalar><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+        in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
+        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), bsr, rhs);
+    NRICP_LAUNCH_CHECK("node_blocks");
+    if (L.group_count > 0) {
+        s.touched.resize(L.group_count);
+        pair_blocks<Scalar><<<blocks_for(L.group_count, kThreads), kThreads, 0, stream>>>(
+            in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.group_count, L.group_offsets.get(),
+            L.group_entries.get(), L.pair_position.get(), bsr, s.touched.get());
+        NRICP_LAUNCH_CHECK("pair_blocks");
+    }
+}
+
+template void assemble<double>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, double*, double*, double*,
+                               hipStream_t);
+template void assemble<float>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, float*, double*, double*,
+                              hipStream_t);
+
+}  // namespace nricp_hip

@@ -1,0 +1,645 @@
+// Target-side kernels: vertex normals, the LBVH over the target triangles, and
+// the closest-point query with the correspondence filters and IRLS weights.
+//
+// Reference behaviour: closest point and filters follow
+// src/geometry/closest_point.cpp:24-279 and src/solver/correspondence.cpp:18-96
+// (FP64, operation order kept so distances match the CPU route bit for bit);
+// the hierarchy replaces src/cuda/lbvh.cu with a 64-bit-key Karras tree whose
+// traversal is stackless (parent links), so no subtree can be dropped.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "kernels.hpp"
+
+namespace nricp_hip {
+namespace {
+
+// --------------------------------------------------------------- normals ---
+
+__global__ void face_cross_kernel(const double* __restrict__ p, const int32_t* __restrict__ f, int64_t nf,
+                                  double* __restrict__ w) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const d3 a = ld3(p, f[3 * t]), b = ld3(p, f[3 * t + 1]), c = ld3(p, f[3 * t + 2]);
+    st3(w, t, cross(b - a, c - a));
+}
+
+// Sum in increasing face order starting from zero, then normalise: the same
+// sequence of additions the CPU route performs per vertex.
+__global__ void gather_normals_kernel(const double* __restrict__ w, const int32_t* __restrict__ off,
+                                      const int32_t* __restrict__ faces, int64_t nv, double* __restrict__ out) {
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    d3 s{0.0, 0.0, 0.0};
+    for (int32_t k = off[v]; k < off[v + 1]; ++k) s = s + ld3(w, faces[k]);
+    st3(out, v, unit(s));
+}
+
+__global__ void vertex_face_entries(const int32_t* __restrict__ f, int64_t nf, uint64_t* __restrict__ keys,
+                                    uint32_t* __restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 3 * nf) return;
+    keys[i] = static_cast<uint64_t>(static_cast<uint32_t>(f[i]));
+    vals[i] = static_cast<uint32_t>(i / 3);
+}
+
+__global__ void copy_u32_to_i32(const uint32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<int32_t>(in[i]);
+}
+
+// ------------------------------------------------------------------ LBVH ---
+
+__global__ void centroid_kernel(const double* __restrict__ p, const int32_t* __restrict__ f, int64_t nt,
+                                double* __restrict__ c) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const d3 a = ld3(p, f[3 * t]), b = ld3(p, f[3 * t + 1]), cc = ld3(p, f[3 * t + 2]);
+    st3(c, t, (a + b + cc) / 3.0);
+}
+
+// Two-stage min/max of the centroids: per-block partials, then one block.
+__global__ void __launch_bounds__(256) bounds_partial(const double* __restrict__ c, int64_t n, double* __restrict__ part) {
+    __shared__ double lo[3][256], hi[3][256];
+    double l[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, h[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fmin(l[k], c[3 * i + k]);
+            h[k] = fmax(h[k], c[3 * i + k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        lo[k][threadIdx.x] = l[k];
+        hi[k][threadIdx.x] = h[k];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                lo[k][threadIdx.x] = fmin(lo[k][threadIdx.x], lo[k][threadIdx.x + s]);
+                hi[k][threadIdx.x] = fmax(hi[k][threadIdx.x], hi[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+            part[6 * blockIdx.x + k] = lo[k][0];
+            part[6 * blockIdx.x + 3 + k] = hi[k][0];
+        }
+}
+
+__global__ void bounds_final(double* part, int blocks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int b = 1; b < blocks; ++b)
+        for (int k = 0; k < 3; ++k) {
+            part[k] = fmin(part[k], part[6 * b + k]);
+            part[3 + k] = fmax(part[3 + k], part[6 * b + 3 + k]);
+        }
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void morton_kernel(const double* __restrict__ c, int64_t nt, const double* __restrict__ box,
+                              int index_bits, uint64_t* __restrict__ keys) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        const double extent = box[3 + k] - box[k];
+        const double u = extent > 0.0 ? (c[3 * t + k] - box[k]) / extent : 0.0;
+        double s = u * 1024.0;
+        s = s < 0.0 ? 0.0 : (s > 1023.0 ? 1023.0 : s);
+        q[k] = static_cast<uint32_t>(s);
+    }
+    const uint64_t code = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    keys[t] = (code << index_bits) | static_cast<uint64_t>(t);
+}
+
+__global__ void leaf_kernel(const uint64_t* __restrict__ keys, int64_t nt, uint64_t index_mask,
+                            const double* __restrict__ p, const int32_t* __restrict__ f,
+                            int32_t* __restrict__ leaf_triangle, double* __restrict__ corners) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int32_t t = static_cast<int32_t>(keys[i] & index_mask);
+    leaf_triangle[i] = t;
+    for (int k = 0; k < 3; ++k) {
+        const d3 v = ld3(p, f[3 * t + k]);
+        corners[9 * i + 3 * k] = v.x;
+        corners[9 * i + 3 * k + 1] = v.y;
+        corners[9 * i + 3 * k + 2] = v.z;
+    }
+}
+
+__device__ inline int delta_keys(const uint64_t* keys, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(static_cast<long long>(keys[i] ^ keys[j]));
+}
+
+// Karras (2012), "Maximizing parallelism in the construction of BVHs": the
+// keys are unique (triangle index in the low bits), so the range and split of
+// internal node i follow from common-prefix lengths alone.
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, int64_t n, BvhNode* __restrict__ nodes,
+                              int32_t* __restrict__ leaf_parent) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta_keys(keys, n, i, i + 1) - delta_keys(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta_keys(keys, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta_keys(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta_keys(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta_keys(keys, n, i, j);
+    int64_t s = 0;
+    int64_t t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta_keys(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j;
+    const int64_t hi = i < j ? j : i;
+    int32_t left, right;
+    if (lo == gamma) {
+        left = ~static_cast<int32_t>(gamma);
+        leaf_parent[gamma] = static_cast<int32_t>(i);
+    } else {
+        left = static_cast<int32_t>(gamma);
+        nodes[gamma].parent = static_cast<int32_t>(i);
+    }
+    if (hi == gamma + 1) {
+        right = ~static_cast<int32_t>(gamma + 1);
+        leaf_parent[gamma + 1] = static_cast<int32_t>(i);
+    } else {
+        right = static_cast<int32_t>(gamma + 1);
+        nodes[gamma + 1].parent = static_cast<int32_t>(i);
+    }
+    nodes[i].child[0] = left;
+    nodes[i].child[1] = right;
+    if (i == 0) nodes[0].parent = -1;
+}
+
+// Bottom-up refit: a leaf writes its triangle box into its parent's slot;
+// the second child to arrive at a node merges both slots and carries the
+// union one level up. Release/acquire at agent scope orders the slot writes
+// against the arrival counter across compute dies.
+__global__ void refit_kernel(const double* __restrict__ corners, int64_t n, const int32_t* __restrict__ leaf_parent,
+                             BvhNode* nodes, int32_t* arrivals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        const double a = corners[9 * i + k], b = corners[9 * i + 3 + k], c = corners[9 * i + 6 + k];
+        lo[k] = fmin(fmin(a, b), c);
+        hi[k] = fmax(fmax(a, b), c);
+    }
+    int32_t child = ~static_cast<int32_t>(i);
+    int32_t node = leaf_parent[i];
+    while (node >= 0) {
+        BvhNode* nd = nodes + node;
+        const int side = nd->child[0] == child ? 0 : 1;
+        for (int k = 0; k < 3; ++k) {
+            nd->box[6 * side + k] = lo[k];
+            nd->box[6 * side + 3 + k] = hi[k];
+        }
+        const int32_t prior =
+            __hip_atomic_fetch_add(arrivals + node, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0) return;
+        const int other = 1 - side;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], __hip_atomic_load(&nd->box[6 * other + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            hi[k] = fmax(hi[k], __hip_atomic_load(&nd->box[6 * other + 3 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+        child = node;
+        node = nd->parent;
+    }
+}
+
+// ------------------------------------------------------------- query -------
+
+// Ericson's Voronoi-region closest point with the CPU route's degenerate
+// fallbacks; every operation in the CPU route's order.
+__device__ inline d3 closest_on_triangle(d3 p, d3 a, d3 b, d3 c, double* bary) {
+    const d3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        bary[0] = 1.0; bary[1] = 0.0; bary[2] = 0.0;
+        return a;
+    }
+    const d3 bp = p - b;
+    const double d3v = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3v >= 0.0 && d4 <= d3v) {
+        bary[0] = 0.0; bary[1] = 1.0; bary[2] = 0.0;
+        return b;
+    }
+    const double vc = d1 * d4 - d3v * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3v <= 0.0) {
+        const double den = d1 - d3v;
+        const double v = den != 0.0 ? d1 / den : 0.0;
+        bary[0] = 1.0 - v; bary[1] = v; bary[2] = 0.0;
+        return a + ab * v;
+    }
+    const d3 cp = p - c;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
+        bary[0] = 0.0; bary[1] = 0.0; bary[2] = 1.0;
+        return c;
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double den = d2 - d6;
+        const double w = den != 0.0 ? d2 / den : 0.0;
+        bary[0] = 1.0 - w; bary[1] = 0.0; bary[2] = w;
+        return a + ac * w;
+    }
+    const double va = d3v * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3v) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double den = (d4 - d3v) + (d5 - d6);
+        const double w = den != 0.0 ? (d4 - d3v) / den : 0.0;
+        bary[0] = 0.0; bary[1] = 1.0 - w; bary[2] = w;
+        return b + (c - b) * w;
+    }
+    const double den = va + vb + vc;
+    if (fabs(den) < 1e-300) {
+        bary[0] = 1.0; bary[1] = 0.0; bary[2] = 0.0;
+        return a;
+    }
+    const double v = vb / den, w = vc / den;
+    bary[0] = 1.0 - v - w; bary[1] = v; bary[2] = w;
+    return a + ab * v + ac * w;
+}
+
+__device__ inline double box_d2(d3 p, const double* b) {
+    const double cx = fmin(fmax(p.x, b[0]), b[3]);
+    const double cy = fmin(fmax(p.y, b[1]), b[4]);
+    const double cz = fmin(fmax(p.z, b[2]), b[5]);
+    const d3 d{p.x - cx, p.y - cy, p.z - cz};
+    return norm2(d);
+}
+
+struct Best {
+    double d2;
+    int32_t leaf;
+    int32_t triangle;
+    d3 point;
+    double bary[3];
+};
+
+__device__ inline void test_leaf(int32_t leaf, d3 p, const double* __restrict__ corners,
+                                 const int32_t* __restrict__ leaf_triangle, Best& best) {
+    const double* c = corners + 9 * static_cast<int64_t>(leaf);
+    const d3 a{c[0], c[1], c[2]}, b{c[3], c[4], c[5]}, cc{c[6], c[7], c[8]};
+    double bary[3];
+    const d3 q = closest_on_triangle(p, a, b, cc, bary);
+    const double d2 = norm2(q - p);
+    const int32_t tri = leaf_triangle[leaf];
+    if (d2 < best.d2 || (d2 == best.d2 && tri < best.triangle)) {
+        best.d2 = d2;
+        best.leaf = leaf;
+        best.triangle = tri;
+        best.point = q;
+        best.bary[0] = bary[0];
+        best.bary[1] = bary[1];
+        best.bary[2] = bary[2];
+    }
+}
+
+// Stackless traversal with parent links: at every internal node the nearer
+// child (by box distance, ties to child 0) is visited first; on the way back
+// up the same rule tells whether the far child is still pending. A box is
+// skipped only when strictly farther than the best distance, so the result is
+// the lexicographic minimum of (distance^2, triangle index) over all triangles.
+__device__ inline void nearest_triangle(d3 p, const BvhNode* __restrict__ nodes, int64_t n,
+                                        const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
+                                        Best& best) {
+    // `best` arrives initialised: empty (+inf), or seeded with a real
+    // triangle whose distance bounds the search from the start.
+    if (n == 1) {
+        test_leaf(0, p, corners, leaf_triangle, best);
+        return;
+    }
+    int32_t node = 0;
+    bool after_near = false;
+    // Every internal node is entered, resumed and left at most once, so a
+    // well-formed tree needs fewer than 4n steps; the bound only guards the
+    // device against a malformed one (the query then reports triangle -1).
+    int64_t budget = 4 * n + 64;
+    for (;;) {
+        if (--budget < 0) {
+            best.triangle = -1;
+            return;
+        }
+        const BvhNode* nd = nodes + node;
+        const double dl = box_d2(p, nd->box), dr = box_d2(p, nd->box + 6);
+        const bool left_near = dl <= dr;
+        const int32_t near = left_near ? nd->child[0] : nd->child[1];
+        const int32_t far = left_near ? nd->child[1] : nd->child[0];
+        const double dn = left_near ? dl : dr, df = left_near ? dr : dl;
+        if (!after_near) {
+            if (!(dn > best.d2)) {
+                if (near < 0) {
+                    test_leaf(~near, p, corners, leaf_triangle, best);
+                    after_near = true;
+                    continue;
+                }
+                node = near;
+                continue;
+            }
+        } else {
+            if (!(df > best.d2)) {
+                if (far >= 0) {
+                    node = far;
+                    after_near = false;
+                    continue;
+                }
+                test_leaf(~far, p, corners, leaf_triangle, best);
+            }
+        }
+        // This node is finished: climb until a parent still owes its far child.
+        for (;;) {
+            if (node == 0) return;
+            if (--budget < 0) {
+                best.triangle = -1;
+                return;
+            }
+            const int32_t parent = nodes[node].parent;
+            const BvhNode* pn = nodes + parent;
+            const double pl = box_d2(p, pn->box), pr = box_d2(p, pn->box + 6);
+            const int32_t pnear = pl <= pr ? pn->child[0] : pn->child[1];
+            const int32_t came_from = node;
+            node = parent;
+            if (pnear == came_from) {
+                after_near = true;
+                break;
+            }
+        }
+    }
+}
+
+__device__ inline double robust_weight(int kind, double s, double t) {
+    if (s < 0.0) s = 0.0;
+    if (kind == 1) {
+        if (t <= 0.0) return 1.0;
+        if (s <= t * t) return 1.0;
+        return t / sqrt(s);
+    }
+    if (kind == 2) {
+        if (t <= 0.0) return 1.0;
+        const double d = 1.0 + s / (t * t);
+        return 1.0 / (d * d);
+    }
+    return 1.0;
+}
+
+// One thread per query: mask, nearest triangle, interpolated normal, the
+// distance / normal / slope filters in the CPU route's order
+// (src/solver/correspondence.cpp:33-59), then the IRLS weight of
+// src/solver/route_a.cpp:19-38 for accepted rows.
+__global__ void __launch_bounds__(256)
+query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __restrict__ corners,
+             const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
+             const double* __restrict__ vnormals, const double* __restrict__ queries,
+             const double* __restrict__ qnormals, const uint8_t* __restrict__ mask, int64_t count,
+             QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
+             double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
+             int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (mask && mask[i] == 0) {
+        st3(out_p, i, d3{0.0, 0.0, 0.0});
+        st3(out_n, i, d3{0.0, 0.0, 0.0});
+        out_d[i] = 0.0;
+        out_t[i] = -1;
+        out_rej[i] = 4;
+        if (out_u) out_u[i] = 1.0;
+        return;
+    }
+    const d3 q = ld3(queries, i);
+    Best best;
+    best.d2 = __builtin_inf();
+    best.leaf = -1;
+    best.triangle = 0x7fffffff;
+    // Warm start: the triangle this query matched last time is an exact
+    // upper bound. The search still returns the lexicographic minimum of
+    // (distance^2, triangle) -- boxes at exactly the bound are still visited --
+    // so the result is the one a cold search gives, found with less work.
+    const int32_t seed = warm ? warm[i] : -1;
+    if (seed >= 0 && seed < nt) {
+        const int32_t* sf = faces + 3 * static_cast<int64_t>(seed);
+        double bary[3];
+        const d3 c = closest_on_triangle(q, ld3(tpoints, sf[0]), ld3(tpoints, sf[1]), ld3(tpoints, sf[2]), bary);
+        best.d2 = norm2(c - q);
+        best.triangle = seed;
+        best.point = c;
+        best.bary[0] = bary[0];
+        best.bary[1] = bary[1];
+        best.bary[2] = bary[2];
+    }
+    nearest_triangle(q, nodes, nt, corners, leaf_triangle, best);
+    if (best.triangle < 0 || best.triangle >= nt) {
+        // Only a malformed hierarchy gets here: report the row as unusable.
+        st3(out_p, i, q);
+        st3(out_n, i, d3{0.0, 0.0, 0.0});
+        out_d[i] = __builtin_inf();
+        out_t[i] = -1;
+        out_rej[i] = 1;
+        if (out_u) out_u[i] = 1.0;
+        return;
+    }
+    const int32_t* f = faces + 3 * static_cast<int64_t>(best.triangle);
+    const d3 na = ld3(vnormals, f[0]), nb = ld3(vnormals, f[1]), nc = ld3(vnormals, f[2]);
+    const d3 normal = unit((na * best.bary[0] + nb * best.bary[1]) + nc * best.bary[2]);
+    const double distance = sqrt(best.d2);
+    uint8_t rej = 0;
+    if (distance > s.distance_threshold) {
+        rej = 1;
+    } else if (qnormals && dot(ld3(qnormals, i), normal) < s.min_normal_cosine) {
+        rej = 2;
+    } else if (distance > 0.0) {
+        const double slope = fabs(dot(normal, q - best.point)) / distance;
+        if (slope < s.min_slope) rej = 3;
+    }
+    st3(out_p, i, best.point);
+    st3(out_n, i, normal);
+    out_d[i] = distance;
+    out_t[i] = best.triangle;
+    out_rej[i] = rej;
+    if (out_u) {
+        const d3 off = q - best.point;
+        out_u[i] = (rej == 0 && s.robust_threshold > 0.0)
+                       ? robust_weight(s.robust_kernel, norm2(off), s.robust_threshold)
+                       : 1.0;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+summary_partial(const uint8_t* __restrict__ rej, const double* __restrict__ dist, int64_t n, double* __restrict__ part) {
+    __shared__ double lds[6 * 4];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        const int r = rej[i];
+        if (r == 0) {
+            v[0] += 1.0;
+            v[5] += dist[i];
+        } else if (r <= 4) {
+            v[r] += 1.0;
+        }
+    }
+    double out[6];
+    block_sum<6, 256>(v, lds, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = out[k];
+}
+
+__global__ void summary_final(const double* __restrict__ part, int blocks, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < blocks; ++b)
+        for (int k = 0; k < 6; ++k) s[k] += part[6 * b + k];
+    for (int k = 0; k < 6; ++k) out[k] = s[k];
+}
+
+__global__ void distance_keys(const uint8_t* __restrict__ rej, const double* __restrict__ dist, int64_t n,
+                              uint64_t* __restrict__ keys) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // Non-negative doubles order like their bit patterns; rejected rows sort last.
+    keys[i] = rej[i] == 0 ? static_cast<uint64_t>(__double_as_longlong(dist[i])) : ~0ull;
+}
+
+__global__ void pick_quantiles(const uint64_t* __restrict__ keys, int64_t accepted, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (accepted <= 0) {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        return;
+    }
+    const double pos = 0.95 * static_cast<double>(accepted - 1);
+    int64_t k = static_cast<int64_t>(pos < static_cast<double>(accepted - 1) ? pos : static_cast<double>(accepted - 1));
+    out[0] = __longlong_as_double(static_cast<long long>(keys[k]));
+    out[1] = __longlong_as_double(static_cast<long long>(keys[accepted - 1]));
+}
+
+}  // namespace
+
+void build_vertex_faces(const int32_t* faces, int64_t nf, int64_t nv, VertexFaces& out, SortScratch& scratch,
+                        hipStream_t stream) {
+    DeviceBuffer<uint64_t> keys(3 * nf);
+    DeviceBuffer<uint32_t> vals(3 * nf);
+    if (nf > 0) {
+        vertex_face_entries<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(faces, nf, keys.get(), vals.get());
+        NRICP_LAUNCH_CHECK("vertex_face_entries");
+        radix_sort_pairs(keys.get(), vals.get(), 3 * nf, bits_for(static_cast<uint64_t>(nv)), scratch, stream);
+    }
+    out.offsets.resize(nv + 1);
+    out.faces.resize(3 * nf);
+    offsets_from_sorted(keys.get(), 3 * nf, 0, ~0ull, nv, out.offsets.get(), stream);
+    if (nf > 0) {
+        copy_u32_to_i32<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(vals.get(), 3 * nf, out.faces.get());
+        NRICP_LAUNCH_CHECK("copy_u32_to_i32");
+    }
+    NRICP_CHECK(hipStreamSynchronize(stream));  // keys/vals are released on return
+}
+
+void vertex_normals(const double* points, const int32_t* faces, int64_t nf, const VertexFaces& vf, int64_t nv,
+                    double* face_scratch, double* normals, hipStream_t stream) {
+    if (nf > 0) {
+        face_cross_kernel<<<blocks_for(nf, 256), 256, 0, stream>>>(points, faces, nf, face_scratch);
+        NRICP_LAUNCH_CHECK("face_cross_kernel");
+    }
+    if (nv > 0) {
+        gather_normals_kernel<<<blocks_for(nv, 256), 256, 0, stream>>>(face_scratch, vf.offsets.get(),
+                                                                        vf.faces.get(), nv, normals);
+        NRICP_LAUNCH_CHECK("gather_normals_kernel");
+    }
+}
+
+void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hipStream_t stream) {
+    const int64_t nt = bvh.triangle_count;
+    if (nt <= 0) throw InvalidArgument("the target has no triangles");
+    if (nt >= (1ll << 31)) throw InvalidArgument("too many target triangles");
+    // Target vertex normals (face order per vertex, as the CPU route).
+    build_vertex_faces(bvh.faces.get(), nt, nv, vf, scratch, stream);
+    bvh.vertex_normals.resize(3 * nv);
+    DeviceBuffer<double> cross(3 * nt);
+    vertex_normals(bvh.points.get(), bvh.faces.get(), nt, vf, nv, cross.get(), bvh.vertex_normals.get(), stream);
+    // Morton keys of the centroids, triangle index in the low bits.
+    bvh.centroids.resize(3 * nt);
+    centroid_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.points.get(), bvh.faces.get(), nt, bvh.centroids.get());
+    NRICP_LAUNCH_CHECK("centroid_kernel");
+    const int parts = static_cast<int>(std::min<int64_t>(blocks_for(nt, 256), 256));
+    bvh.bounds.resize(6 * parts);
+    bounds_partial<<<parts, 256, 0, stream>>>(bvh.centroids.get(), nt, bvh.bounds.get());
+    NRICP_LAUNCH_CHECK("bounds_partial");
+    bounds_final<<<1, 64, 0, stream>>>(bvh.bounds.get(), parts);
+    NRICP_LAUNCH_CHECK("bounds_final");
+    const int index_bits = std::max(1, bits_for(static_cast<uint64_t>(nt - 1)));
+    bvh.keys.resize(nt);
+    morton_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.centroids.get(), nt, bvh.bounds.get(), index_bits,
+                                                            bvh.keys.get());
+    NRICP_LAUNCH_CHECK("morton_kernel");
+    radix_sort_pairs(bvh.keys.get(), nullptr, nt, 30 + index_bits, scratch, stream);
+    bvh.leaf_triangle.resize(nt);
+    bvh.leaf_corners.resize(9 * nt);
+    leaf_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.keys.get(), nt, (1ull << index_bits) - 1,
+                                                          bvh.points.get(), bvh.faces.get(), bvh.leaf_triangle.get(),
+                                                          bvh.leaf_corners.get());
+    NRICP_LAUNCH_CHECK("leaf_kernel");
+    bvh.leaf_parent.resize(nt);
+    bvh.nodes.resize(std::max<int64_t>(nt - 1, 1));
+    if (nt > 1) {
+        karras_kernel<<<blocks_for(nt - 1, 256), 256, 0, stream>>>(bvh.keys.get(), nt, bvh.nodes.get(),
+                                                                    bvh.leaf_parent.get());
+        NRICP_LAUNCH_CHECK("karras_kernel");
+        bvh.flags.resize(nt - 1);
+        bvh.flags.zero(stream);
+        refit_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.leaf_corners.get(), nt, bvh.leaf_parent.get(),
+                                                               bvh.nodes.get(), bvh.flags.get());
+        NRICP_LAUNCH_CHECK("refit_kernel");
+    }
+    NRICP_CHECK(hipStreamSynchronize(stream));  // `cross` is released on return
+}
+
+void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
+                    int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
+                    double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
+                    hipStream_t stream, const int32_t* warm) {
+    if (count <= 0) return;
+    query_kernel<<<blocks_for(count, 256), 256, 0, stream>>>(
+        bvh.nodes.get(), bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(), bvh.faces.get(),
+        bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm, out_points,
+        out_normals, out_distance, out_triangle, out_rejection, out_robust);
+    NRICP_LAUNCH_CHECK("query_kernel");
+}
+
+void summarize_correspondences(const uint8_t* rejection, const double* distance, int64_t count,
+                               DeviceBuffer<double>& partials, double* out, hipStream_t stream) {
+    const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(count, 256), 240)));
+    partials.resize(6 * blocks);
+    summary_partial<<<blocks, 256, 0, stream>>>(rejection, distance, count, partials.get());
+    NRICP_LAUNCH_CHECK("summary_partial");
+    summary_final<<<1, 64, 0, stream>>>(partials.get(), blocks, out);
+    NRICP_LAUNCH_CHECK("summary_final");
+}
+
+void distance_quantiles(const uint8_t* rejection, const double* distance, int64_t count, DeviceBuffer<uint64_t>& keys,
+                        SortScratch& scratch, double* out, int64_t accepted, hipStream_t stream) {
+    if (count <= 0) return;
+    keys.resize(count);
+    distance_keys<<<blocks_for(count, 256), 256, 0, stream>>>(rejection, distance, count, keys.get());
+    NRICP_LAUNCH_CHECK("distance_keys");
+    radix_sort_pairs(keys.get(), nullptr, count, 64, scratch, stream);
+    pick_quantiles<<<1, 64, 0, stream>>>(keys.get(), accepted, out);
+    NRICP_LAUNCH_CHECK("pick_quantiles");
+}
+
+}  // namespace nricp_hip

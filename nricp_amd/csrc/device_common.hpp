@@ -1,0 +1,130 @@
+// Device-side helpers shared by the gfx950 kernels: FP64 vector math, wave64
+// reductions, a grid barrier for persistent launches, and error plumbing.
+//
+// Every kernel is compiled with -ffp-contract=off: the closest-point, sampling
+// and binding arithmetic is compared bit for bit with the FP64 CPU route, so
+// no multiply-add may be fused behind the source's back. Kernels that want an
+// FMA ask for it with fma().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace nricp_hip {
+
+// ------------------------------------------------------------ errors -------
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+inline void check(hipError_t status, const char* what) {
+    if (status != hipSuccess) {
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(status));
+    }
+}
+#define NRICP_CHECK(expr) ::nricp_hip::check((expr), #expr)
+#define NRICP_LAUNCH_CHECK(name) ::nricp_hip::check(hipGetLastError(), name)
+
+constexpr int kWave = 64;
+
+inline int blocks_for(int64_t n, int threads) {
+    return static_cast<int>((n + threads - 1) / threads);
+}
+
+// ------------------------------------------------------------ vectors ------
+
+struct d3 {
+    double x, y, z;
+};
+__host__ __device__ inline d3 mk(double x, double y, double z) { return {x, y, z}; }
+__host__ __device__ inline d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline d3 operator/(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline d3 cross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline double norm2(d3 a) { return dot(a, a); }
+__device__ inline d3 unit(d3 a) {
+    const double n = sqrt(dot(a, a));
+    return n > 0.0 ? a / n : d3{0.0, 0.0, 0.0};
+}
+__device__ inline d3 ld3(const double* p, int64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ inline void st3(double* p, int64_t i, d3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+// Row-major 3x3 times vector, same operation order as the CPU route.
+__device__ inline d3 mul3(const double* r, d3 v) {
+    return {r[0] * v.x + r[1] * v.y + r[2] * v.z, r[3] * v.x + r[4] * v.y + r[5] * v.z,
+            r[6] * v.x + r[7] * v.y + r[8] * v.z};
+}
+
+// ----------------------------------------------------------- reductions ----
+
+__device__ inline double wave_sum(double v) {
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
+    return v;
+}
+__device__ inline float wave_sum(float v) {
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
+    return v;
+}
+
+// Block-wide sum of `count` values per thread into out[0..count), fixed order
+// (wave butterflies then waves in index order) so the result is reproducible.
+template <int Count, int Threads>
+__device__ inline void block_sum(const double (&v)[Count], double* lds /* Count * Threads/64 */,
+                                 double (&out)[Count]) {
+    constexpr int waves = Threads / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < Count; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) lds[k * waves + wave] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < Count; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < waves; ++w) s += lds[k * waves + w];
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
+// --------------------------------------------------------- grid barrier ----
+
+// Sense-free generation barrier for persistent grids launched with
+// hipLaunchCooperativeKernel (which guarantees co-residency). `state[0]` is the
+// arrival counter, `state[1]` the generation. Agent-scope acquire/release
+// orders every global write before the barrier against every read after it.
+__device__ inline void grid_barrier(unsigned* state, unsigned participants) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(state + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived =
+            __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == participants - 1) {
+            __hip_atomic_store(state, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(state + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(state + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace nricp_hip

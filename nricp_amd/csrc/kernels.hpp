@@ -1,0 +1,216 @@
+// Host-callable launchers of the gfx950 kernels, grouped by stage. All take a
+// stream and enqueue asynchronously; sizes are checked by the caller (the
+// session or the operator entry points) before anything is launched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "buffers.hpp"
+#include "primitives.hpp"
+
+namespace nricp_hip {
+
+// ------------------------------------------------------------- meshes ------
+
+// vertex -> incident faces, faces in increasing index per vertex (the order
+// the CPU route accumulates face normals in).
+struct VertexFaces {
+    DeviceBuffer<int32_t> offsets;  // V + 1
+    DeviceBuffer<int32_t> faces;    // 3F
+};
+void build_vertex_faces(const int32_t* faces, int64_t face_count, int64_t vertex_count, VertexFaces& out,
+                        SortScratch& scratch, hipStream_t stream);
+// Area-weighted unit vertex normals, summed per vertex in face order.
+void vertex_normals(const double* points, const int32_t* faces, int64_t face_count, const VertexFaces& vf,
+                    int64_t vertex_count, double* face_scratch /* 3F */, double* normals, hipStream_t stream);
+
+// --------------------------------------------------------------- LBVH ------
+
+struct BvhNode {
+    double box[12];  // child 0: lo xyz, hi xyz; child 1: lo xyz, hi xyz
+    int32_t child[2];  // >= 0 internal node, < 0 leaf ~index
+    int32_t parent;
+    int32_t pad;
+};
+static_assert(sizeof(BvhNode) == 112, "node layout");
+
+struct Lbvh {
+    int64_t triangle_count = 0;
+    DeviceBuffer<BvhNode> nodes;        // T - 1 internal nodes, root = 0
+    DeviceBuffer<int32_t> leaf_parent;  // T
+    DeviceBuffer<int32_t> leaf_triangle;  // T: triangle id of each leaf (Morton order)
+    DeviceBuffer<double> leaf_corners;  // T x 9: a, b, c of each leaf
+    DeviceBuffer<int32_t> faces;        // T x 3 (original order)
+    DeviceBuffer<double> vertex_normals;  // Vt x 3
+    DeviceBuffer<double> points;          // Vt x 3
+    DeviceBuffer<uint64_t> keys;
+    DeviceBuffer<int32_t> flags;
+    DeviceBuffer<double> centroids;
+    DeviceBuffer<double> bounds;  // partial boxes
+};
+// Builds the hierarchy over a target mesh already on the device (points,
+// faces copied into the Lbvh). Morton codes (10 bits per axis) of the triangle
+// centroids, radix sorted with the triangle index as tie-break, Karras (2012)
+// topology, bottom-up refit of exact FP64 boxes.
+void build_lbvh(Lbvh& bvh, int64_t vertex_count, SortScratch& scratch, VertexFaces& vf, hipStream_t stream);
+
+struct QuerySettings {
+    double distance_threshold;
+    double min_normal_cosine;
+    double min_slope;
+    int robust_kernel;  // 0 none, 1 huber, 2 geman-mcclure
+    double robust_threshold;  // <= 0 disables reweighting
+};
+// Closest point for every query, then the correspondence filters (mask,
+// distance, normal, slope) and the IRLS weight. Masked queries are not
+// searched. `query_normals` may be null (no normal filter). `warm`, when
+// given, holds each query's previous triangle (may alias out_triangle).
+void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
+                    int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
+                    double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
+                    hipStream_t stream, const int32_t* warm = nullptr);
+
+// counts[0..5) = accepted, distance, normal, slope, masked; sums[0] = sum of
+// accepted distances. Deterministic two-stage reduction.
+void summarize_correspondences(const uint8_t* rejection, const double* distance, int64_t count,
+                               DeviceBuffer<double>& partials, double* out /* 6: counts..., sum */,
+                               hipStream_t stream);
+// p95 and max of the accepted distances by radix-sorting their bit patterns.
+void distance_quantiles(const uint8_t* rejection, const double* distance, int64_t count,
+                        DeviceBuffer<uint64_t>& keys, SortScratch& scratch, double* out /* p95, max */,
+                        int64_t accepted, hipStream_t stream);
+
+// ------------------------------------------------------- deformation -------
+
+// v' = sum over the binding's slots of w (R (v - g) + g + t), invalid nodes
+// skipped (all slots, as the CPU evaluate does).
+void evaluate_deformation(const double* points, const int32_t* bind_nodes, const double* bind_weights,
+                          const int32_t* bind_counts, int64_t vertex_count, const double* rotations,
+                          const double* translations, const double* rest, int64_t node_count, double* out,
+                          hipStream_t stream);
+// R <- exp([w]) R, t <- t + dt per node, written to out (may alias input).
+void retract(const double* rotations, const double* translations, const double* delta, int64_t node_count,
+             double* out_rotations, double* out_translations, hipStream_t stream);
+
+// ------------------------------------------------------------- graph -------
+
+// Farthest-point seeds (first maximum wins), as a persistent cooperative grid.
+void farthest_point_seeds(const double* points, int64_t count, int32_t seeds, int32_t first, int32_t* out,
+                          DeviceBuffer<double>& partial, DeviceBuffer<unsigned>& barrier, hipStream_t stream);
+// Nearest `k` nodes by (d^2, index), weights (1 - d/dmax)^2 normalised.
+void bind_vertices(const double* points, int64_t count, const double* nodes, int64_t node_count, int32_t k,
+                   int32_t* out_nodes, double* out_weights, int32_t* out_counts, hipStream_t stream);
+// k nearest by (distance, index) per node, union as sorted unique (min, max)
+// pairs. Returns the edge count (synchronises).
+int64_t build_node_edges(const double* nodes, int64_t node_count, int32_t k, DeviceBuffer<int32_t>& edges,
+                         DeviceBuffer<uint64_t>& keys, DeviceBuffer<int32_t>& flags, DeviceBuffer<int32_t>& pos,
+                         SortScratch& scratch, hipStream_t stream);
+// Warm start of a finer level from a coarser one.
+void transfer_level(const double* from_nodes, int64_t from_count, const double* from_rotations,
+                    const double* from_translations, const double* to_nodes, int64_t to_count, int32_t k,
+                    double* out_rotations, double* out_translations, hipStream_t stream);
+
+// ------------------------------------------------------------ layout -------
+
+// Gather/scatter structure of one level: which (vertex, slot) rows each node
+// owns, which node pairs interact (binding pairs and graph edges, sorted and
+// grouped), the node -> incident-edge lists, and the block-sparse row (BSR)
+// structure of the symmetric system with the position of every block.
+struct LevelLayout {
+    int64_t node_count = 0, edge_count = 0, group_count = 0, nnzb = 0;
+    DeviceBuffer<int32_t> gather_offsets;  // N + 1
+    DeviceBuffer<int32_t> gather_entries;  // vertex << 2 | slot
+    DeviceBuffer<int32_t> node_edge_offsets;  // N + 1
+    DeviceBuffer<int32_t> node_edge_entries;  // edge << 1 | is_second_endpoint
+    DeviceBuffer<int32_t> group_offsets;   // P + 1
+    DeviceBuffer<int32_t> group_entries;   // >= 0: vertex << 4 | lo << 2 | hi; < 0: ~edge
+    DeviceBuffer<int32_t> group_pairs;     // P x 2 (a < b)
+    DeviceBuffer<int32_t> row_offsets;     // N + 1
+    DeviceBuffer<int32_t> columns;         // nnzb
+    DeviceBuffer<int32_t> diag_position;   // N
+    DeviceBuffer<int32_t> pair_position;   // P x 2: block (a,b), block (b,a)
+    // scratch
+    DeviceBuffer<uint64_t> keys;
+    DeviceBuffer<uint32_t> vals;
+    DeviceBuffer<int32_t> counts, offsets, flags, runs;
+};
+void build_level_layout(const int32_t* bind_nodes, const double* bind_weights, const int32_t* bind_counts,
+                        int64_t vertex_count, const int32_t* edges, int64_t edge_count, int64_t node_count,
+                        LevelLayout& layout, SortScratch& scratch, hipStream_t stream);
+
+// ---------------------------------------------------------- assembly -------
+
+struct EnergyWeights {
+    double p2p, plane, smooth, anchor, landmark;
+};
+struct AssemblyInputs {
+    int64_t vertex_count, node_count, edge_count;
+    const double* points;        // rest template, V x 3
+    const int32_t* bind_nodes;   // V x 4
+    const double* bind_weights;  // V x 4
+    const int32_t* bind_counts;  // V
+    const double* rest;          // N x 3
+    const double* rotations;     // N x 9
+    const double* translations;  // N x 3
+    const int32_t* edges;        // E x 2
+    const double* corr_points;   // V x 3
+    const double* corr_normals;  // V x 3
+    const uint8_t* active;       // V: nonzero when the data term applies; when null,
+    const uint8_t* rejection;    //    the data term applies where rejection == 0
+    const double* robust;        // V
+    const double* landmark_target;  // V x 3
+    const uint8_t* landmark_flag;   // V
+    EnergyWeights w;
+};
+struct AssemblyScratch {
+    DeviceBuffer<double> vertex_scale;  // V x 3: p2p, plane, landmark scale (0 = row absent)
+    DeviceBuffer<double> vertex_resid;  // V x 7
+    DeviceBuffer<double> vertex_arm;    // V x 12: rotated offsets per slot
+    DeviceBuffer<double> partials;      // blocks x 5
+    DeviceBuffer<int32_t> touched;      // P
+};
+// Energy only: out[0..5) = p2p, plane, smoothness, anchor, landmark.
+void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream);
+// The system: diagonal and pair blocks into `bsr` (nnzb x 36, at the layout's
+// positions), rhs = -J^T r (N x 6), plus the energies of the same state.
+template <typename Scalar>
+void assemble(const AssemblyInputs& in, const LevelLayout& layout, AssemblyScratch& s, Scalar* bsr, double* rhs,
+              double* energies_out, hipStream_t stream);
+
+// -------------------------------------------------------------- PCG --------
+
+struct PcgOutcome {
+    double iterations;
+    double relative_residual;
+    double converged;
+    double breakdown;
+};
+struct PcgWorkspace {
+    DeviceBuffer<double> x, r, u, w, p, s, damp, minv;  // 6N each, minv 36N
+    DeviceBuffer<double> reduce;    // 2 x blocks x 4
+    DeviceBuffer<unsigned> barrier;  // 2
+    DeviceBuffer<double> outcome;   // PcgOutcome
+    DeviceBuffer<float> r32, u32, w32, p32, s32, x32, minv32;
+    DeviceBuffer<double> xr;        // refinement accumulator
+    // When set, recorded on the stream right before and after the persistent
+    // kernel (device timing of exactly that kernel).
+    hipEvent_t kernel_start = nullptr, kernel_stop = nullptr;
+    int last_groups = 0;             // workgroups of the last launch
+};
+// Solves (A + lambda D) x = b with block-Jacobi preconditioned conjugate
+// gradients (Chronopoulos-Gear form: two grid barriers per iteration) in one
+// persistent launch. FP64 matrix. x is written to `solution`; the outcome is
+// left on the device in ws.outcome.
+void pcg_solve(const LevelLayout& layout, const double* bsr, const double* rhs, int64_t node_count,
+               double lambda, bool diagonal_scaling, double tolerance, int max_iterations, double* solution,
+               PcgWorkspace& ws, hipStream_t stream);
+// Mixed precision: FP32 matrix and inner iterations, FP64 outer residual
+// refinement (r = b - A x in FP64 arithmetic) until the FP64 relative residual
+// reaches `tolerance`.
+void pcg_solve_mixed(const LevelLayout& layout, const float* bsr, const double* rhs, int64_t node_count,
+                     double lambda, bool diagonal_scaling, double tolerance, int max_iterations,
+                     double* solution, PcgWorkspace& ws, hipStream_t stream);
+
+}  // namespace nricp_hip

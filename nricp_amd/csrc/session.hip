@@ -1,0 +1,700 @@
+// The wrap session: device-resident state and the coarse-to-fine schedule.
+//
+// Reference behaviour: the schedule of route A (src/solver/route_a.cpp:67-209,
+// src/solver/route_shared.h:16-38) -- levels, outer correspondence updates,
+// inner damped Gauss-Newton steps with accept iff the energy decreases, damping
+// x0.3 (floor 1e-12) on accept and x4 on reject or solver failure -- cut into
+// the step phases of RouteBSession (src/cuda/session.cu:504-542). Everything
+// that is not a decision runs on the device; the host reads back three numbers
+// per damping attempt (current energy, solver outcome, candidate energy) with
+// one stream synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <iomanip>
+#include <sstream>
+
+#include "session.hpp"
+
+namespace nricp_hip {
+namespace {
+
+__global__ void gather_rows3(const double* __restrict__ p, const int32_t* __restrict__ idx, int64_t n,
+                             double* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t j = idx[i];
+    out[3 * i] = p[3 * j];
+    out[3 * i + 1] = p[3 * j + 1];
+    out[3 * i + 2] = p[3 * j + 2];
+}
+
+__global__ void identity_state(double* __restrict__ R, double* __restrict__ T, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 9; ++k) R[9 * i + k] = (k % 4 == 0) ? 1.0 : 0.0;
+    T[3 * i] = T[3 * i + 1] = T[3 * i + 2] = 0.0;
+}
+
+__global__ void scatter_landmarks(const int32_t* __restrict__ vertices, const double* __restrict__ targets, int64_t n,
+                                  int64_t nv, uint8_t* __restrict__ flag, double* __restrict__ target) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t v = vertices[i];
+    if (v < 0 || v >= nv) return;
+    flag[v] = 1;
+    target[3 * v] = targets[3 * i];
+    target[3 * v + 1] = targets[3 * i + 1];
+    target[3 * v + 2] = targets[3 * i + 2];
+}
+
+double relative_change(double before, double after) {
+    return (before - after) / std::max(std::abs(before), 1e-12);
+}
+
+int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level) {
+    const double g = std::pow(static_cast<double>(std::max(1, growth)), static_cast<double>(level));
+    const int64_t req = std::max<int64_t>(1, static_cast<int64_t>(std::lround(n0 * g)));
+    (void)levels;
+    return std::min<int64_t>(req, nv);
+}
+
+std::string number(double v) {
+    if (!std::isfinite(v)) return "null";
+    std::ostringstream o;
+    o << std::setprecision(17) << v;
+    return o.str();
+}
+
+std::string escape(const std::string& t) {
+    std::string o;
+    for (char c : t) {
+        switch (c) {
+            case '"': o += "\\\""; break;
+            case '\\': o += "\\\\"; break;
+            case '\n': o += "\\n"; break;
+            case '\t': o += "\\t"; break;
+            default: o += c;
+        }
+    }
+    return o;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ graph --
+
+void build_device_graph(const double* points, int64_t nv, int levels, int n0, int growth, int k, int neighbours,
+                        DeviceGraph& g, SortScratch& scratch, hipStream_t stream) {
+    if (nv <= 0) throw InvalidArgument("the template has no vertices");
+    const int lv = std::max(1, levels);
+    g.vertex_count = nv;
+    g.levels.clear();
+    g.levels.resize(lv);
+    int64_t finest = 0;
+    for (int l = 0; l < lv; ++l) finest = std::max(finest, level_count_for(lv, n0, growth, nv, l));
+    DeviceBuffer<int32_t> seeds(finest);
+    DeviceBuffer<double> partial;
+    DeviceBuffer<unsigned> barrier;
+    farthest_point_seeds(points, nv, static_cast<int32_t>(finest), 0, seeds.get(), partial, barrier, stream);
+    DeviceBuffer<uint64_t> keys;
+    DeviceBuffer<int32_t> flags, pos;
+    for (int l = 0; l < lv; ++l) {
+        GraphLevel& L = g.levels[l];
+        L.node_count = level_count_for(lv, n0, growth, nv, l);
+        L.node_indices.resize(L.node_count);
+        NRICP_CHECK(hipMemcpyAsync(L.node_indices.get(), seeds.get(), L.node_count * sizeof(int32_t),
+                                   hipMemcpyDeviceToDevice, stream));
+        L.node_positions.resize(3 * L.node_count);
+        gather_rows3<<<blocks_for(L.node_count, 256), 256, 0, stream>>>(points, L.node_indices.get(), L.node_count,
+                                                                         L.node_positions.get());
+        NRICP_LAUNCH_CHECK("gather_rows3");
+        L.bind_nodes.resize(4 * nv);
+        L.bind_weights.resize(4 * nv);
+        L.bind_counts.resize(nv);
+        bind_vertices(points, nv, L.node_positions.get(), L.node_count, k, L.bind_nodes.get(), L.bind_weights.get(),
+                      L.bind_counts.get(), stream);
+        L.edge_count = build_node_edges(L.node_positions.get(), L.node_count, neighbours, L.edges, keys, flags, pos,
+                                        scratch, stream);
+    }
+    NRICP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ----------------------------------------------------------------- report --
+
+// Key order and layout of src/core/report.cpp:46-105.
+std::string report_json(const Report& r) {
+    std::ostringstream o;
+    o << "{\n";
+    o << "  \"version\": \"" << escape(r.version) << "\",\n";
+    o << "  \"route\": \"" << escape(r.route) << "\",\n";
+    o << "  \"precision\": \"" << escape(r.precision) << "\",\n";
+    o << "  \"setup_seconds\": " << number(r.setup_seconds) << ",\n";
+    o << "  \"solve_seconds\": " << number(r.solve_seconds) << ",\n";
+    static const char* names[5] = {"point_to_point", "point_to_plane", "smoothness", "anchor", "landmark"};
+    o << "  \"energy\": {\n";
+    double total = 0.0;
+    for (int i = 0; i < 5; ++i) {
+        o << "    \"" << names[i] << "\": " << number(r.energy[i]) << ",\n";
+        total += r.energy[i];
+    }
+    o << "    \"total\": " << number(total) << "\n  },\n";
+    static const char* cnames[6] = {"total", "accepted", "rejected_distance", "rejected_normal", "rejected_slope",
+                                    "rejected_masked"};
+    o << "  \"correspondence\": {\n";
+    for (int i = 0; i < 6; ++i) o << "    \"" << cnames[i] << "\": " << r.counts[i] << ",\n";
+    o << "    \"distance_mean\": " << number(r.stats[0]) << ",\n";
+    o << "    \"distance_p95\": " << number(r.stats[1]) << ",\n";
+    o << "    \"distance_max\": " << number(r.stats[2]) << "\n  },\n";
+    o << "  \"levels\": [";
+    for (size_t i = 0; i < r.levels.size(); ++i) {
+        const LevelView& l = r.levels[i];
+        o << (i == 0 ? "\n" : ",\n");
+        o << "    {\"level\": " << l.level << ", \"nodes\": " << l.nodes << ", \"outer_iterations\": " << l.outer
+          << ", \"inner_iterations\": " << l.inner << ", \"energy_start\": " << number(l.energy_start)
+          << ", \"energy_end\": " << number(l.energy_end) << ", \"correspondence_mean\": " << number(l.corr_mean)
+          << ", \"converged\": " << (l.converged ? "true" : "false") << "}";
+    }
+    o << (r.levels.empty() ? "],\n" : "\n  ],\n");
+    o << "  \"settings\": {";
+    for (size_t i = 0; i < r.settings.size(); ++i) {
+        o << (i == 0 ? "\n" : ",\n");
+        o << "    \"" << escape(r.settings[i].first) << "\": " << number(r.settings[i].second);
+    }
+    o << (r.settings.empty() ? "}\n" : "\n  }\n");
+    o << "}\n";
+    return o.str();
+}
+
+// ------------------------------------------------------------ kernel clock --
+
+hipEvent_t KernelClock::take() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    NRICP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void KernelClock::begin(const char* name, hipStream_t s) {
+    if (!on_) return;
+    Pending p{name, take(), take()};
+    NRICP_CHECK(hipEventRecord(p.start, s));
+    pending_.push_back(p);
+}
+
+bool KernelClock::external(const char* name, hipEvent_t* start, hipEvent_t* stop) {
+    if (!on_) return false;
+    Pending p{name, take(), take()};
+    *start = p.start;
+    *stop = p.stop;
+    pending_.push_back(p);
+    return true;
+}
+
+void KernelClock::end(hipStream_t s) {
+    if (!on_ || pending_.empty()) return;
+    NRICP_CHECK(hipEventRecord(pending_.back().stop, s));
+}
+
+void KernelClock::collect() {
+    for (Pending& p : pending_) {
+        float ms = 0.0f;
+        NRICP_CHECK(hipEventSynchronize(p.stop));
+        NRICP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        auto& t = totals[p.name];
+        t.first += ms;
+        t.second += 1;
+        pool_.push_back(p.start);
+        pool_.push_back(p.stop);
+    }
+    pending_.clear();
+}
+
+KernelClock::~KernelClock() {
+    for (Pending& p : pending_) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+// ---------------------------------------------------------------- session --
+
+Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const DeviceGraph* graph) : settings_(s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (s.level_settings_count > 0 && s.levels == nullptr) throw InvalidArgument("level settings pointer is null");
+    level_settings_.assign(s.levels, s.levels + std::max(0, s.level_settings_count));
+    settings_.levels = nullptr;
+    if (pb.template_vertex_count <= 0 || pb.template_face_count <= 0)
+        throw InvalidArgument("the template needs vertices and faces");
+    if (pb.target_vertex_count <= 0 || pb.target_face_count <= 0)
+        throw InvalidArgument("the target needs vertices and faces");
+    if (s.route != NRICP_HIP_ROUTE_A && s.route != NRICP_HIP_ROUTE_B && s.route != NRICP_HIP_ROUTE_C)
+        throw InvalidArgument("unknown route");
+    if (s.precision != NRICP_HIP_PRECISION_DOUBLE && s.precision != NRICP_HIP_PRECISION_MIXED)
+        throw InvalidArgument("unknown precision");
+    if (s.route == NRICP_HIP_ROUTE_C) throw InvalidArgument("route_c is not available in this build");
+    report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
+    report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
+    hipStream_t st = stream_.get();
+    nv_ = pb.template_vertex_count;
+    nf_ = pb.template_face_count;
+    points_.upload(pb.template_points, 3 * nv_, st);
+    faces_.upload(pb.template_faces, 3 * nf_, st);
+    if (pb.mask) {
+        mask_.upload(pb.mask, nv_, st);
+        has_mask_ = true;
+    }
+    landmark_flag_.resize(nv_);
+    landmark_flag_.zero(st);
+    landmark_target_.resize(3 * nv_);
+    landmark_target_.zero(st);
+    if (pb.landmark_count > 0) {
+        DeviceBuffer<int32_t> lv;
+        DeviceBuffer<double> lt;
+        lv.upload(pb.landmark_vertices, pb.landmark_count, st);
+        lt.upload(pb.landmark_targets, 3 * pb.landmark_count, st);
+        scatter_landmarks<<<blocks_for(pb.landmark_count, 256), 256, 0, st>>>(
+            lv.get(), lt.get(), pb.landmark_count, nv_, landmark_flag_.get(), landmark_target_.get());
+        NRICP_LAUNCH_CHECK("scatter_landmarks");
+        NRICP_CHECK(hipStreamSynchronize(st));
+    }
+    face_scratch_.resize(3 * nf_);
+    // Target mesh; its hierarchy is built by the first step, on the device.
+    bvh_.triangle_count = pb.target_face_count;
+    target_vertex_count_ = pb.target_vertex_count;
+    bvh_.points.upload(pb.target_points, 3 * pb.target_vertex_count, st);
+    bvh_.faces.upload(pb.target_faces, 3 * pb.target_face_count, st);
+    // Template graph: a caller-provided one is reused as is; otherwise the
+    // first step builds it.
+    int64_t nmax = 0;
+    if (graph) {
+        if (graph->vertex_count != nv_) throw InvalidArgument("the graph was built for a different template");
+        shared_graph_ = graph;
+        graph_ = graph;
+        for (const GraphLevel& L : graph->levels) nmax = std::max(nmax, L.node_count);
+    } else {
+        const int lv = std::max(1, s.graph_levels);
+        for (int l = 0; l < lv; ++l) nmax = std::max(nmax, level_count_for(lv, s.nodes_initial, s.growth, nv_, l));
+        graph_ = &own_graph_;
+    }
+    if (s.max_nodes_per_vertex < 1) throw InvalidArgument("max_nodes_per_vertex must be at least 1");
+    if (s.graph_neighbours < 0 || s.graph_neighbours > 16) throw InvalidArgument("graph_neighbours must be in [0, 16]");
+    // Buffers at the finest level's size.
+    for (auto* b : {&rot_, &rot_cand_, &rot_prev_}) b->resize(9 * nmax);
+    for (auto* b : {&trans_, &trans_cand_, &trans_prev_}) b->resize(3 * nmax);
+    delta_.resize(6 * nmax);
+    rhs_.resize(6 * nmax);
+    for (auto* b : {&deformed_, &qnormals_, &corr_points_, &corr_normals_}) b->resize(3 * nv_);
+    corr_dist_.resize(nv_);
+    robust_.resize(nv_);
+    corr_tri_.resize(nv_);
+    rejection_.resize(nv_);
+    energy_cur_.resize(5);
+    energy_cand_.resize(5);
+    summary_.resize(6);
+    quantiles_.resize(2);
+    NRICP_CHECK(hipStreamSynchronize(st));
+    report_.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+const nricp_hip_level_settings& Session::level_settings() const {
+    static const nricp_hip_level_settings defaults = {{1.0, 1.0, 1.0, 0.0, 1.0}, 1, 0.0, 10, 10, 1e-3, 1e-4};
+    if (level_settings_.empty()) return defaults;
+    return level_settings_[std::min(level_, level_settings_.size() - 1)];
+}
+
+AssemblyInputs Session::inputs(const double* R, const double* T) const {
+    const GraphLevel& L = graph_->levels[level_];
+    const nricp_hip_level_settings& ls = level_settings();
+    AssemblyInputs in;
+    in.vertex_count = nv_;
+    in.node_count = L.node_count;
+    in.edge_count = L.edge_count;
+    in.points = points_.get();
+    in.bind_nodes = L.bind_nodes.get();
+    in.bind_weights = L.bind_weights.get();
+    in.bind_counts = L.bind_counts.get();
+    in.rest = L.node_positions.get();
+    in.rotations = R;
+    in.translations = T;
+    in.edges = L.edges.get();
+    in.corr_points = corr_points_.get();
+    in.corr_normals = corr_normals_.get();
+    in.active = nullptr;
+    in.rejection = rejection_.get();
+    in.robust = robust_.get();
+    in.landmark_target = landmark_target_.get();
+    in.landmark_flag = landmark_flag_.get();
+    in.w = EnergyWeights{ls.weights.point_to_point, ls.weights.point_to_plane, ls.weights.smoothness,
+                         ls.weights.anchor, ls.weights.landmark};
+    return in;
+}
+
+void Session::sync() {
+    NRICP_CHECK(hipStreamSynchronize(stream_.get()));
+    clock_.collect();
+}
+
+// Device-side preparation of one wrap: the template's vertex -> face lists,
+// the target hierarchy with its vertex normals, and the template graph.
+void Session::prepare() {
+    hipStream_t st = stream_.get();
+    build_vertex_faces(faces_.get(), nf_, nv_, template_vf_, sort_, st);
+    build_lbvh(bvh_, target_vertex_count_, sort_, target_vf_, st);
+    if (!shared_graph_) {
+        build_device_graph(points_.get(), nv_, settings_.graph_levels, settings_.nodes_initial, settings_.growth,
+                           settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_, sort_, st);
+    }
+    level_ = 0;
+}
+
+void Session::reset() {
+    phase_ = Phase::Prepare;
+    level_ = 0;
+    outer_ = inner_ = 0;
+    damping_ = current_energy_ = previous_level_energy_ = 0.0;
+    assembled_ = false;
+    last_energy_ = {};
+    view_ = LevelView{};
+    const std::string route = report_.route, precision = report_.precision;
+    const double setup = report_.setup_seconds;
+    report_ = Report{};
+    report_.route = route;
+    report_.precision = precision;
+    report_.setup_seconds = setup;
+    trace_.clear();
+    have_warm_ = false;
+    corr_mean_ = 0.0;
+    solve_seconds_ = 0.0;
+}
+
+void Session::begin_level() {
+    hipStream_t st = stream_.get();
+    const GraphLevel& L = graph_->levels[level_];
+    if (level_ == 0) {
+        identity_state<<<blocks_for(L.node_count, 256), 256, 0, st>>>(rot_.get(), trans_.get(), L.node_count);
+        NRICP_LAUNCH_CHECK("identity_state");
+    } else {
+        const GraphLevel& P = graph_->levels[level_ - 1];
+        transfer_level(P.node_positions.get(), P.node_count, rot_.get(), trans_.get(), L.node_positions.get(),
+                       L.node_count, 4, rot_cand_.get(), trans_cand_.get(), st);
+        std::swap(rot_, rot_cand_);
+        std::swap(trans_, trans_cand_);
+    }
+    clock_.begin("layout", st);
+    build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
+                       L.edge_count, L.node_count, layout_, sort_, st);
+    clock_.end(st);
+    if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
+        bsr32_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
+    else
+        bsr_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
+    view_ = LevelView{};
+    view_.level = static_cast<int>(level_);
+    view_.nodes = static_cast<int>(L.node_count);
+    damping_ = settings_.initial_damping;
+    previous_level_energy_ = 0.0;
+    outer_ = 0;
+    inner_ = 0;
+    current_energy_ = 0.0;
+}
+
+void Session::refresh() {
+    hipStream_t st = stream_.get();
+    const GraphLevel& L = graph_->levels[level_];
+    const nricp_hip_level_settings& ls = level_settings();
+    evaluate_deformation(points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_,
+                         rot_.get(), trans_.get(), L.node_positions.get(), L.node_count, deformed_.get(), st);
+    vertex_normals(deformed_.get(), faces_.get(), nf_, template_vf_, nv_, face_scratch_.get(), qnormals_.get(), st);
+    QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, ls.robust_kernel,
+                    ls.robust_threshold};
+    clock_.begin("closest_points", st);
+    closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
+                   corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
+                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr);
+    have_warm_ = true;
+    clock_.end(st);
+    counters_[kQueryLaunches] += 1.0;
+    counters_[kQueryRows] += static_cast<double>(nv_);
+    inner_ = 0;
+    assembled_ = false;
+}
+
+void Session::solve(double damping) {
+    hipStream_t st = stream_.get();
+    const int64_t nn = graph_->levels[level_].node_count;
+    const int max_it = settings_.pcg_max_iterations > 0 ? settings_.pcg_max_iterations : 1;
+    // The clock brackets exactly the persistent kernel (the preconditioner
+    // set-up before it is a separate, much shorter launch).
+    pcg_.kernel_start = pcg_.kernel_stop = nullptr;
+    clock_.external("pcg", &pcg_.kernel_start, &pcg_.kernel_stop);
+    if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
+        pcg_solve_mixed(layout_, bsr32_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
+                        settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
+    else
+        pcg_solve(layout_, bsr_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
+                  settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
+    pcg_.kernel_start = pcg_.kernel_stop = nullptr;
+}
+
+// One inner iteration with all its damping attempts (src/solver/route_a.cpp:131-169).
+bool Session::inner_step() {
+    hipStream_t st = stream_.get();
+    const nricp_hip_level_settings& ls = level_settings();
+    const int64_t nn = graph_->levels[level_].node_count;
+    if (!assembled_) {
+        const AssemblyInputs in = inputs(rot_.get(), trans_.get());
+        clock_.begin("assemble", st);
+        if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
+            assemble<float>(in, layout_, assembly_, bsr32_.get(), rhs_.get(), energy_cur_.get(), st);
+        else
+            assemble<double>(in, layout_, assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st);
+        clock_.end(st);
+        assembled_ = true;
+        // Compulsory traffic of one assembly: per vertex its rest point,
+        // binding, correspondence, normal, weight and flags (134 B); per node
+        // R, t, g in and the diagonal block and rhs out; per pair one block out.
+        const double sb = settings_.precision == NRICP_HIP_PRECISION_MIXED ? 4.0 : 8.0;
+        const double N = static_cast<double>(nn), P = static_cast<double>(layout_.group_count);
+        counters_[kAssembleLaunches] += 1.0;
+        counters_[kAssembleBytes] += 134.0 * static_cast<double>(nv_) + N * (120.0 + 36.0 * sb + 48.0) + P * 36.0 * sb;
+    }
+    const int attempts = std::max(1, settings_.max_damping_attempts);
+    bool accepted = false;
+    double cur = 0.0, cand = 0.0;
+    for (int a = 0; a < attempts; ++a) {
+        solve(damping_);
+        retract(rot_.get(), trans_.get(), delta_.get(), nn, rot_cand_.get(), trans_cand_.get(), st);
+        clock_.begin("energy", st);
+        energies(inputs(rot_cand_.get(), trans_cand_.get()), assembly_, energy_cand_.get(), st);
+        clock_.end(st);
+        NRICP_CHECK(hipMemcpyAsync(staging_.get(), energy_cur_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+        NRICP_CHECK(hipMemcpyAsync(staging_.get() + 5, energy_cand_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+        NRICP_CHECK(hipMemcpyAsync(staging_.get() + 10, pcg_.outcome.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        const double* h = staging_.get();
+        cur = (((h[0] + h[1]) + h[2]) + h[3]) + h[4];
+        if (a == 0) {
+            for (int k = 0; k < 5; ++k) last_energy_[k] = h[k];
+            current_energy_ = cur;
+            trace_.push_back(cur);
+            if (outer_ == 0 && inner_ == 0) view_.energy_start = cur;
+        }
+        // Algorithmic traffic of the solve (matrix with each symmetric pair
+        // counted once, the block preconditioner, and u written and read
+        // once per iteration) and of the candidate energy pass.
+        {
+            const double s = settings_.precision == NRICP_HIP_PRECISION_MIXED ? 4.0 : 8.0;
+            const double P = static_cast<double>(layout_.group_count), N = static_cast<double>(nn);
+            counters_[kPcgLaunches] += 1.0;
+            counters_[kPcgIterations] += h[10];
+            counters_[kPcgBytes] += h[10] * s * (36.0 * (2.0 * N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
+            counters_[kEnergyLaunches] += 1.0;
+            counters_[kEnergyBytes] += 134.0 * static_cast<double>(nv_) + 120.0 * N +
+                                       8.0 * static_cast<double>(graph_->levels[level_].edge_count) + 40.0;
+            counters_[kGaussNewtonSteps] += 1.0;
+        }
+        const bool ok = h[12] != 0.0;
+        if (!ok) {
+            damping_ *= settings_.damping_increase;
+            continue;
+        }
+        cand = (((h[5] + h[6]) + h[7]) + h[8]) + h[9];
+        if (cand < cur) {
+            accepted = true;
+            for (int k = 0; k < 5; ++k) last_energy_[k] = h[5 + k];
+            break;
+        }
+        damping_ *= settings_.damping_increase;
+    }
+    if (!accepted) return false;
+    std::swap(rot_, rot_cand_);
+    std::swap(trans_, trans_cand_);
+    std::swap(energy_cur_, energy_cand_);
+    assembled_ = false;
+    current_energy_ = cand;
+    damping_ = std::max(damping_ * settings_.damping_decrease, 1e-12);
+    inner_ += 1;
+    view_.inner += 1;
+    if (relative_change(cur, cand) <= ls.inner_tolerance) return false;
+    return inner_ < std::max(1, ls.max_inner_iterations);
+}
+
+bool Session::close_outer() {
+    const nricp_hip_level_settings& ls = level_settings();
+    const double level_energy = (((last_energy_[0] + last_energy_[1]) + last_energy_[2]) + last_energy_[3]) + last_energy_[4];
+    view_.energy_end = level_energy;
+    view_.outer += 1;
+    bool converged = false;
+    if (outer_ > 0 && relative_change(previous_level_energy_, level_energy) <= ls.outer_tolerance) {
+        view_.converged = true;
+        converged = true;
+    }
+    previous_level_energy_ = level_energy;
+    outer_ += 1;
+    return converged || outer_ >= std::max(1, ls.max_outer_iterations);
+}
+
+bool Session::end_level() {
+    hipStream_t st = stream_.get();
+    const nricp_hip_level_settings& ls = level_settings();
+    summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
+    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    const double accepted = staging_[16];
+    corr_mean_ = accepted > 0.0 ? staging_[21] / accepted : 0.0;
+    view_.corr_mean = corr_mean_;
+    report_.levels.push_back(view_);
+    const std::string pre = "level" + std::to_string(level_) + "_";
+    report_.settings.emplace_back(pre + "point_to_point", ls.weights.point_to_point);
+    report_.settings.emplace_back(pre + "point_to_plane", ls.weights.point_to_plane);
+    report_.settings.emplace_back(pre + "smoothness", ls.weights.smoothness);
+    report_.settings.emplace_back(pre + "anchor", ls.weights.anchor);
+    report_.settings.emplace_back(pre + "landmark", ls.weights.landmark);
+    report_.settings.emplace_back(pre + "robust_threshold", ls.robust_threshold);
+    if (level_ + 1 == graph_->levels.size()) {
+        final_statistics();
+        return true;
+    }
+    ++level_;
+    return false;
+}
+
+// A fresh correspondence pass at the final state (src/solver/route_a.cpp:40-63)
+// and the energies of the last evaluated state (:199-207).
+void Session::final_statistics() {
+    hipStream_t st = stream_.get();
+    const GraphLevel& L = graph_->levels[level_];
+    evaluate_deformation(points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_,
+                         rot_.get(), trans_.get(), L.node_positions.get(), L.node_count, deformed_.get(), st);
+    vertex_normals(deformed_.get(), faces_.get(), nf_, template_vf_, nv_, face_scratch_.get(), qnormals_.get(), st);
+    QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, 0, 0.0};
+    closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
+                   corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
+                   nullptr, st, have_warm_ ? corr_tri_.get() : nullptr);
+    summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
+    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    const int64_t accepted = static_cast<int64_t>(staging_[16]);
+    distance_quantiles(rejection_.get(), corr_dist_.get(), nv_, quantile_keys_, sort_, quantiles_.get(), accepted, st);
+    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 24, quantiles_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    report_.counts = {nv_, accepted, static_cast<int64_t>(staging_[17]), static_cast<int64_t>(staging_[18]),
+                      static_cast<int64_t>(staging_[19]), static_cast<int64_t>(staging_[20])};
+    report_.stats = {accepted > 0 ? staging_[21] / static_cast<double>(accepted) : 0.0, staging_[24], staging_[25]};
+    for (int k = 0; k < 5; ++k) report_.energy[k] = last_energy_[k];
+}
+
+nricp_hip_progress Session::step() {
+    const auto t0 = std::chrono::steady_clock::now();
+    switch (phase_) {
+        case Phase::Prepare:
+            prepare();
+            begin_level();
+            phase_ = Phase::Refresh;
+            break;
+        case Phase::BeginLevel:
+            begin_level();
+            phase_ = Phase::Refresh;
+            break;
+        case Phase::Refresh:
+            refresh();
+            phase_ = Phase::InnerStep;
+            break;
+        case Phase::InnerStep:
+            phase_ = inner_step() ? Phase::InnerStep : Phase::CloseOuter;
+            break;
+        case Phase::CloseOuter:
+            phase_ = close_outer() ? Phase::EndLevel : Phase::Refresh;
+            break;
+        case Phase::EndLevel:
+            phase_ = end_level() ? Phase::Done : Phase::BeginLevel;
+            break;
+        case Phase::Done:
+            break;
+    }
+    solve_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    report_.solve_seconds = solve_seconds_;
+    nricp_hip_progress p;
+    p.level = static_cast<int32_t>(level_);
+    p.outer = outer_;
+    p.inner = inner_;
+    p.finished = phase_ == Phase::Done ? 1 : 0;
+    p.level_converged = view_.converged ? 1 : 0;
+    p.energy = current_energy_;
+    p.correspondence_mean = corr_mean_;
+    return p;
+}
+
+void Session::run() {
+    while (phase_ != Phase::Done) step();
+}
+
+void Session::vertices(double* out) {
+    hipStream_t st = stream_.get();
+    if (phase_ == Phase::Prepare || (phase_ == Phase::BeginLevel && level_ == 0)) {
+        NRICP_CHECK(hipMemcpyAsync(out, points_.get(), 3 * nv_ * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        return;
+    }
+    // Between levels (phase BeginLevel of level > 0) the state still belongs to
+    // the previous level's nodes.
+    const GraphLevel& S = (phase_ == Phase::BeginLevel) ? graph_->levels[level_ - 1] : graph_->levels[level_];
+    evaluate_deformation(points_.get(), S.bind_nodes.get(), S.bind_weights.get(), S.bind_counts.get(), nv_,
+                         rot_.get(), trans_.get(), S.node_positions.get(), S.node_count, deformed_.get(), st);
+    NRICP_CHECK(hipMemcpyAsync(out, deformed_.get(), 3 * nv_ * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+}
+
+int64_t Session::node_count() const {
+    if (graph_->levels.empty()) return 0;
+    if (phase_ == Phase::BeginLevel && level_ > 0) return graph_->levels[level_ - 1].node_count;
+    return graph_->levels[level_].node_count;
+}
+
+int Session::level_count() const {
+    if (!graph_->levels.empty()) return static_cast<int>(graph_->levels.size());
+    return std::max(1, settings_.graph_levels);
+}
+
+void Session::level_nodes(int64_t* out) const {
+    const int lv = level_count();
+    for (int l = 0; l < lv; ++l)
+        out[l] = static_cast<size_t>(l) < graph_->levels.size()
+                     ? graph_->levels[l].node_count
+                     : level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, l);
+}
+
+void Session::state(double* R, double* T, double* G) {
+    hipStream_t st = stream_.get();
+    if (graph_->levels.empty()) return;  // nothing built yet: no nodes
+    const size_t l = (phase_ == Phase::BeginLevel && level_ > 0) ? level_ - 1 : level_;
+    const GraphLevel& L = graph_->levels[l];
+    const int64_t nn = L.node_count;
+    if (phase_ == Phase::Prepare || (phase_ == Phase::BeginLevel && level_ == 0)) {
+        identity_state<<<blocks_for(nn, 256), 256, 0, st>>>(rot_.get(), trans_.get(), nn);
+        NRICP_LAUNCH_CHECK("identity_state");
+    }
+    if (R) NRICP_CHECK(hipMemcpyAsync(R, rot_.get(), 9 * nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (T) NRICP_CHECK(hipMemcpyAsync(T, trans_.get(), 3 * nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (G) NRICP_CHECK(hipMemcpyAsync(G, L.node_positions.get(), 3 * nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+}
+
+std::string Session::report_text() {
+    Report r = report_;
+    if (phase_ != Phase::Done) {
+        // A partial run reports the last energy evaluation it saw.
+        for (int k = 0; k < 5; ++k) r.energy[k] = last_energy_[k];
+    }
+    return report_json(r);
+}
+
+}  // namespace nricp_hip

@@ -1,0 +1,180 @@
+// Host-side objects behind the C ABI: the device template graph and the wrap
+// session (schedule + resident device state), and the run report.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/nricp_hip.h"
+#include "kernels.hpp"
+
+namespace nricp_hip {
+
+struct GraphLevel {
+    int64_t node_count = 0, edge_count = 0;
+    DeviceBuffer<int32_t> node_indices;
+    DeviceBuffer<double> node_positions;
+    DeviceBuffer<int32_t> bind_nodes;
+    DeviceBuffer<double> bind_weights;
+    DeviceBuffer<int32_t> bind_counts;
+    DeviceBuffer<int32_t> edges;
+};
+
+struct DeviceGraph {
+    int64_t vertex_count = 0;
+    std::vector<GraphLevel> levels;
+};
+
+// Nodes per level min(round(n0 g^l), V); FPS once for the finest count (every
+// coarser level's seeds are a prefix of it); per level: bindings and kNN edges.
+void build_device_graph(const double* points, int64_t vertex_count, int levels, int nodes_initial, int growth,
+                        int max_nodes_per_vertex, int neighbours, DeviceGraph& graph, SortScratch& scratch,
+                        hipStream_t stream);
+
+// Report of a run (RunReport, src/core/report.h:12-59).
+struct LevelView {
+    int level = 0, nodes = 0, outer = 0, inner = 0;
+    double energy_start = 0.0, energy_end = 0.0, corr_mean = 0.0;
+    bool converged = false;
+};
+struct Report {
+    std::string route = "route_a", precision = "double", version = "0.1.0";
+    double setup_seconds = 0.0, solve_seconds = 0.0;
+    std::array<double, 5> energy{};
+    std::array<int64_t, 6> counts{};  // total, accepted, distance, normal, slope, masked
+    std::array<double, 3> stats{};    // mean, p95, max
+    std::vector<LevelView> levels;
+    std::vector<std::pair<std::string, double>> settings;
+};
+std::string report_json(const Report& r);
+
+// Optional per-kernel device timing with HIP events, enabled per session.
+class KernelClock {
+public:
+    void enable(bool on) { on_ = on; }
+    bool enabled() const { return on_; }
+    void begin(const char* name, hipStream_t s);
+    void end(hipStream_t s);
+    // A pending entry whose two events the caller records itself; returns
+    // false (and no events) when the clock is off.
+    bool external(const char* name, hipEvent_t* start, hipEvent_t* stop);
+    void collect();  // after a stream synchronisation
+    std::map<std::string, std::pair<double, int64_t>> totals;  // name -> (ms, launches)
+    ~KernelClock();
+
+private:
+    struct Pending {
+        std::string name;
+        hipEvent_t start, stop;
+    };
+    bool on_ = false;
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t take();
+};
+
+// Work and algorithmic-traffic counters of a session (see the per-unit
+// byte formulas in session.hip), for throughput and roofline reporting.
+enum Counter : int {
+    kPcgLaunches = 0,
+    kPcgIterations,
+    kPcgBytes,
+    kAssembleLaunches,
+    kAssembleBytes,
+    kEnergyLaunches,
+    kEnergyBytes,
+    kQueryLaunches,
+    kQueryRows,
+    kGaussNewtonSteps,
+    kCounterCount
+};
+
+class Session {
+public:
+    Session(const nricp_hip_problem& problem, const nricp_hip_settings& settings, const DeviceGraph* graph);
+    nricp_hip_progress step();
+    void run();
+    // Rewind to the start of the wrap, keeping the uploaded inputs; the next
+    // step rebuilds the hierarchy and graph on the device.
+    void reset();
+    const std::array<double, kCounterCount>& counters() const { return counters_; }
+    bool finished() const { return phase_ == Phase::Done; }
+    void vertices(double* out);
+    int64_t node_count() const;
+    int level_count() const;
+    void level_nodes(int64_t* out) const;
+    void state(double* rotations, double* translations, double* rest);
+    std::string report_text();
+    const std::vector<double>& trace() const { return trace_; }
+    KernelClock& clock() { return clock_; }
+    const KernelClock& clock() const { return clock_; }
+
+private:
+    enum class Phase { Prepare, BeginLevel, Refresh, InnerStep, CloseOuter, EndLevel, Done };
+    const nricp_hip_level_settings& level_settings() const;
+    AssemblyInputs inputs(const double* R, const double* T) const;
+    void prepare();
+    void begin_level();
+    void refresh();
+    bool inner_step();
+    bool close_outer();
+    bool end_level();
+    void solve(double damping);
+    void final_statistics();
+    void sync();
+
+    nricp_hip_settings settings_;
+    std::vector<nricp_hip_level_settings> level_settings_;
+    Stream stream_;
+    SortScratch sort_;
+    int64_t nv_ = 0, nf_ = 0;
+    DeviceBuffer<double> points_;
+    DeviceBuffer<int32_t> faces_;
+    DeviceBuffer<uint8_t> mask_;
+    bool has_mask_ = false;
+    DeviceBuffer<uint8_t> landmark_flag_;
+    DeviceBuffer<double> landmark_target_;
+    VertexFaces template_vf_, target_vf_;
+    DeviceBuffer<double> face_scratch_;
+    Lbvh bvh_;
+    int64_t target_vertex_count_ = 0;
+    DeviceGraph own_graph_;
+    const DeviceGraph* graph_ = nullptr;
+    const DeviceGraph* shared_graph_ = nullptr;
+    std::array<double, kCounterCount> counters_{};
+    LevelLayout layout_;
+    AssemblyScratch assembly_;
+    PcgWorkspace pcg_;
+    // state
+    DeviceBuffer<double> rot_, trans_, rot_cand_, trans_cand_, rot_prev_, trans_prev_, delta_;
+    DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
+    DeviceBuffer<int32_t> corr_tri_;
+    DeviceBuffer<uint8_t> rejection_;
+    DeviceBuffer<double> bsr_;
+    DeviceBuffer<float> bsr32_;
+    DeviceBuffer<double> rhs_;
+    DeviceBuffer<double> energy_cur_, energy_cand_, summary_, summary_partials_, quantiles_;
+    DeviceBuffer<uint64_t> quantile_keys_;
+    PinnedBuffer<double> staging_{32};
+    // schedule
+    Phase phase_ = Phase::Prepare;
+    size_t level_ = 0;
+    int outer_ = 0, inner_ = 0;
+    double damping_ = 0.0, current_energy_ = 0.0, previous_level_energy_ = 0.0;
+    bool assembled_ = false;
+    bool have_warm_ = false;  // corr_tri_ holds the previous query's triangles
+    std::array<double, 5> last_energy_{};
+    LevelView view_;
+    Report report_;
+    std::vector<double> trace_;
+    double corr_mean_ = 0.0;
+    KernelClock clock_;
+    double solve_seconds_ = 0.0;
+};
+
+}  // namespace nricp_hip
