@@ -4,8 +4,9 @@
 // Reference behaviour: closest point and filters follow
 // src/geometry/closest_point.cpp:24-279 and src/solver/correspondence.cpp:18-96
 // (FP64, operation order kept so distances match the CPU route bit for bit);
-// the hierarchy replaces src/cuda/lbvh.cu with a 64-bit-key Karras tree whose
-// traversal is stackless (parent links), so no subtree can be dropped.
+// the hierarchy replaces src/cuda/lbvh.cu with a 64-bit-key Karras tree with
+// outward-rounded FP32 boxes, traversed depth-first with a per-lane LDS stack
+// deep enough for any tree over such keys, so no subtree can be dropped.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -138,6 +139,12 @@ __global__ void leaf_kernel(const uint64_t* __restrict__ keys, int64_t nt, uint6
     }
 }
 
+__global__ void key_index_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t index_mask,
+                                 int32_t* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<int32_t>(keys[i] & index_mask);
+}
+
 __device__ inline int delta_keys(const uint64_t* keys, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
     return __clzll(static_cast<long long>(keys[i] ^ keys[j]));
@@ -202,22 +209,29 @@ __global__ void refit_kernel(const double* __restrict__ corners, int64_t n, cons
         lo[k] = fmin(fmin(a, b), c);
         hi[k] = fmax(fmax(a, b), c);
     }
+    // Outward rounding to FP32: the box still contains the triangle exactly;
+    // unions of FP32 boxes (min / max) are exact from here on.
+    float flo[3], fhi[3];
+    for (int k = 0; k < 3; ++k) {
+        flo[k] = __double2float_rd(lo[k]);
+        fhi[k] = __double2float_ru(hi[k]);
+    }
     int32_t child = ~static_cast<int32_t>(i);
     int32_t node = leaf_parent[i];
     while (node >= 0) {
         BvhNode* nd = nodes + node;
         const int side = nd->child[0] == child ? 0 : 1;
         for (int k = 0; k < 3; ++k) {
-            nd->box[6 * side + k] = lo[k];
-            nd->box[6 * side + 3 + k] = hi[k];
+            nd->box[6 * side + k] = flo[k];
+            nd->box[6 * side + 3 + k] = fhi[k];
         }
         const int32_t prior =
             __hip_atomic_fetch_add(arrivals + node, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (prior == 0) return;
         const int other = 1 - side;
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], __hip_atomic_load(&nd->box[6 * other + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            hi[k] = fmax(hi[k], __hip_atomic_load(&nd->box[6 * other + 3 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            flo[k] = fminf(flo[k], __hip_atomic_load(&nd->box[6 * other + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            fhi[k] = fmaxf(fhi[k], __hip_atomic_load(&nd->box[6 * other + 3 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         }
         child = node;
         node = nd->parent;
@@ -278,10 +292,12 @@ __device__ inline d3 closest_on_triangle(d3 p, d3 a, d3 b, d3 c, double* bary) {
     return a + ab * v + ac * w;
 }
 
-__device__ inline double box_d2(d3 p, const double* b) {
-    const double cx = fmin(fmax(p.x, b[0]), b[3]);
-    const double cy = fmin(fmax(p.y, b[1]), b[4]);
-    const double cz = fmin(fmax(p.z, b[2]), b[5]);
+// Squared distance from p to an FP32 box, in FP64 (the box bounds convert
+// exactly).
+__device__ inline double box_d2(d3 p, const float* b) {
+    const double cx = fmin(fmax(p.x, static_cast<double>(b[0])), static_cast<double>(b[3]));
+    const double cy = fmin(fmax(p.y, static_cast<double>(b[1])), static_cast<double>(b[4]));
+    const double cz = fmin(fmax(p.z, static_cast<double>(b[2])), static_cast<double>(b[5]));
     const d3 d{p.x - cx, p.y - cy, p.z - cz};
     return norm2(d);
 }
@@ -313,75 +329,81 @@ __device__ inline void test_leaf(int32_t leaf, d3 p, const double* __restrict__ 
     }
 }
 
-// Stackless traversal with parent links: at every internal node the nearer
-// child (by box distance, ties to child 0) is visited first; on the way back
-// up the same rule tells whether the far child is still pending. A box is
-// skipped only when strictly farther than the best distance, so the result is
-// the lexicographic minimum of (distance^2, triangle index) over all triangles.
+// Depth-first traversal with a per-lane stack in LDS. At every internal node
+// both child boxes are measured; the nearer child (ties to child 0) is entered
+// and the farther one pushed with its box distance, rounded down to FP32. A box
+// is skipped only when its distance exceeds the best squared distance times
+// (1 + 1e-12) -- a margin over the rounding of the FP64 distances -- so every
+// triangle that can tie or beat the best is tested, and the result is the
+// lexicographic minimum of (distance^2, triangle index) over all triangles.
+// A Karras tree over unique keys of at most 61 bits is at most 61 deep, so a
+// 64-entry stack cannot overflow; a malformed tree is caught by the step
+// budget and the overflow check instead (the query then reports triangle -1).
+constexpr int kQueryThreads = 64;
+constexpr int kStackDepth = 64;
+constexpr double kPruneSlack = 1.0 + 1e-12;
+
+struct LaneStack {
+    int32_t* node;  // [kStackDepth][kQueryThreads]
+    float* dist;
+};
+
 __device__ inline void nearest_triangle(d3 p, const BvhNode* __restrict__ nodes, int64_t n,
                                         const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
-                                        Best& best) {
+                                        LaneStack stack, Best& best) {
     // `best` arrives initialised: empty (+inf), or seeded with a real
     // triangle whose distance bounds the search from the start.
     if (n == 1) {
         test_leaf(0, p, corners, leaf_triangle, best);
         return;
     }
+    const int lane = threadIdx.x;
+    int sp = 0;
     int32_t node = 0;
-    bool after_near = false;
-    // Every internal node is entered, resumed and left at most once, so a
-    // well-formed tree needs fewer than 4n steps; the bound only guards the
-    // device against a malformed one (the query then reports triangle -1).
-    int64_t budget = 4 * n + 64;
+    int64_t budget = 2 * n + 64;
     for (;;) {
+        // `node` is an internal node that has to be opened.
         if (--budget < 0) {
             best.triangle = -1;
             return;
         }
-        const BvhNode* nd = nodes + node;
-        const double dl = box_d2(p, nd->box), dr = box_d2(p, nd->box + 6);
+        const BvhNode nd = nodes[node];
+        const double dl = box_d2(p, nd.box), dr = box_d2(p, nd.box + 6);
         const bool left_near = dl <= dr;
-        const int32_t near = left_near ? nd->child[0] : nd->child[1];
-        const int32_t far = left_near ? nd->child[1] : nd->child[0];
+        const int32_t near = left_near ? nd.child[0] : nd.child[1];
+        const int32_t far = left_near ? nd.child[1] : nd.child[0];
         const double dn = left_near ? dl : dr, df = left_near ? dr : dl;
-        if (!after_near) {
-            if (!(dn > best.d2)) {
-                if (near < 0) {
-                    test_leaf(~near, p, corners, leaf_triangle, best);
-                    after_near = true;
-                    continue;
-                }
-                node = near;
-                continue;
-            }
-        } else {
-            if (!(df > best.d2)) {
-                if (far >= 0) {
-                    node = far;
-                    after_near = false;
-                    continue;
-                }
-                test_leaf(~far, p, corners, leaf_triangle, best);
-            }
-        }
-        // This node is finished: climb until a parent still owes its far child.
-        for (;;) {
-            if (node == 0) return;
-            if (--budget < 0) {
+        if (!(df > best.d2 * kPruneSlack)) {
+            if (sp == kStackDepth) {
                 best.triangle = -1;
                 return;
             }
-            const int32_t parent = nodes[node].parent;
-            const BvhNode* pn = nodes + parent;
-            const double pl = box_d2(p, pn->box), pr = box_d2(p, pn->box + 6);
-            const int32_t pnear = pl <= pr ? pn->child[0] : pn->child[1];
-            const int32_t came_from = node;
-            node = parent;
-            if (pnear == came_from) {
-                after_near = true;
-                break;
-            }
+            stack.node[sp * kQueryThreads + lane] = far;
+            stack.dist[sp * kQueryThreads + lane] = __double2float_rd(df);
+            ++sp;
         }
+        if (!(dn > best.d2 * kPruneSlack)) {
+            if (near >= 0) {
+                node = near;
+                continue;
+            }
+            test_leaf(~near, p, corners, leaf_triangle, best);
+        }
+        // Pop until an entry is still within reach.
+        node = -1;
+        while (sp > 0) {
+            --sp;
+            const int32_t c = stack.node[sp * kQueryThreads + lane];
+            const double d = static_cast<double>(stack.dist[sp * kQueryThreads + lane]);
+            if (d > best.d2 * kPruneSlack) continue;
+            if (c < 0) {
+                test_leaf(~c, p, corners, leaf_triangle, best);
+                continue;
+            }
+            node = c;
+            break;
+        }
+        if (node < 0) return;
     }
 }
 
@@ -404,16 +426,20 @@ __device__ inline double robust_weight(int kind, double s, double t) {
 // distance / normal / slope filters in the CPU route's order
 // (src/solver/correspondence.cpp:33-59), then the IRLS weight of
 // src/solver/route_a.cpp:19-38 for accepted rows.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kQueryThreads)
 query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __restrict__ corners,
              const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
              const double* __restrict__ vnormals, const double* __restrict__ queries,
              const double* __restrict__ qnormals, const uint8_t* __restrict__ mask, int64_t count,
              QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
              double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
-             int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+             int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
+             const int32_t* __restrict__ order) {
+    __shared__ int32_t stack_node[kStackDepth * kQueryThreads];
+    __shared__ float stack_dist[kStackDepth * kQueryThreads];
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const int64_t i = order ? static_cast<int64_t>(order[slot]) : slot;
     if (mask && mask[i] == 0) {
         st3(out_p, i, d3{0.0, 0.0, 0.0});
         st3(out_n, i, d3{0.0, 0.0, 0.0});
@@ -444,7 +470,7 @@ query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __rest
         best.bary[1] = bary[1];
         best.bary[2] = bary[2];
     }
-    nearest_triangle(q, nodes, nt, corners, leaf_triangle, best);
+    nearest_triangle(q, nodes, nt, corners, leaf_triangle, LaneStack{stack_node, stack_dist}, best);
     if (best.triangle < 0 || best.triangle >= nt) {
         // Only a malformed hierarchy gets here: report the row as unusable.
         st3(out_p, i, q);
@@ -609,15 +635,34 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     NRICP_CHECK(hipStreamSynchronize(stream));  // `cross` is released on return
 }
 
+void morton_order(const double* points, int64_t n, int32_t* out, SortScratch& scratch, DeviceBuffer<uint64_t>& keys,
+                  DeviceBuffer<double>& bounds, hipStream_t stream) {
+    if (n <= 0) return;
+    if (n >= (1ll << 31)) throw InvalidArgument("too many points to order");
+    const int parts = static_cast<int>(std::min<int64_t>(blocks_for(n, 256), 256));
+    bounds.resize(6 * parts);
+    bounds_partial<<<parts, 256, 0, stream>>>(points, n, bounds.get());
+    NRICP_LAUNCH_CHECK("bounds_partial");
+    bounds_final<<<1, 64, 0, stream>>>(bounds.get(), parts);
+    NRICP_LAUNCH_CHECK("bounds_final");
+    const int index_bits = std::max(1, bits_for(static_cast<uint64_t>(n - 1)));
+    keys.resize(n);
+    morton_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(points, n, bounds.get(), index_bits, keys.get());
+    NRICP_LAUNCH_CHECK("morton_kernel");
+    radix_sort_pairs(keys.get(), nullptr, n, 30 + index_bits, scratch, stream);
+    key_index_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(keys.get(), n, (1ull << index_bits) - 1, out);
+    NRICP_LAUNCH_CHECK("key_index_kernel");
+}
+
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
-                    hipStream_t stream, const int32_t* warm) {
+                    hipStream_t stream, const int32_t* warm, const int32_t* order) {
     if (count <= 0) return;
-    query_kernel<<<blocks_for(count, 256), 256, 0, stream>>>(
+    query_kernel<<<blocks_for(count, kQueryThreads), kQueryThreads, 0, stream>>>(
         bvh.nodes.get(), bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(), bvh.faces.get(),
         bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm, out_points,
-        out_normals, out_distance, out_triangle, out_rejection, out_robust);
+        out_normals, out_distance, out_triangle, out_rejection, out_robust, order);
     NRICP_LAUNCH_CHECK("query_kernel");
 }
 

@@ -135,136 +135,207 @@ __global__ void retract_kernel(const double* __restrict__ R, const double* __res
 constexpr int kFpsThreads = 256;
 constexpr int kFpsPerThread = 16;
 
+// A sampling candidate: its running nearest squared distance, its index and
+// its position (carried along so the winner's position needs no extra load).
 struct ArgMax {
     double value;
     int64_t index;
+    d3 point;
 };
 
 // The larger value wins; equal values go to the lower index (the first
-// maximum of an in-order sweep).
+// maximum of an in-order sweep). A total order, so any reduction order gives
+// the same winner.
 __device__ inline ArgMax better(ArgMax a, ArgMax b) {
     if (b.value > a.value || (b.value == a.value && b.index < a.index)) return b;
     return a;
 }
 
-__device__ inline ArgMax block_argmax(ArgMax v, double* lds_v, int64_t* lds_i) {
-    for (int offset = 32; offset > 0; offset >>= 1) {
-        ArgMax o{__shfl_xor(v.value, offset, 64), __shfl_xor(v.index, offset, 64)};
-        v = better(v, o);
-    }
+__device__ inline ArgMax shfl_xor(const ArgMax& v, int m) {
+    return ArgMax{__shfl_xor(v.value, m, 64), __shfl_xor(v.index, m, 64),
+                  d3{__shfl_xor(v.point.x, m, 64), __shfl_xor(v.point.y, m, 64), __shfl_xor(v.point.z, m, 64)}};
+}
+
+struct ArgMaxLds {
+    double value[kFpsThreads / 64], x[kFpsThreads / 64], y[kFpsThreads / 64], z[kFpsThreads / 64];
+    int64_t index[kFpsThreads / 64];
+};
+
+// Winner over the block; every thread receives it.
+__device__ inline ArgMax block_argmax(ArgMax v, ArgMaxLds& lds) {
+    for (int offset = 32; offset > 0; offset >>= 1) v = better(v, shfl_xor(v, offset));
     const int wave = threadIdx.x / 64;
     if (threadIdx.x % 64 == 0) {
-        lds_v[wave] = v.value;
-        lds_i[wave] = v.index;
+        lds.value[wave] = v.value;
+        lds.index[wave] = v.index;
+        lds.x[wave] = v.point.x;
+        lds.y[wave] = v.point.y;
+        lds.z[wave] = v.point.z;
     }
     __syncthreads();
-    ArgMax r{lds_v[0], lds_i[0]};
-    for (int w = 1; w < kFpsThreads / 64; ++w) r = better(r, ArgMax{lds_v[w], lds_i[w]});
+    ArgMax r{lds.value[0], lds.index[0], d3{lds.x[0], lds.y[0], lds.z[0]}};
+    for (int w = 1; w < kFpsThreads / 64; ++w)
+        r = better(r, ArgMax{lds.value[w], lds.index[w], d3{lds.x[w], lds.y[w], lds.z[w]}});
     __syncthreads();
     return r;
 }
 
 // Persistent FPS: each workgroup owns a contiguous chunk of points and keeps
-// their running nearest squared distances in registers. One grid barrier per
-// seed: every group publishes its chunk's (max, index), and after the barrier
-// every group folds the same partials in the same order.
+// them and their running nearest squared distances in registers. One grid
+// barrier per seed: every group publishes its chunk's winner (value, index,
+// position); after the barrier thread t of every group loads winner t and
+// the group reduces them, so every group picks the same seed.
 __global__ void __launch_bounds__(kFpsThreads)
 fps_kernel(const double* __restrict__ p, int64_t n, int32_t seeds, int32_t first, int32_t* __restrict__ out,
-           double* partial_v, int64_t* partial_i, unsigned* barrier, int64_t chunk) {
-    __shared__ double lds_v[kFpsThreads / 64];
-    __shared__ int64_t lds_i[kFpsThreads / 64];
+           double* partial_v /* 2 x groups x 4 */, int64_t* partial_i /* 2 x groups */, unsigned* barrier,
+           int64_t chunk) {
+    __shared__ ArgMaxLds lds;
     const unsigned groups = gridDim.x;
     const int64_t begin = static_cast<int64_t>(blockIdx.x) * chunk;
     double nearest[kFpsPerThread];
     d3 mine[kFpsPerThread];
     const d3 c0 = ld3(p, first);
-    ArgMax local{-1.0, 0x7fffffffffffffffll};
+    const ArgMax none{-1.0, 0x7fffffffffffffffll, d3{0.0, 0.0, 0.0}};
+    ArgMax local = none;
 #pragma unroll
     for (int k = 0; k < kFpsPerThread; ++k) {
         const int64_t i = begin + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
         if (i < n && i < begin + chunk) {
             mine[k] = ld3(p, i);
             nearest[k] = norm2(mine[k] - c0);
-            local = better(local, ArgMax{nearest[k], i});
+            local = better(local, ArgMax{nearest[k], i, mine[k]});
         } else {
+            mine[k] = d3{0.0, 0.0, 0.0};
             nearest[k] = -1.0;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = first;
     for (int32_t s = 1; s < seeds; ++s) {
-        const ArgMax g = block_argmax(local, lds_v, lds_i);
-        double* pv = partial_v + (s & 1) * groups;
-        int64_t* pi = partial_i + (s & 1) * groups;
-        if (threadIdx.x == 0) {
-            pv[blockIdx.x] = g.value;
-            pi[blockIdx.x] = g.index;
-        }
-        if (groups > 1) grid_barrier(barrier, groups);
-        ArgMax best{-1.0, 0x7fffffffffffffffll};
+        const ArgMax g = block_argmax(local, lds);
+        ArgMax best = g;
         if (groups > 1) {
-            for (unsigned b = 0; b < groups; ++b) best = better(best, ArgMax{pv[b], pi[b]});
-        } else {
-            best = g;
+            double* pv = partial_v + (s & 1) * 4 * groups;
+            int64_t* pi = partial_i + (s & 1) * groups;
+            if (threadIdx.x == 0) {
+                pv[4 * blockIdx.x] = g.value;
+                pv[4 * blockIdx.x + 1] = g.point.x;
+                pv[4 * blockIdx.x + 2] = g.point.y;
+                pv[4 * blockIdx.x + 3] = g.point.z;
+                pi[blockIdx.x] = g.index;
+            }
+            grid_barrier(barrier, groups);
+            ArgMax mine_slot = none;
+            if (threadIdx.x < groups) {
+                const int64_t t = threadIdx.x;
+                mine_slot = ArgMax{pv[4 * t], pi[t], d3{pv[4 * t + 1], pv[4 * t + 2], pv[4 * t + 3]}};
+            }
+            best = block_argmax(mine_slot, lds);
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) out[s] = static_cast<int32_t>(best.index);
-        const d3 c = ld3(p, best.index);
-        local = ArgMax{-1.0, 0x7fffffffffffffffll};
+        const d3 c = best.point;
+        local = none;
 #pragma unroll
         for (int k = 0; k < kFpsPerThread; ++k) {
             if (nearest[k] >= 0.0) {
                 const double d = norm2(mine[k] - c);
                 nearest[k] = fmin(nearest[k], d);
                 const int64_t i = begin + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
-                local = better(local, ArgMax{nearest[k], i});
+                local = better(local, ArgMax{nearest[k], i, mine[k]});
             }
         }
     }
 }
 
-// Nearest `support` nodes by (d^2, index) with the insertion order of the CPU
-// route's keep_nearest, then (1 - d/dmax)^2 weights normalised.
-__device__ inline void nearest_nodes(d3 x, const double* __restrict__ nodes, int64_t nn, int support,
-                                     double* d2, int32_t* id) {
-    int held = 0;
-    for (int64_t j = 0; j < nn; ++j) {
-        const double s = norm2(x - ld3(nodes, j));
-        if (held == support) {
-            const bool beats = s < d2[support - 1] || (s == d2[support - 1] && j < id[support - 1]);
-            if (!beats) continue;
-        }
-        int at = held < support ? held++ : support - 1;
-        while (at > 0 && (d2[at - 1] > s || (d2[at - 1] == s && id[at - 1] > j))) {
-            d2[at] = d2[at - 1];
-            id[at] = id[at - 1];
-            --at;
-        }
-        d2[at] = s;
-        id[at] = static_cast<int32_t>(j);
+// Sorted insertion of (d, j) into the KMAX best entries kept in ascending
+// (d, index) order: the candidate bubbles down the fully unrolled list, so
+// every index is a compile-time constant and the list stays in registers.
+// Entries start at (+inf, INT32_MAX), which every real candidate beats.
+template <int KMAX>
+__device__ inline void insert_sorted(double d, int32_t j, double* dist, int32_t* id) {
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        const bool beats = d < dist[m] || (d == dist[m] && j < id[m]);
+        const double td = dist[m];
+        const int32_t tj = id[m];
+        dist[m] = beats ? d : td;
+        id[m] = beats ? j : tj;
+        d = beats ? td : d;
+        j = beats ? tj : j;
     }
 }
 
+// Nearest 4 nodes by (d^2, index), the order the CPU route's keep_nearest
+// produces (candidates visited in increasing index, ties kept by the lower
+// index); callers use the first `support` of them. Nodes are staged through
+// LDS a tile at a time by the whole block, so every thread of the block must
+// call it, including those past the end of the rows.
+template <int Threads>
+__device__ inline void nearest4(d3 x, const double* __restrict__ nodes, int64_t nn, double* d2, int32_t* id,
+                                double* tile) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        d2[m] = __builtin_inf();
+        id[m] = 0x7fffffff;
+    }
+    for (int64_t base = 0; base < nn; base += Threads) {
+        __syncthreads();
+        const int64_t j = base + threadIdx.x;
+        if (j < nn) {
+            tile[3 * threadIdx.x] = nodes[3 * j];
+            tile[3 * threadIdx.x + 1] = nodes[3 * j + 1];
+            tile[3 * threadIdx.x + 2] = nodes[3 * j + 2];
+        }
+        __syncthreads();
+        const int count = static_cast<int>(nn - base < Threads ? nn - base : Threads);
+        for (int t = 0; t < count; ++t) {
+            const d3 g{tile[3 * t], tile[3 * t + 1], tile[3 * t + 2]};
+            const double s = norm2(x - g);
+            if (s < d2[3] || (s == d2[3] && base + t < id[3]))
+                insert_sorted<4>(s, static_cast<int32_t>(base + t), d2, id);
+        }
+    }
+}
+
+__device__ inline double pick4(const double* v, int k) {
+    return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
+
+// (1 - d/dmax)^2 over the first `support` entries, normalised (uniform when
+// they all vanish); entries past `support` get weight 0.
 __device__ inline void distance_weights(const double* d2, int support, double* w) {
-    const double dmax = sqrt(d2[support - 1]);
+    const double dmax = sqrt(pick4(d2, support - 1));
     double total = 0.0;
-    for (int k = 0; k < support; ++k) {
-        const double d = sqrt(d2[k]);
-        const double ratio = dmax > 0.0 ? d / dmax : 0.0;
-        const double f = 1.0 - ratio;
-        w[k] = f * f;
-        total += w[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        w[k] = 0.0;
+        if (k < support) {
+            const double d = sqrt(d2[k]);
+            const double ratio = dmax > 0.0 ? d / dmax : 0.0;
+            const double f = 1.0 - ratio;
+            w[k] = f * f;
+            total += w[k];
+        }
     }
-    for (int k = 0; k < support; ++k) w[k] = total > 0.0 ? w[k] / total : 1.0 / support;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < support) w[k] = total > 0.0 ? w[k] / total : 1.0 / support;
 }
 
-__global__ void bind_kernel(const double* __restrict__ p, int64_t n, const double* __restrict__ nodes, int64_t nn,
-                            int support, int32_t* __restrict__ on, double* __restrict__ ow, int32_t* __restrict__ oc) {
+constexpr int kBindThreads = 128;
+
+__global__ void __launch_bounds__(kBindThreads)
+bind_kernel(const double* __restrict__ p, int64_t n, const double* __restrict__ nodes, int64_t nn, int support,
+            int32_t* __restrict__ on, double* __restrict__ ow, int32_t* __restrict__ oc) {
+    __shared__ double tile[3 * kBindThreads];
     const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
+    const d3 x = v < n ? ld3(p, v) : d3{0.0, 0.0, 0.0};
     double d2[4];
     int32_t id[4];
-    nearest_nodes(ld3(p, v), nodes, nn, support, d2, id);
+    nearest4<kBindThreads>(x, nodes, nn, d2, id, tile);
+    if (v >= n) return;
     double w[4];
     distance_weights(d2, support, w);
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         on[4 * v + k] = k < support ? id[k] : -1;
         ow[4 * v + k] = k < support ? w[k] : 0.0;
@@ -273,31 +344,45 @@ __global__ void bind_kernel(const double* __restrict__ p, int64_t n, const doubl
 }
 
 // k nearest other nodes by (Euclidean distance, index); emits the (min, max)
-// pair of each as a sortable key.
-__global__ void knn_kernel(const double* __restrict__ nodes, int64_t nn, int k, uint64_t* __restrict__ keys) {
+// pair of each as a sortable key. KMAX >= k entries are kept (the first k are
+// the answer); node positions are staged through LDS a tile at a time.
+constexpr int kKnnThreads = 64;
+template <int KMAX>
+__global__ void __launch_bounds__(kKnnThreads)
+knn_kernel(const double* __restrict__ nodes, int64_t nn, int k, uint64_t* __restrict__ keys) {
+    __shared__ double tile[3 * kKnnThreads];
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nn) return;
-    double dist[16];
-    int32_t id[16];
-    int held = 0;
-    const d3 x = ld3(nodes, i);
-    for (int64_t j = 0; j < nn; ++j) {
-        if (j == i) continue;
-        const double s = sqrt(norm2(x - ld3(nodes, j)));
-        if (held == k) {
-            const bool beats = s < dist[k - 1] || (s == dist[k - 1] && j < id[k - 1]);
-            if (!beats) continue;
-        }
-        int at = held < k ? held++ : k - 1;
-        while (at > 0 && (dist[at - 1] > s || (dist[at - 1] == s && id[at - 1] > j))) {
-            dist[at] = dist[at - 1];
-            id[at] = id[at - 1];
-            --at;
-        }
-        dist[at] = s;
-        id[at] = static_cast<int32_t>(j);
+    double dist[KMAX];
+    int32_t id[KMAX];
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        dist[m] = __builtin_inf();
+        id[m] = 0x7fffffff;
     }
-    for (int m = 0; m < k; ++m) {
+    const d3 x = i < nn ? ld3(nodes, i) : d3{0.0, 0.0, 0.0};
+    for (int64_t base = 0; base < nn; base += kKnnThreads) {
+        __syncthreads();
+        const int64_t j = base + threadIdx.x;
+        if (j < nn) {
+            tile[3 * threadIdx.x] = nodes[3 * j];
+            tile[3 * threadIdx.x + 1] = nodes[3 * j + 1];
+            tile[3 * threadIdx.x + 2] = nodes[3 * j + 2];
+        }
+        __syncthreads();
+        const int count = static_cast<int>(nn - base < kKnnThreads ? nn - base : kKnnThreads);
+        for (int t = 0; t < count; ++t) {
+            if (base + t == i) continue;
+            const d3 g{tile[3 * t], tile[3 * t + 1], tile[3 * t + 2]};
+            const double s = sqrt(norm2(x - g));
+            if (s < dist[KMAX - 1] || (s == dist[KMAX - 1] && base + t < id[KMAX - 1]))
+                insert_sorted<KMAX>(s, static_cast<int32_t>(base + t), dist, id);
+        }
+    }
+    if (i >= nn) return;
+    const int held = static_cast<int>(nn - 1 < k ? nn - 1 : k);
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        if (m >= k) break;
         uint64_t key = ~0ull;
         if (m < held) {
             const uint64_t a = static_cast<uint64_t>(i < id[m] ? i : id[m]);
@@ -320,28 +405,33 @@ __global__ void unique_edges(const uint64_t* __restrict__ keys, const int32_t* _
 // Blend of the bound source rotations in quaternion space, signs aligned to
 // the heaviest contributor, heaviest-neighbour fallback when the blend nearly
 // cancels; translation maps the new rest position onto the field's value.
-__global__ void transfer_kernel(const double* __restrict__ from, int64_t nf, const double* __restrict__ R,
+__global__ void __launch_bounds__(kBindThreads)
+transfer_kernel(const double* __restrict__ from, int64_t nf, const double* __restrict__ R,
                                 const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
                                 double* __restrict__ Ro, double* __restrict__ To) {
+    __shared__ double tile[3 * kBindThreads];
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    const d3 x = ld3(to, i);
+    const d3 x = i < nt ? ld3(to, i) : d3{0.0, 0.0, 0.0};
     double d2[4];
     int32_t id[4];
-    nearest_nodes(x, from, nf, support, d2, id);
+    nearest4<kBindThreads>(x, from, nf, d2, id, tile);
+    if (i >= nt) return;
     double w[4];
     distance_weights(d2, support, w);
     int32_t heavy = id[0];
     double hw = w[0];
-    for (int s = 1; s < support; ++s)
-        if (w[s] > hw) {
+#pragma unroll
+    for (int s = 1; s < 4; ++s)
+        if (s < support && w[s] > hw) {
             hw = w[s];
             heavy = id[s];
         }
     const quat ref = quat_from_matrix(R + 9 * static_cast<int64_t>(heavy));
     quat blend{0.0, 0.0, 0.0, 0.0};
     d3 pos{0.0, 0.0, 0.0};
-    for (int s = 0; s < support; ++s) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s >= support) break;
         const int32_t j = id[s];
         quat q = quat_from_matrix(R + 9 * static_cast<int64_t>(j));
         if (qdot(q, ref) < 0.0) q = quat{-q.w, -q.x, -q.y, -q.z};
@@ -393,13 +483,15 @@ void farthest_point_seeds(const double* points, int64_t n, int32_t seeds, int32_
     const int64_t resident = static_cast<int64_t>(cus) * std::max(per_cu, 1);
     if (groups64 > resident)
         throw InvalidArgument("farthest_point_seeds: " + std::to_string(n) + " points exceed the resident capacity");
+    if (groups64 > kFpsThreads)
+        throw InvalidArgument("farthest_point_seeds: " + std::to_string(n) + " points exceed one persistent launch");
     const unsigned groups = static_cast<unsigned>(groups64);
     const int64_t chunk = per_group;
-    partial.resize(4 * static_cast<int64_t>(groups));  // 2 buffers x (value, index)
+    partial.resize(10 * static_cast<int64_t>(groups));  // 2 buffers x (value, x, y, z, index)
     barrier.resize(2);
     barrier.zero(stream);
     double* pv = partial.get();
-    int64_t* pi = reinterpret_cast<int64_t*>(partial.get() + 2 * groups);
+    int64_t* pi = reinterpret_cast<int64_t*>(partial.get() + 8 * groups);
     int32_t s = seeds;
     int32_t f = first;
     int64_t nn = n;
@@ -413,7 +505,7 @@ void bind_vertices(const double* points, int64_t n, const double* nodes, int64_t
                    double* ow, int32_t* oc, hipStream_t stream) {
     if (n <= 0 || nn <= 0) return;
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nn, 4)), 4}));
-    bind_kernel<<<blocks_for(n, 128), 128, 0, stream>>>(points, n, nodes, nn, support, on, ow, oc);
+    bind_kernel<<<blocks_for(n, kBindThreads), kBindThreads, 0, stream>>>(points, n, nodes, nn, support, on, ow, oc);
     NRICP_LAUNCH_CHECK("bind_kernel");
 }
 
@@ -428,7 +520,13 @@ int64_t build_node_edges(const double* nodes, int64_t nn, int32_t k, DeviceBuffe
     const int kk = static_cast<int>(std::min<int64_t>(k, nn - 1));
     const int64_t m = nn * kk;
     keys.resize(m);
-    knn_kernel<<<blocks_for(nn, 128), 128, 0, stream>>>(nodes, nn, kk, keys.get());
+    const dim3 grid(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
+    if (kk <= 4)
+        knn_kernel<4><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+    else if (kk <= 8)
+        knn_kernel<8><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+    else
+        knn_kernel<16><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
     NRICP_LAUNCH_CHECK("knn_kernel");
     radix_sort_pairs(keys.get(), nullptr, m, 64, scratch, stream);
     flags.resize(m + 1);
@@ -450,7 +548,7 @@ void transfer_level(const double* from, int64_t nf, const double* R, const doubl
                     int32_t k, double* Ro, double* To, hipStream_t stream) {
     if (nt <= 0) return;
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nf, 4)), 4}));
-    transfer_kernel<<<blocks_for(nt, 128), 128, 0, stream>>>(from, nf, R, T, to, nt, support, Ro, To);
+    transfer_kernel<<<blocks_for(nt, kBindThreads), kBindThreads, 0, stream>>>(from, nf, R, T, to, nt, support, Ro, To);
     NRICP_LAUNCH_CHECK("transfer_kernel");
 }
 

@@ -28,13 +28,17 @@ void vertex_normals(const double* points, const int32_t* faces, int64_t face_cou
 
 // --------------------------------------------------------------- LBVH ------
 
+// Child boxes are FP32, rounded outward from the exact FP64 triangle bounds,
+// so each contains its subtree exactly and a node fits one 64-byte line; the
+// pruning distance to such a box never exceeds the distance to the triangles
+// inside it, and the triangle tests themselves run in FP64.
 struct BvhNode {
-    double box[12];  // child 0: lo xyz, hi xyz; child 1: lo xyz, hi xyz
+    float box[12];  // child 0: lo xyz, hi xyz; child 1: lo xyz, hi xyz
     int32_t child[2];  // >= 0 internal node, < 0 leaf ~index
     int32_t parent;
     int32_t pad;
 };
-static_assert(sizeof(BvhNode) == 112, "node layout");
+static_assert(sizeof(BvhNode) == 64, "node layout");
 
 struct Lbvh {
     int64_t triangle_count = 0;
@@ -67,10 +71,17 @@ struct QuerySettings {
 // distance, normal, slope) and the IRLS weight. Masked queries are not
 // searched. `query_normals` may be null (no normal filter). `warm`, when
 // given, holds each query's previous triangle (may alias out_triangle).
+// `order`, when given, is a permutation of the queries in which they are
+// processed (spatially coherent orders keep a wave's traversals together);
+// results are written at each query's own index either way.
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
-                    hipStream_t stream, const int32_t* warm = nullptr);
+                    hipStream_t stream, const int32_t* warm = nullptr, const int32_t* order = nullptr);
+// Permutation of `count` points sorted by the Morton code (10 bits per axis)
+// of their position in the points' bounding box, ties by index.
+void morton_order(const double* points, int64_t count, int32_t* out_order, SortScratch& scratch,
+                  DeviceBuffer<uint64_t>& keys, DeviceBuffer<double>& bounds, hipStream_t stream);
 
 // counts[0..5) = accepted, distance, normal, slope, masked; sums[0] = sum of
 // accepted distances. Deterministic two-stage reduction.

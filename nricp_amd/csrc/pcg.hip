@@ -4,22 +4,23 @@
 // Reference behaviour: replaces src/cuda/pcg_solver.cu:15-326 (block-Jacobi
 // PCG from x0 = 0, stop when |r|/|b| <= tol, ok <=> converged). Differences,
 // chosen for gfx950:
-//   * Chronopoulos-Gear recurrence: both inner products of an iteration are
-//     reduced together, so an iteration costs two grid barriers, not four;
+//   * pipelined CG recurrence (Ghysels and Vanroose): the inner products and
+//     the matrix product of an iteration are independent, so an iteration
+//     costs one grid barrier, not four;
 //   * the matrix is block-sparse rows of the full symmetric system, so the
-//     product is a gather with no atomics (one lane per block row of a node,
-//     eight lanes per node, six active);
-//   * each lane keeps its rows of x, r, p, s in registers for the whole solve;
-//     only the preconditioned residual u crosses lanes (through memory, for
-//     the neighbours' products);
+//     product is a gather with no atomics: the eight lanes of a node split its
+//     blocks and sum their partial products with wave shuffles;
+//   * each lane keeps its rows of every CG vector and of the preconditioner in
+//     registers for the whole solve; only the vector the product needs
+//     crosses lanes (through memory, for the neighbours' products);
 //   * damping is the CPU route's (src/solver/normal_equations.cpp:81-96):
 //     lambda * max(A_ii, 1e-12) with diagonal scaling, lambda otherwise;
 //   * inner products and the refinement residual are accumulated in FP64 and
 //     reduced in a fixed order, so a solve is reproducible bit for bit;
-//   * mixed mode runs the iteration in FP32 on an FP32 matrix and refines in
-//     FP64 inside the same launch: x += e, r = b - A x evaluated in FP64
-//     arithmetic, restart, until the FP64 relative residual reaches the
-//     tolerance.
+//   * convergence is decided on the true residual b - A x, evaluated in FP64;
+//     mixed mode runs the iteration in FP32 on an FP32 matrix and refines in
+//     FP64 inside the same launch: x += e, r = b - A x, restart, until the
+//     FP64 relative residual reaches the tolerance.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -30,53 +31,92 @@ namespace {
 constexpr int kPcgThreads = 512;
 constexpr int kNodesPerPass = kPcgThreads / 8;  // 8 lanes per node, 6 active
 
-// Damped diagonal blocks inverted once per solve (Gauss-Jordan with partial
-// pivoting; a singular block falls back to the identity as the reference's
-// factor kernel does).
+// Inverse of the symmetric 6x6 block `a` (lower triangle read) plus `jitter`
+// on its diagonal, through its Cholesky factor: L L^T = a, X = L^-1,
+// inverse = X^T X. Fully unrolled, so the block never leaves registers.
+// Returns false when a pivot is not positive.
+__device__ inline bool spd_inverse6(const double (&a)[36], double jitter, double (&inv)[36]) {
+    double L[36];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = a[6 * j + j] + jitter;
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k];
+        ok = ok && d > 0.0;
+        const double ljj = sqrt(d > 0.0 ? d : 1.0);
+        L[6 * j + j] = ljj;
+        const double r = 1.0 / ljj;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double t = a[6 * i + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= L[6 * i + k] * L[6 * j + k];
+            L[6 * i + j] = t * r;
+        }
+    }
+    double X[36];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        X[6 * j + j] = 1.0 / L[6 * j + j];
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = j; k < i; ++k) t -= L[6 * i + k] * X[6 * k + j];
+            X[6 * i + j] = t / L[6 * i + i];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+#pragma unroll
+        for (int c = r; c < 6; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = c; k < 6; ++k) t += X[6 * k + r] * X[6 * k + c];
+            inv[6 * r + c] = t;
+            inv[6 * c + r] = t;
+        }
+    }
+    return ok;
+}
+
+// Damping of the system (the CPU route's lambda * max(A_ii, 1e-12) with
+// diagonal scaling, lambda otherwise) and the block-Jacobi preconditioner:
+// the inverse of each damped diagonal block by Cholesky, retried with a
+// growing diagonal jitter and replaced by the identity when the block is not
+// positive definite (the recovery of the reference's factor kernel,
+// src/cuda/pcg_solver.cu:15-46). The jitter only shapes the preconditioner;
+// the system keeps its damping.
 template <typename S>
 __global__ void precondition_kernel(const S* __restrict__ bsr, const int32_t* __restrict__ diag_pos, int64_t nn,
                                     double lambda, int scaling, S* __restrict__ damp, S* __restrict__ minv) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nn) return;
     const S* blk = bsr + 36 * static_cast<int64_t>(diag_pos[i]);
-    double a[6][12];
+    double a[36];
+#pragma unroll
     for (int r = 0; r < 6; ++r) {
-        for (int c = 0; c < 6; ++c) {
-            a[r][c] = static_cast<double>(blk[6 * r + c]);
-            a[r][6 + c] = r == c ? 1.0 : 0.0;
-        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[6 * r + c] = static_cast<double>(blk[6 * r + c]);
         double d = 0.0;
-        if (lambda > 0.0) d = lambda * (scaling ? fmax(a[r][r], 1e-12) : 1.0);
+        if (lambda > 0.0) d = lambda * (scaling ? fmax(a[6 * r + r], 1e-12) : 1.0);
         damp[6 * i + r] = static_cast<S>(d);
-        a[r][r] += d;
+        a[6 * r + r] += d;
     }
-    bool ok = true;
-    for (int c = 0; c < 6 && ok; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < 6; ++r)
-            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
-        if (!(fabs(a[piv][c]) > 0.0)) {
-            ok = false;
-            break;
-        }
-        if (piv != c)
-            for (int k = 0; k < 12; ++k) {
-                const double t = a[c][k];
-                a[c][k] = a[piv][k];
-                a[piv][k] = t;
-            }
-        const double inv = 1.0 / a[c][c];
-        for (int k = 0; k < 12; ++k) a[c][k] *= inv;
-        for (int r = 0; r < 6; ++r) {
-            if (r == c) continue;
-            const double f = a[r][c];
-            if (f == 0.0) continue;
-            for (int k = 0; k < 12; ++k) a[r][k] -= f * a[c][k];
-        }
+    double inv[36];
+    bool ok = spd_inverse6(a, 0.0, inv);
+    double jitter = 1e-6;
+#pragma unroll 1
+    for (int attempt = 1; attempt < 8 && !ok; ++attempt) {
+        ok = spd_inverse6(a, jitter, inv);
+        jitter *= 10.0;
     }
+#pragma unroll
     for (int r = 0; r < 6; ++r)
+#pragma unroll
         for (int c = 0; c < 6; ++c)
-            minv[36 * i + 6 * r + c] = static_cast<S>(ok ? a[r][6 + c] : (r == c ? 1.0 : 0.0));
+            minv[36 * i + 6 * r + c] = static_cast<S>(ok ? inv[6 * r + c] : (r == c ? 1.0 : 0.0));
 }
 
 template <bool Grid>
@@ -87,196 +127,280 @@ __device__ inline void sync_all(unsigned* barrier) {
         __syncthreads();
 }
 
-// Sum of 3 FP64 partials over the block, then over all blocks in index order.
-template <bool Grid>
-__device__ inline void reduce3(double a, double b, double c, double* slots, double* lds, unsigned* barrier,
-                               double (&out)[3]) {
-    double v[3] = {a, b, c}, s[3];
-    block_sum<3, kPcgThreads>(v, lds, s);
+// Reproducible sums over the whole launch. publish(): the block's sum of the
+// per-thread partials goes to the block's slot. After a grid barrier, fold():
+// thread t loads slot t and the block adds them in its fixed tree order, so
+// every block obtains the same bits. Needs gridDim.x <= kPcgThreads.
+template <int Count>
+__device__ inline void publish(const double (&v)[Count], double* slots, double* lds) {
+    double blk[Count];
+    block_sum<Count, kPcgThreads>(v, lds, blk);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < Count; ++k) slots[Count * blockIdx.x + k] = blk[k];
+}
+template <int Count>
+__device__ inline void fold(const double* slots, double* lds, double (&out)[Count]) {
+    double v[Count];
+#pragma unroll
+    for (int k = 0; k < Count; ++k) v[k] = threadIdx.x < gridDim.x ? slots[Count * threadIdx.x + k] : 0.0;
+    block_sum<Count, kPcgThreads>(v, lds, out);
+}
+
+// One launch-wide sum with its own synchronisation (used where no vector
+// exchange shares the barrier).
+template <int Count, bool Grid>
+__device__ inline void launch_sum(const double (&v)[Count], double* slots, double* lds, unsigned* barrier,
+                                  double (&out)[Count]) {
     if (!Grid) {
-        out[0] = s[0];
-        out[1] = s[1];
-        out[2] = s[2];
+        block_sum<Count, kPcgThreads>(v, lds, out);
         return;
     }
-    if (threadIdx.x == 0) {
-        slots[3 * blockIdx.x] = s[0];
-        slots[3 * blockIdx.x + 1] = s[1];
-        slots[3 * blockIdx.x + 2] = s[2];
-    }
+    publish<Count>(v, slots, lds);
     grid_barrier(barrier, gridDim.x);
-    out[0] = out[1] = out[2] = 0.0;
-    for (unsigned g = 0; g < gridDim.x; ++g) {
-        out[0] += slots[3 * g];
-        out[1] += slots[3 * g + 1];
-        out[2] += slots[3 * g + 2];
-    }
+    fold<Count>(slots, lds, out);
 }
 
-// One row of (A + diag(damp)) u for a node: the node's blocks are
-// contiguous, and the six lanes of the node read one 36-entry block together.
-// Acc is the accumulation precision (the matrix precision for the iteration,
-// FP64 for the refinement residual).
-template <typename S, typename V, typename Acc>
-__device__ inline Acc row_product(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
-                                  const S* __restrict__ bsr, const S* __restrict__ damp, const V* __restrict__ u,
-                                  int64_t node, int row) {
-    Acc acc = Acc(0);
-    for (int32_t b = row_off[node]; b < row_off[node + 1]; ++b) {
-        const S* blk = bsr + 36 * static_cast<int64_t>(b) + 6 * row;
-        const V* x = u + 6 * static_cast<int64_t>(cols[b]);
-        Acc t = Acc(0);
+// Row `lane` of the node's preconditioner block times the node's 6-vector,
+// which is held one row per lane: the six values come from the node's lanes
+// by shuffles. Every lane must call it.
+template <typename S>
+__device__ inline S precondition_row(const S (&mrow)[6], S v, int lane_base) {
+    double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) t += static_cast<Acc>(blk[c]) * static_cast<Acc>(x[c]);
-        acc += t;
-    }
-    return acc + static_cast<Acc>(damp[6 * node + row]) * static_cast<Acc>(u[6 * node + row]);
+    for (int c = 0; c < 6; ++c)
+        acc += static_cast<double>(mrow[c]) * static_cast<double>(__shfl(v, lane_base + c, 64));
+    return static_cast<S>(acc);
 }
 
-// S: matrix / iteration precision. Refine: FP64 outer refinement (S = float).
-template <int K, typename S, bool Refine, bool Grid>
+// (A + diag(damp)) u for the six rows of one node, evaluated by the node's
+// eight lanes together: lane j multiplies blocks j, j + 8, ... of the node's
+// block row (the full 6x6 block times the column node's 6-vector), and the
+// eight partial
+// 6-vectors are summed across the lanes in a fixed butterfly order. Every lane
+// of the wave must call it (the sum is a wave shuffle); `valid` false adds
+// zeros. Returns row `lane` of the product (lanes 6 and 7 get row 0). Acc is
+// the accumulation precision (the matrix precision for the iteration, FP64
+// for the refinement residual).
+template <typename S, typename V, typename Acc>
+__device__ inline Acc node_product(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
+                                   const S* __restrict__ bsr, const S* __restrict__ damp, const V* __restrict__ u,
+                                   int64_t node, bool valid, int lane) {
+    Acc acc[6] = {Acc(0), Acc(0), Acc(0), Acc(0), Acc(0), Acc(0)};
+    if (valid) {
+        const int32_t end = row_off[node + 1];
+        for (int32_t b = row_off[node] + lane; b < end; b += 8) {
+            const S* blk = bsr + 36 * static_cast<int64_t>(b);
+            const V* x = u + 6 * static_cast<int64_t>(cols[b]);
+            Acc xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) xv[c] = static_cast<Acc>(x[c]);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                Acc t = Acc(0);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) t += static_cast<Acc>(blk[6 * r + c]) * xv[c];
+                acc[r] += t;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        acc[r] += __shfl_xor(acc[r], 1, 64);
+        acc[r] += __shfl_xor(acc[r], 2, 64);
+        acc[r] += __shfl_xor(acc[r], 4, 64);
+    }
+    Acc out = acc[0];
+#pragma unroll
+    for (int r = 1; r < 6; ++r) out = lane == r ? acc[r] : out;
+    if (valid && lane < 6)
+        out += static_cast<Acc>(damp[6 * node + lane]) * static_cast<Acc>(u[6 * node + lane]);
+    return out;
+}
+
+// Preconditioned pipelined CG (Ghysels and Vanroose, 2014): the iteration's
+// two inner products and its matrix product are independent, so the
+// preconditioned vector m = M w and the partial sums are published together
+// and a single grid barrier per iteration serves both. Each inner cycle runs
+// in S arithmetic (FP32 or FP64) on the S matrix until the recurrence residual
+// falls below max(inner_tol * |r_cycle|, tol * |b|); then x accumulates into
+// FP64 and the true residual b - A x is evaluated in FP64. The solve ends when
+// that true residual meets tol * |b|; otherwise the next cycle restarts from
+// it (iterative refinement in mixed precision, residual replacement in FP64).
+//
+// Exchange buffers are double-buffered by barrier parity: data written before
+// barrier j is read after it, and the same half is next written after barrier
+// j + 1, when every block has finished reading it.
+template <int K, typename S, bool Grid>
 __global__ void __launch_bounds__(kPcgThreads)
 pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
            const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ rhs, int64_t nn,
-           double tol, double inner_tol, int max_iter, S* __restrict__ ubuf, double* __restrict__ xbuf,
-           double* __restrict__ slots, unsigned* __restrict__ barrier, double* __restrict__ solution,
-           double* __restrict__ outcome) {
+           double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
+           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
+           unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
     __shared__ double lds[3 * (kPcgThreads / kWave)];
     const int lane = threadIdx.x & 7;
     const int lane_base = (threadIdx.x & 63) & ~7;
     const bool row_lane = lane < 6;
+    const int64_t stride = 6 * nn;
     int64_t node[K];
-    bool own[K];
+    bool own[K], live[K];
+    S mrow[K][6];
+    double b64[K], x64[K];
+    S x[K], r[K], u[K], w[K], p[K], s[K], q[K], z[K];
+    double bb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         node[k] = (static_cast<int64_t>(blockIdx.x) * K + k) * kNodesPerPass + (threadIdx.x >> 3);
-        own[k] = row_lane && node[k] < nn;
-    }
-    double b64[K], x64[K];
-    S x[K], r[K], p[K], s[K], u[K], w[K];
-    double bb_local = 0.0;
+        live[k] = node[k] < nn;
+        own[k] = row_lane && live[k];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
+        for (int c = 0; c < 6; ++c) mrow[k][c] = own[k] ? minv[36 * node[k] + 6 * lane + c] : S(0);
         b64[k] = own[k] ? rhs[6 * node[k] + lane] : 0.0;
         x64[k] = 0.0;
-        x[k] = p[k] = s[k] = u[k] = w[k] = S(0);
         r[k] = static_cast<S>(b64[k]);
-        bb_local += b64[k] * b64[k];
+        x[k] = u[k] = w[k] = p[k] = s[k] = q[k] = z[k] = S(0);
+        bb += b64[k] * b64[k];
     }
+    int phase = 0;
     double red[3];
-    double* slot_a = slots;
-    double* slot_b = slots + 3 * gridDim.x;
-    reduce3<Grid>(bb_local, 0.0, 0.0, slot_a, lds, barrier, red);
+    {
+        const double v[1] = {bb};
+        double o[1];
+        launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+        phase ^= 1;
+        red[0] = o[0];
+    }
     const double bnorm = sqrt(red[0]);
-    double target = tol * bnorm;
-    double inner_target = Refine ? inner_tol * bnorm : target;
-    double gamma_prev = 1.0, alpha_prev = 1.0;
-    int it = 0, cycle_it = 0;
+    const double target = tol * bnorm;
+    double rel = bnorm > 0.0 ? 1.0 : 0.0;
+    double cycle_norm = bnorm;
     bool converged = bnorm == 0.0, breakdown = false;
-    double rel = bnorm == 0.0 ? 0.0 : 1.0;
-    int parity = 1;
+    int it = 0;
     while (!converged && it < max_iter) {
-        // u = M^-1 r (rows of the node exchanged through lane shuffles).
-        double gamma_l = 0.0, rho_l = 0.0;
+        const double inner_target = fmax(inner_tol * cycle_norm, target);
+        // Cycle start: u = M r, w = A u.
+        S* ex = vbuf + phase * stride;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            // Preconditioner row read per iteration (it stays in cache); the
-            // node's residual rows come from the neighbouring lanes.
-            const S* mrow = minv + 36 * (own[k] ? node[k] : 0) + 6 * (row_lane ? lane : 0);
-            double uu = 0.0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const double rc = static_cast<double>(__shfl(r[k], lane_base + c, 64));
-                uu += static_cast<double>(mrow[c]) * rc;
-            }
-            u[k] = own[k] ? static_cast<S>(uu) : S(0);
-            if (own[k]) ubuf[6 * node[k] + lane] = u[k];
-            gamma_l += static_cast<double>(r[k]) * static_cast<double>(u[k]);
-            rho_l += static_cast<double>(r[k]) * static_cast<double>(r[k]);
+            const S v = precondition_row(mrow[k], r[k], lane_base);
+            u[k] = own[k] ? v : S(0);
+            if (own[k]) ex[6 * node[k] + lane] = u[k];
+            x[k] = p[k] = s[k] = q[k] = z[k] = S(0);
         }
         sync_all<Grid>(barrier);
-        double delta_l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (own[k]) {
-                w[k] = row_product<S, S, S>(row_off, cols, bsr, damp, ubuf, node[k], lane);
-                delta_l += static_cast<double>(w[k]) * static_cast<double>(u[k]);
-            }
+            const S v = node_product<S, S, S>(row_off, cols, bsr, damp, ex, node[k], live[k], lane);
+            w[k] = own[k] ? v : S(0);
         }
-        double* sl = parity ? slot_b : slot_a;
-        parity ^= 1;
-        reduce3<Grid>(gamma_l, rho_l, delta_l, sl, lds, barrier, red);
-        const double gamma = red[0], rho = red[1], delta = red[2];
-        const double rnorm = sqrt(rho);
-        if (!Refine) rel = bnorm > 0.0 ? rnorm / bnorm : 0.0;
-        if (rnorm <= inner_target) {
-            if (!Refine) {
-                converged = true;
+        phase ^= 1;
+        double gamma_prev = 1.0, alpha_prev = 1.0;
+        int cycle_it = 0;
+        bool inner_done = false;
+        while (it < max_iter) {
+            S m[K];
+            double part[3] = {0.0, 0.0, 0.0};
+            ex = vbuf + phase * stride;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const S v = precondition_row(mrow[k], w[k], lane_base);
+                m[k] = own[k] ? v : S(0);
+                if (own[k]) ex[6 * node[k] + lane] = m[k];
+                const double rk = static_cast<double>(r[k]), uk = static_cast<double>(u[k]);
+                part[0] += rk * uk;
+                part[1] += static_cast<double>(w[k]) * uk;
+                part[2] += rk * rk;
+            }
+            if (Grid) {
+                publish<3>(part, slots + phase * 3 * gridDim.x, lds);
+                grid_barrier(barrier, gridDim.x);
+                fold<3>(slots + phase * 3 * gridDim.x, lds, red);
+            } else {
+                block_sum<3, kPcgThreads>(part, lds, red);
+            }
+            const double gamma = red[0], delta = red[1], rnorm = sqrt(red[2]);
+            rel = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+            if (rnorm <= inner_target) {
+                phase ^= 1;
+                inner_done = true;
                 break;
             }
-            // Refinement: x64 += x, r64 = b - A x64 in FP64 arithmetic.
+            S n[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                x64[k] += static_cast<double>(x[k]);
-                if (own[k]) xbuf[6 * node[k] + lane] = x64[k];
+                const S v = node_product<S, S, S>(row_off, cols, bsr, damp, ex, node[k], live[k], lane);
+                n[k] = own[k] ? v : S(0);
             }
-            sync_all<Grid>(barrier);
-            double rr_l = 0.0;
-            double r64[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                r64[k] = own[k] ? b64[k] - row_product<S, double, double>(row_off, cols, bsr, damp, xbuf, node[k], lane) : 0.0;
-                rr_l += r64[k] * r64[k];
+            phase ^= 1;
+            double beta, alpha;
+            if (cycle_it == 0) {
+                beta = 0.0;
+                alpha = delta != 0.0 ? gamma / delta : 0.0;
+            } else {
+                beta = gamma / gamma_prev;
+                alpha = gamma / (delta - beta * gamma / alpha_prev);
             }
-            double* sl2 = parity ? slot_b : slot_a;
-            parity ^= 1;
-            reduce3<Grid>(rr_l, 0.0, 0.0, sl2, lds, barrier, red);
-            const double r64norm = sqrt(red[0]);
-            rel = bnorm > 0.0 ? r64norm / bnorm : 0.0;
-            if (r64norm <= target) {
-                converged = true;
+            if (!(alpha > 0.0) || !isfinite(alpha)) {
+                breakdown = true;
                 break;
             }
-            // Restart the inner iteration on the FP64 residual.
-            inner_target = inner_tol * r64norm;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                r[k] = static_cast<S>(r64[k]);
-                x[k] = p[k] = s[k] = S(0);
+                z[k] = static_cast<S>(static_cast<double>(n[k]) + beta * static_cast<double>(z[k]));
+                q[k] = static_cast<S>(static_cast<double>(m[k]) + beta * static_cast<double>(q[k]));
+                s[k] = static_cast<S>(static_cast<double>(w[k]) + beta * static_cast<double>(s[k]));
+                p[k] = static_cast<S>(static_cast<double>(u[k]) + beta * static_cast<double>(p[k]));
+                x[k] = static_cast<S>(static_cast<double>(x[k]) + alpha * static_cast<double>(p[k]));
+                r[k] = static_cast<S>(static_cast<double>(r[k]) - alpha * static_cast<double>(s[k]));
+                u[k] = static_cast<S>(static_cast<double>(u[k]) - alpha * static_cast<double>(q[k]));
+                w[k] = static_cast<S>(static_cast<double>(w[k]) - alpha * static_cast<double>(z[k]));
             }
-            cycle_it = 0;
+            gamma_prev = gamma;
+            alpha_prev = alpha;
             ++it;
-            continue;
+            ++cycle_it;
         }
-        double beta, alpha;
-        if (cycle_it == 0) {
-            beta = 0.0;
-            alpha = delta != 0.0 ? gamma / delta : 0.0;
-        } else {
-            beta = gamma / gamma_prev;
-            alpha = gamma / (delta - beta * gamma / alpha_prev);
+        if (!inner_done) break;
+        // True residual of the accumulated FP64 solution.
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            x64[k] += static_cast<double>(x[k]);
+            x[k] = S(0);
+            if (own[k]) xbuf[6 * node[k] + lane] = x64[k];
         }
-        if (!(alpha > 0.0) || !isfinite(alpha)) {
-            breakdown = true;
+        sync_all<Grid>(barrier);
+        double rr = 0.0;
+        double r64[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double v = node_product<S, double, double>(row_off, cols, bsr, damp, xbuf, node[k], live[k], lane);
+            r64[k] = own[k] ? b64[k] - v : 0.0;
+            rr += r64[k] * r64[k];
+        }
+        {
+            const double v[1] = {rr};
+            double o[1];
+            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            phase ^= 1;
+            red[0] = o[0];
+        }
+        const double tnorm = sqrt(red[0]);
+        rel = bnorm > 0.0 ? tnorm / bnorm : 0.0;
+        if (tnorm <= target) {
+            converged = true;
             break;
         }
+        // A cycle that no longer reduces the true residual ends the solve.
+        if (!(tnorm < cycle_norm)) break;
+        cycle_norm = tnorm;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            p[k] = static_cast<S>(static_cast<double>(u[k]) + beta * static_cast<double>(p[k]));
-            s[k] = static_cast<S>(static_cast<double>(w[k]) + beta * static_cast<double>(s[k]));
-            x[k] = static_cast<S>(static_cast<double>(x[k]) + alpha * static_cast<double>(p[k]));
-            r[k] = static_cast<S>(static_cast<double>(r[k]) - alpha * static_cast<double>(s[k]));
-        }
-        gamma_prev = gamma;
-        alpha_prev = alpha;
-        ++it;
-        ++cycle_it;
+        for (int k = 0; k < K; ++k) r[k] = static_cast<S>(r64[k]);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!own[k]) continue;
-        solution[6 * node[k] + lane] = Refine ? x64[k] + (converged ? 0.0 : static_cast<double>(x[k]))
-                                              : static_cast<double>(x[k]);
+        solution[6 * node[k] + lane] = x64[k] + static_cast<double>(x[k]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         outcome[0] = it;
@@ -285,11 +409,6 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         outcome[3] = breakdown ? 1.0 : 0.0;
     }
 }
-
-struct Launch {
-    int k;
-    int groups;
-};
 
 int resident_groups(const void* kernel) {
     int device = 0;
@@ -301,62 +420,62 @@ int resident_groups(const void* kernel) {
     return cus * std::max(per_cu, 1);
 }
 
-template <typename S, bool Refine>
+// Node slots per lane group K: at most 2 (register budget: every CG vector
+// and the preconditioner rows stay in registers). Systems that fit one workgroup run without grid barriers; larger
+// ones spread one slot per lane group over as many groups as needed (at most
+// the co-resident count and kPcgThreads, which the fold of the partial sums
+// assumes), growing K beyond that.
+template <typename S>
 void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t nn, double lambda, bool scaling,
-                double tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp, S* minv, S* ubuf,
-                hipStream_t stream) {
+                double tol, double inner_tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp, S* minv,
+                DeviceBuffer<S>& vbuf, hipStream_t stream) {
     if (nn <= 0) return;
     precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, L.diag_position.get(), nn, lambda,
                                                                      scaling ? 1 : 0, damp, minv);
     NRICP_LAUNCH_CHECK("precondition_kernel");
-    // Node slots per group: 64 per K. Small systems run in one workgroup
-    // (no grid barrier); larger ones spread one K=1 slot per lane group
-    // over as many groups as are co-resident, growing K beyond that.
+    constexpr int kmax = 2;
     const int64_t passes = (nn + kNodesPerPass - 1) / kNodesPerPass;
     int k = 1;
     bool grid = true;
-    if (passes <= 4) {
-        k = passes <= 1 ? 1 : (passes <= 2 ? 2 : 4);
-        grid = false;
-    }
-    const void* kernel = nullptr;
-#define NRICP_PCG_PICK(KK, GG) reinterpret_cast<const void*>(pcg_kernel<KK, S, Refine, GG>)
     int groups = 1;
-    if (grid) {
-        const int resident = resident_groups(NRICP_PCG_PICK(1, true));
+    if (passes <= kmax) {
+        while (k < passes) k *= 2;
+        grid = false;
+    } else {
+        const int limit = std::min(resident_groups(reinterpret_cast<const void*>(pcg_kernel<1, S, true>)),
+                                   kPcgThreads);
         int64_t need = passes;
-        while (need > resident && k < 4) {
+        while (need > limit && k < kmax) {
             k *= 2;
             need = (passes + k - 1) / k;
         }
-        if (need > resident) throw InvalidArgument("linear system too large for one persistent launch");
+        if (need > limit) throw InvalidArgument("linear system too large for one persistent launch");
         groups = static_cast<int>(need);
-        if (groups == 1) grid = false;
     }
-    switch (k * 2 + (grid ? 1 : 0)) {
-        case 2: kernel = NRICP_PCG_PICK(1, false); break;
-        case 3: kernel = NRICP_PCG_PICK(1, true); break;
-        case 4: kernel = NRICP_PCG_PICK(2, false); break;
-        case 5: kernel = NRICP_PCG_PICK(2, true); break;
-        case 8: kernel = NRICP_PCG_PICK(4, false); break;
-        default: kernel = NRICP_PCG_PICK(4, true); break;
+    const void* kernel = nullptr;
+    if (grid) {
+        kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, true>)
+                        : reinterpret_cast<const void*>(pcg_kernel<2, S, true>);
+    } else {
+        kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, false>)
+                        : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
     }
-#undef NRICP_PCG_PICK
-    ws.reduce.resize(6 * static_cast<int64_t>(groups) + 6);
+    ws.reduce.resize(6 * static_cast<int64_t>(groups));
     ws.barrier.resize(2);
     ws.barrier.zero(stream);
     ws.outcome.resize(4);
     ws.xr.resize(6 * nn);
+    vbuf.resize(12 * nn);
     const int32_t* row_off = L.row_offsets.get();
     const int32_t* cols = L.columns.get();
-    double inner_tol = Refine ? std::max(tol, 1e-5) : tol;
     int64_t n = nn;
+    S* vb = vbuf.get();
     double* xbuf = ws.xr.get();
     double* slots = ws.reduce.get();
     unsigned* bar = ws.barrier.get();
     double* out = ws.outcome.get();
     void* args[] = {&row_off, &cols, const_cast<S**>(&bsr), &damp, &minv, const_cast<double**>(&rhs), &n, &tol,
-                    &inner_tol, &max_iter, &ubuf, &xbuf, &slots, &bar, &solution, &out};
+                    &inner_tol, &max_iter, &vb, &xbuf, &slots, &bar, &solution, &out};
     if (ws.kernel_start) NRICP_CHECK(hipEventRecord(ws.kernel_start, stream));
     if (grid) {
         NRICP_CHECK(hipLaunchCooperativeKernel(kernel, dim3(groups), dim3(kPcgThreads), args, 0, stream));
@@ -373,18 +492,18 @@ void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64
                double tol, int max_iter, double* solution, PcgWorkspace& ws, hipStream_t stream) {
     ws.damp.resize(6 * nn);
     ws.minv.resize(36 * nn);
-    ws.u.resize(6 * nn);
-    launch_pcg<double, false>(L, bsr, rhs, nn, lambda, scaling, tol, max_iter, solution, ws, ws.damp.get(),
-                              ws.minv.get(), ws.u.get(), stream);
+    launch_pcg<double>(L, bsr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
+                       ws.minv.get(), ws.u, stream);
 }
 
 void pcg_solve_mixed(const LevelLayout& L, const float* bsr, const double* rhs, int64_t nn, double lambda,
                      bool scaling, double tol, int max_iter, double* solution, PcgWorkspace& ws, hipStream_t stream) {
-    ws.u32.resize(6 * nn);
     ws.w32.resize(6 * nn);  // damping vector (FP32)
     ws.minv32.resize(36 * nn);
-    launch_pcg<float, true>(L, bsr, rhs, nn, lambda, scaling, tol, max_iter, solution, ws, ws.w32.get(),
-                            ws.minv32.get(), ws.u32.get(), stream);
+    // FP32 cycles stop at 1e-5 of the cycle's starting residual, well above
+    // where the FP32 recurrence loses accuracy; FP64 refinement does the rest.
+    launch_pcg<float>(L, bsr, rhs, nn, lambda, scaling, tol, std::max(tol, 1e-5), max_iter, solution, ws,
+                      ws.w32.get(), ws.minv32.get(), ws.u32, stream);
 }
 
 }  // namespace nricp_hip

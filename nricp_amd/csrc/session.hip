@@ -295,6 +295,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     corr_dist_.resize(nv_);
     robust_.resize(nv_);
     corr_tri_.resize(nv_);
+    query_order_.resize(nv_);
     rejection_.resize(nv_);
     energy_cur_.resize(5);
     energy_cand_.resize(5);
@@ -346,11 +347,18 @@ void Session::sync() {
 // the target hierarchy with its vertex normals, and the template graph.
 void Session::prepare() {
     hipStream_t st = stream_.get();
+    clock_.begin("target_bvh", st);
     build_vertex_faces(faces_.get(), nf_, nv_, template_vf_, sort_, st);
     build_lbvh(bvh_, target_vertex_count_, sort_, target_vf_, st);
+    // Queries run in the Morton order of the rest template: the deformation
+    // is smooth, so neighbouring queries stay neighbours.
+    morton_order(points_.get(), nv_, query_order_.get(), sort_, order_keys_, order_bounds_, st);
+    clock_.end(st);
     if (!shared_graph_) {
+        clock_.begin("graph", st);
         build_device_graph(points_.get(), nv_, settings_.graph_levels, settings_.nodes_initial, settings_.growth,
                            settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_, sort_, st);
+        clock_.end(st);
     }
     level_ = 0;
 }
@@ -410,15 +418,17 @@ void Session::refresh() {
     hipStream_t st = stream_.get();
     const GraphLevel& L = graph_->levels[level_];
     const nricp_hip_level_settings& ls = level_settings();
+    clock_.begin("deform", st);
     evaluate_deformation(points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_,
                          rot_.get(), trans_.get(), L.node_positions.get(), L.node_count, deformed_.get(), st);
     vertex_normals(deformed_.get(), faces_.get(), nf_, template_vf_, nv_, face_scratch_.get(), qnormals_.get(), st);
+    clock_.end(st);
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, ls.robust_kernel,
                     ls.robust_threshold};
     clock_.begin("closest_points", st);
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
-                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr);
+                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get());
     have_warm_ = true;
     clock_.end(st);
     counters_[kQueryLaunches] += 1.0;
@@ -578,7 +588,7 @@ void Session::final_statistics() {
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, 0, 0.0};
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
-                   nullptr, st, have_warm_ ? corr_tri_.get() : nullptr);
+                   nullptr, st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get());
     summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
     NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     sync();

@@ -153,7 +153,9 @@ private:
     // state
     DeviceBuffer<double> rot_, trans_, rot_cand_, trans_cand_, rot_prev_, trans_prev_, delta_;
     DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
-    DeviceBuffer<int32_t> corr_tri_;
+    DeviceBuffer<int32_t> corr_tri_, query_order_;
+    DeviceBuffer<uint64_t> order_keys_;
+    DeviceBuffer<double> order_bounds_;
     DeviceBuffer<uint8_t> rejection_;
     DeviceBuffer<double> bsr_;
     DeviceBuffer<float> bsr32_;
