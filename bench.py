@@ -98,6 +98,7 @@ def run_form(dp, job, settings, kwargs, form: str, steps: int, warmup: int) -> d
         "stages": stages,
         "vertices": np.asarray(result["vertices"]),
         "report": report,
+        "max_query_steps": after.get("query_max_steps", 0.0),
     }
 
 
@@ -118,6 +119,18 @@ def roofline(form: dict) -> dict:
         "avg_launch_us": round(1e6 * seconds / launches, 2),
         "algorithmic_bytes_per_launch": round(bytes_per_launch),
         "iterations_per_launch": round(form["counters"]["pcg_iterations"] / launches, 2),
+    }
+
+
+def query_stats(form: dict) -> dict:
+    c = form["counters"]
+    rows = max(c.get("query_rows", 0.0), 1.0)
+    ms = form["stages"]["closest_points"]["ms"]
+    return {
+        "queries_per_s": round(rows / (ms * 1e-3), 1) if ms > 0 else None,
+        "node_opens_per_query": round(c.get("query_node_opens", 0.0) / rows, 2),
+        "triangle_tests_per_query": round(c.get("query_leaf_tests", 0.0) / rows, 2),
+        "max_steps_one_query": int(form["max_query_steps"]),
     }
 
 
@@ -189,6 +202,7 @@ def main(argv=None) -> int:
             "gn_solves_per_s": round(r["gn_solves_per_s"], 2),
             "pcg_iterations_per_solve": round(r["counters"]["pcg_iterations"] / max(r["counters"]["pcg_launches"], 1), 2),
             "stage_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in r["stages"].items()},
+            "query": query_stats(r),
             "roofline_pcg": roofline(r),
         }
         if cpu is not None:

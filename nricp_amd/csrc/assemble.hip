@@ -16,12 +16,13 @@
 // residuals, per-slot arms, scales) and their energies; (2) one thread per
 // edge/node adds smoothness and anchor energies; (3) one wave per node gathers
 // its diagonal block and right-hand side from its vertex rows, edges and
-// anchor; (4) one thread per interacting pair gathers the off-diagonal block.
+// anchor; (4) eight lanes per interacting pair gather the off-diagonal block.
 // Energies are reduced in a fixed order, so the energy of a state is the same
 // bits whether it comes from an assembly or an energy-only pass.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "so3.hpp"
 
 namespace nricp_hip {
 namespace {
@@ -31,38 +32,6 @@ constexpr int kThreads = 256;
 __device__ inline d3 apply_node(const double* R, const double* T, const double* G, int64_t j, d3 p) {
     const d3 g = ld3(G, j);
     return (mul3(R + 9 * j, p - g) + g) + ld3(T, j);
-}
-
-// log through the (w >= 0) unit quaternion, as src/core/rotation.h:50-62.
-__device__ inline d3 log_so3(const double* r) {
-    double w, x, y, z;
-    const double tr = r[0] + r[4] + r[8];
-    if (tr > 0.0) {
-        const double s = sqrt(tr + 1.0) * 2.0;
-        w = 0.25 * s; x = (r[7] - r[5]) / s; y = (r[2] - r[6]) / s; z = (r[3] - r[1]) / s;
-    } else if (r[0] > r[4] && r[0] > r[8]) {
-        const double s = sqrt(1.0 + r[0] - r[4] - r[8]) * 2.0;
-        w = (r[7] - r[5]) / s; x = 0.25 * s; y = (r[1] + r[3]) / s; z = (r[2] + r[6]) / s;
-    } else if (r[4] > r[8]) {
-        const double s = sqrt(1.0 + r[4] - r[0] - r[8]) * 2.0;
-        w = (r[2] - r[6]) / s; x = (r[1] + r[3]) / s; y = 0.25 * s; z = (r[5] + r[7]) / s;
-    } else {
-        const double s = sqrt(1.0 + r[8] - r[0] - r[4]) * 2.0;
-        w = (r[3] - r[1]) / s; x = (r[2] + r[6]) / s; y = (r[5] + r[7]) / s; z = 0.25 * s;
-    }
-    const double l = sqrt(w * w + x * x + y * y + z * z);
-    if (l > 0.0) {
-        w /= l; x /= l; y /= l; z /= l;
-    } else {
-        w = 1.0; x = y = z = 0.0;
-    }
-    if (w < 0.0) {
-        w = -w; x = -x; y = -y; z = -z;
-    }
-    const double len = sqrt(x * x + y * y + z * z);
-    if (len < 1e-12) return d3{0.0, 0.0, 0.0};
-    const double angle = 2.0 * atan2(len, w);
-    return d3{x, y, z} * (angle / len);
 }
 
 // J_l^{-1}(w) = I - [w]/2 + c [w]^2 (src/core/rotation.h:69-103).
@@ -243,7 +212,9 @@ __device__ inline void add_plane_row(double* acc, double c, d3 a, d3 n, double v
     }
 }
 
-template <typename Scalar>
+// Blocks = false: the right-hand side only, with its rotation half written as
+// zero (the translation system of the local/global route).
+template <typename Scalar, bool Blocks>
 __global__ void __launch_bounds__(kThreads)
 node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
             const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
@@ -316,10 +287,13 @@ node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
     }
     for (int k = 0; k < kUpper + 6; ++k) acc[k] = wave_sum(acc[k]);
     if (lane == 0) {
-        Scalar* out = bsr + 36 * static_cast<int64_t>(diag_pos[node]);
-        for (int x = 0; x < 6; ++x)
-            for (int y = 0; y < 6; ++y) out[6 * x + y] = static_cast<Scalar>(x <= y ? acc[U(x, y)] : acc[U(y, x)]);
-        for (int x = 0; x < 6; ++x) rhs[6 * node + x] = acc[kUpper + x];
+        if (Blocks) {
+            Scalar* out = bsr + 36 * static_cast<int64_t>(diag_pos[node]);
+            for (int x = 0; x < 6; ++x)
+                for (int y = 0; y < 6; ++y)
+                    out[6 * x + y] = static_cast<Scalar>(x <= y ? acc[U(x, y)] : acc[U(y, x)]);
+        }
+        for (int x = 0; x < 6; ++x) rhs[6 * node + x] = (Blocks || x >= 3) ? acc[kUpper + x] : 0.0;
     }
 }
 
@@ -349,18 +323,25 @@ __device__ inline void add_pair_offset(double* acc, double clo, d3 alo, double c
     acc[6 * 5 + 5] += cc;
 }
 
+// Eight lanes per interacting pair: the lanes split the pair's entries, then
+// sum their 36 partial block entries with shuffles in a fixed order.
+constexpr int kPairLanes = 8;
+
 template <typename Scalar>
 __global__ void __launch_bounds__(kThreads)
 pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
             const double* __restrict__ arm, int64_t groups, const int32_t* __restrict__ goff,
             const int32_t* __restrict__ gent, const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
             int32_t* __restrict__ touched) {
-    const int64_t g = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (g >= groups) return;
+    const int sub = threadIdx.x % kPairLanes;
+    const int64_t g = (static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kPairLanes;
+    const bool valid = g < groups;
     double acc[36];
+#pragma unroll
     for (int k = 0; k < 36; ++k) acc[k] = 0.0;
     int hit = 0;
-    for (int32_t k = goff[g]; k < goff[g + 1]; ++k) {
+    const int32_t end = valid ? goff[g + 1] : 0;
+    for (int32_t k = valid ? goff[g] + sub : 0; k < end; k += kPairLanes) {
         const int32_t ent = gent[k];
         if (ent >= 0) {
             const int64_t v = ent >> 4;
@@ -379,7 +360,9 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
                 const double cl = sl * wlo, ch = sl * whi;
                 const double ql[6] = {cl * nlo.x, cl * nlo.y, cl * nlo.z, cl * n.x, cl * n.y, cl * n.z};
                 const double qh[6] = {ch * nhi.x, ch * nhi.y, ch * nhi.z, ch * n.x, ch * n.y, ch * n.z};
+#pragma unroll
                 for (int x = 0; x < 6; ++x)
+#pragma unroll
                     for (int y = 0; y < 6; ++y) acc[6 * x + y] += ql[x] * qh[y];
             }
             if (sm >= 0.0) {
@@ -392,29 +375,43 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             hit = 1;
             const double s2 = in.w.smooth;
             const d3 rot = mul3(in.rotations + 9 * a, ld3(in.rest, b) - ld3(in.rest, a));
-            // M = J_a J_b^T: columns 3..5 hold -s^2 [ [rot]x ; I ].
-            double m[36];
-            for (int k = 0; k < 36; ++k) m[k] = 0.0;
-            m[6 * 0 + 4] = -s2 * -rot.z;
-            m[6 * 0 + 5] = -s2 * rot.y;
-            m[6 * 1 + 3] = -s2 * rot.z;
-            m[6 * 1 + 5] = -s2 * -rot.x;
-            m[6 * 2 + 3] = -s2 * -rot.y;
-            m[6 * 2 + 4] = -s2 * rot.x;
-            m[6 * 3 + 3] = -s2;
-            m[6 * 4 + 4] = -s2;
-            m[6 * 5 + 5] = -s2;
-            if (a < b) {
-                for (int k = 0; k < 36; ++k) acc[k] += m[k];
-            } else {
-                for (int x = 0; x < 6; ++x)
-                    for (int y = 0; y < 6; ++y) acc[6 * x + y] += m[6 * y + x];
-            }
+            // J_a J_b^T: columns 3..5 hold -s^2 [ [rot]x ; I ]; transposed
+            // into the (lower, higher) block when a > b.
+            const bool fwd = a < b;
+#define NRICP_PAIR_ADD(r, c, v)                  \
+    do {                                         \
+        if (fwd)                                 \
+            acc[6 * (r) + (c)] += (v);           \
+        else                                     \
+            acc[6 * (c) + (r)] += (v);           \
+    } while (0)
+            NRICP_PAIR_ADD(0, 4, -s2 * -rot.z);
+            NRICP_PAIR_ADD(0, 5, -s2 * rot.y);
+            NRICP_PAIR_ADD(1, 3, -s2 * rot.z);
+            NRICP_PAIR_ADD(1, 5, -s2 * -rot.x);
+            NRICP_PAIR_ADD(2, 3, -s2 * -rot.y);
+            NRICP_PAIR_ADD(2, 4, -s2 * rot.x);
+            NRICP_PAIR_ADD(3, 3, -s2);
+            NRICP_PAIR_ADD(4, 4, -s2);
+            NRICP_PAIR_ADD(5, 5, -s2);
+#undef NRICP_PAIR_ADD
         }
     }
+#pragma unroll
+    for (int k = 0; k < 36; ++k) {
+        acc[k] += __shfl_xor(acc[k], 1, kWave);
+        acc[k] += __shfl_xor(acc[k], 2, kWave);
+        acc[k] += __shfl_xor(acc[k], 4, kWave);
+    }
+    hit |= __shfl_xor(hit, 1, kWave);
+    hit |= __shfl_xor(hit, 2, kWave);
+    hit |= __shfl_xor(hit, 4, kWave);
+    if (!valid || sub != 0) return;
     Scalar* ab = bsr + 36 * static_cast<int64_t>(pos[2 * g]);
     Scalar* ba = bsr + 36 * static_cast<int64_t>(pos[2 * g + 1]);
+#pragma unroll
     for (int x = 0; x < 6; ++x)
+#pragma unroll
         for (int y = 0; y < 6; ++y) {
             ab[6 * x + y] = static_cast<Scalar>(acc[6 * x + y]);
             ba[6 * y + x] = static_cast<Scalar>(acc[6 * x + y]);
@@ -453,17 +450,28 @@ void assemble(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s
               double* energies_out, hipStream_t stream) {
     launch_energies(in, s, true, energies_out, stream);
     const int per_block = kThreads / kWave;
-    node_blocks<Scalar><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+    node_blocks<Scalar, true><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
         L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), bsr, rhs);
     NRICP_LAUNCH_CHECK("node_blocks");
     if (L.group_count > 0) {
         s.touched.resize(L.group_count);
-        pair_blocks<Scalar><<<blocks_for(L.group_count, kThreads), kThreads, 0, stream>>>(
+        pair_blocks<Scalar><<<blocks_for(L.group_count * kPairLanes, kThreads), kThreads, 0, stream>>>(
             in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.group_count, L.group_offsets.get(),
             L.group_entries.get(), L.pair_position.get(), bsr, s.touched.get());
         NRICP_LAUNCH_CHECK("pair_blocks");
     }
+}
+
+void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s, double* rhs,
+                     double* energies_out, hipStream_t stream) {
+    launch_energies(in, s, true, energies_out, stream);
+    const int per_block = kThreads / kWave;
+    node_blocks<double, false><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+        in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
+        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), nullptr,
+        rhs);
+    NRICP_LAUNCH_CHECK("node_blocks");
 }
 
 template void assemble<double>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, double*, double*, double*,

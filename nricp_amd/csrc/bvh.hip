@@ -238,6 +238,81 @@ __global__ void refit_kernel(const double* __restrict__ corners, int64_t n, cons
     }
 }
 
+// Binary depth of every internal node (parent links up to the root); the
+// nodes at depths 0, 3, 6, ... root the wide nodes.
+__global__ void depth_kernel(const BvhNode* __restrict__ nodes, int64_t ni, int32_t* __restrict__ depth,
+                             int32_t* __restrict__ is_root) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    int32_t d = 0;
+    int32_t n = static_cast<int32_t>(i);
+    while (n > 0 && d < 128) {
+        n = nodes[n].parent;
+        ++d;
+    }
+    depth[i] = d;
+    is_root[i] = d % 3 == 0 ? 1 : 0;
+}
+
+// Wide node of the binary node i (depth % 3 == 0): its descendants down to
+// three levels below, stopping at leaves; internal nodes three levels down
+// are themselves wide roots and are referenced by their wide index.
+__global__ void collapse_kernel(const BvhNode* __restrict__ nodes, int64_t ni, const int32_t* __restrict__ depth,
+                                const int32_t* __restrict__ wide_index, WideNode* __restrict__ wide) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ni || depth[i] % 3 != 0) return;
+    int32_t ref[kWideArity];
+    float box[kWideArity][6];
+    int count = 0;
+    {
+        const BvhNode& root = nodes[i];
+        for (int c = 0; c < 2; ++c) {
+            ref[count] = root.child[c];
+            for (int k = 0; k < 6; ++k) box[count][k] = root.box[6 * c + k];
+            ++count;
+        }
+    }
+    for (int level = 1; level < 3; ++level) {
+        int32_t nref[kWideArity];
+        float nbox[kWideArity][6];
+        int m = 0;
+        for (int e = 0; e < count; ++e) {
+            const int32_t r = ref[e];
+            if (r >= 0 && r < ni) {
+                const BvhNode& c = nodes[r];
+                for (int h = 0; h < 2; ++h) {
+                    nref[m] = c.child[h];
+                    for (int k = 0; k < 6; ++k) nbox[m][k] = c.box[6 * h + k];
+                    ++m;
+                }
+            } else {
+                nref[m] = r;
+                for (int k = 0; k < 6; ++k) nbox[m][k] = box[e][k];
+                ++m;
+            }
+        }
+        count = m;
+        for (int e = 0; e < count; ++e) {
+            ref[e] = nref[e];
+            for (int k = 0; k < 6; ++k) box[e][k] = nbox[e][k];
+        }
+    }
+    WideNode& w = wide[wide_index[i]];
+    for (int e = 0; e < kWideArity; ++e) {
+        int32_t r = kWideEmpty;
+        if (e < count) {
+            r = ref[e];
+            if (r >= 0) r = (r < ni && depth[r] % 3 == 0) ? wide_index[r] : kWideEmpty;
+        }
+        w.child[e] = r;
+        w.pad[e] = 0;
+        for (int k = 0; k < 3; ++k) {
+            w.lo[k][e] = e < count ? box[e][k] : 0.0f;
+            w.hi[k][e] = e < count ? box[e][3 + k] : 0.0f;
+        }
+    }
+}
+
 // ------------------------------------------------------------- query -------
 
 // Ericson's Voronoi-region closest point with the CPU route's degenerate
@@ -292,114 +367,127 @@ __device__ inline d3 closest_on_triangle(d3 p, d3 a, d3 b, d3 c, double* bary) {
     return a + ab * v + ac * w;
 }
 
-// Squared distance from p to an FP32 box, in FP64 (the box bounds convert
-// exactly).
-__device__ inline double box_d2(d3 p, const float* b) {
-    const double cx = fmin(fmax(p.x, static_cast<double>(b[0])), static_cast<double>(b[3]));
-    const double cy = fmin(fmax(p.y, static_cast<double>(b[1])), static_cast<double>(b[4]));
-    const double cz = fmin(fmax(p.z, static_cast<double>(b[2])), static_cast<double>(b[5]));
-    const d3 d{p.x - cx, p.y - cy, p.z - cz};
-    return norm2(d);
-}
-
-struct Best {
-    double d2;
-    int32_t leaf;
-    int32_t triangle;
-    d3 point;
-    double bary[3];
-};
-
-__device__ inline void test_leaf(int32_t leaf, d3 p, const double* __restrict__ corners,
-                                 const int32_t* __restrict__ leaf_triangle, Best& best) {
-    const double* c = corners + 9 * static_cast<int64_t>(leaf);
-    const d3 a{c[0], c[1], c[2]}, b{c[3], c[4], c[5]}, cc{c[6], c[7], c[8]};
-    double bary[3];
-    const d3 q = closest_on_triangle(p, a, b, cc, bary);
-    const double d2 = norm2(q - p);
-    const int32_t tri = leaf_triangle[leaf];
-    if (d2 < best.d2 || (d2 == best.d2 && tri < best.triangle)) {
-        best.d2 = d2;
-        best.leaf = leaf;
-        best.triangle = tri;
-        best.point = q;
-        best.bary[0] = bary[0];
-        best.bary[1] = bary[1];
-        best.bary[2] = bary[2];
-    }
-}
-
-// Depth-first traversal with a per-lane stack in LDS. At every internal node
-// both child boxes are measured; the nearer child (ties to child 0) is entered
-// and the farther one pushed with its box distance, rounded down to FP32. A box
-// is skipped only when its distance exceeds the best squared distance times
-// (1 + 1e-12) -- a margin over the rounding of the FP64 distances -- so every
-// triangle that can tie or beat the best is tested, and the result is the
-// lexicographic minimum of (distance^2, triangle index) over all triangles.
-// A Karras tree over unique keys of at most 61 bits is at most 61 deep, so a
-// 64-entry stack cannot overflow; a malformed tree is caught by the step
-// budget and the overflow check instead (the query then reports triangle -1).
+// Wide traversal, eight lanes per query. Lane j measures child j of the open
+// node; the group tests its leaf children in parallel and keeps the
+// lexicographic minimum of (distance^2, triangle), then pushes the surviving
+// internal children on the group's LDS stack nearest on top, and pops the
+// nearest entry still within reach. A box is skipped only when its distance
+// exceeds the best squared distance times (1 + 1e-12) -- a margin over the
+// rounding of the FP64 distances -- so every triangle that can tie or beat the
+// best is tested, and the result is the lexicographic minimum over all
+// triangles. Every decision is taken on values the eight lanes share (after
+// shuffles), so the group stays converged. The stack holds at most 7 entries
+// per wide level (21 levels for keys of at most 61 bits); the overflow check
+// and the step budget only guard against a malformed tree (the query then
+// reports triangle -1).
 constexpr int kQueryThreads = 64;
-constexpr int kStackDepth = 64;
+constexpr int kLanes = kWideArity;
+constexpr int kGroupsPerBlock = kQueryThreads / kLanes;
+constexpr int kStackDepth = 192;
 constexpr double kPruneSlack = 1.0 + 1e-12;
 
-struct LaneStack {
-    int32_t* node;  // [kStackDepth][kQueryThreads]
-    float* dist;
+struct Nearest {
+    double d2;
+    int32_t triangle;
+    int32_t leaf;
+    uint32_t opens, tests;  // traversal work, for profiling
 };
 
-__device__ inline void nearest_triangle(d3 p, const BvhNode* __restrict__ nodes, int64_t n,
-                                        const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
-                                        LaneStack stack, Best& best) {
-    // `best` arrives initialised: empty (+inf), or seeded with a real
-    // triangle whose distance bounds the search from the start.
-    if (n == 1) {
-        test_leaf(0, p, corners, leaf_triangle, best);
-        return;
+// Lexicographic (d2, triangle) minimum over the group's eight lanes.
+__device__ inline void group_min(double& d2, int32_t& tri, int32_t& leaf) {
+#pragma unroll
+    for (int m = 1; m < kLanes; m <<= 1) {
+        const double od = __shfl_xor(d2, m, 64);
+        const int32_t ot = __shfl_xor(tri, m, 64);
+        const int32_t ol = __shfl_xor(leaf, m, 64);
+        if (od < d2 || (od == d2 && ot < tri)) {
+            d2 = od;
+            tri = ot;
+            leaf = ol;
+        }
     }
-    const int lane = threadIdx.x;
+}
+
+__device__ inline int group_sum(int v) {
+#pragma unroll
+    for (int m = 1; m < kLanes; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
+                                    const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
+                                    int32_t* stack_node, float* stack_dist, int j, int lane_base, Nearest& best) {
     int sp = 0;
     int32_t node = 0;
-    int64_t budget = 2 * n + 64;
+    int64_t budget = 2 * wide_count + 64;
     for (;;) {
-        // `node` is an internal node that has to be opened.
         if (--budget < 0) {
             best.triangle = -1;
             return;
         }
-        const BvhNode nd = nodes[node];
-        const double dl = box_d2(p, nd.box), dr = box_d2(p, nd.box + 6);
-        const bool left_near = dl <= dr;
-        const int32_t near = left_near ? nd.child[0] : nd.child[1];
-        const int32_t far = left_near ? nd.child[1] : nd.child[0];
-        const double dn = left_near ? dl : dr, df = left_near ? dr : dl;
-        if (!(df > best.d2 * kPruneSlack)) {
-            if (sp == kStackDepth) {
-                best.triangle = -1;
-                return;
-            }
-            stack.node[sp * kQueryThreads + lane] = far;
-            stack.dist[sp * kQueryThreads + lane] = __double2float_rd(df);
-            ++sp;
+        ++best.opens;
+        const WideNode* w = wide + node;
+        const int32_t ref = w->child[j];
+        double d = __builtin_inf();
+        if (ref != kWideEmpty) {
+            const float b[6] = {w->lo[0][j], w->lo[1][j], w->lo[2][j], w->hi[0][j], w->hi[1][j], w->hi[2][j]};
+            const double cx = fmin(fmax(p.x, static_cast<double>(b[0])), static_cast<double>(b[3]));
+            const double cy = fmin(fmax(p.y, static_cast<double>(b[1])), static_cast<double>(b[4]));
+            const double cz = fmin(fmax(p.z, static_cast<double>(b[2])), static_cast<double>(b[5]));
+            d = norm2(d3{p.x - cx, p.y - cy, p.z - cz});
         }
-        if (!(dn > best.d2 * kPruneSlack)) {
-            if (near >= 0) {
-                node = near;
-                continue;
+        // Leaf children within reach: tested in parallel, then merged.
+        double cd2 = best.d2;
+        int32_t ctri = best.triangle, cleaf = best.leaf;
+        if (ref != kWideEmpty && ref < 0 && !(d > best.d2 * kPruneSlack)) {
+            const int32_t leaf = ~ref;
+            const double* c = corners + 9 * static_cast<int64_t>(leaf);
+            double bary[3];
+            const d3 q = closest_on_triangle(p, d3{c[0], c[1], c[2]}, d3{c[3], c[4], c[5]}, d3{c[6], c[7], c[8]},
+                                             bary);
+            const double d2 = norm2(q - p);
+            const int32_t tri = leaf_triangle[leaf];
+            if (d2 < cd2 || (d2 == cd2 && tri < ctri)) {
+                cd2 = d2;
+                ctri = tri;
+                cleaf = leaf;
             }
-            test_leaf(~near, p, corners, leaf_triangle, best);
+            ++best.tests;
         }
-        // Pop until an entry is still within reach.
+        group_min(cd2, ctri, cleaf);
+        best.d2 = cd2;
+        best.triangle = ctri;
+        best.leaf = cleaf;
+        // Internal children still within reach, pushed nearest on top.
+        const bool push = ref != kWideEmpty && ref >= 0 && !(d > best.d2 * kPruneSlack);
+        int rank = 0;
+#pragma unroll
+        for (int k = 1; k < kLanes; ++k) {
+            const int other = (j + k) & (kLanes - 1);
+            const double od = __shfl(d, lane_base + other, 64);
+            const int op = __shfl(push ? 1 : 0, lane_base + other, 64);
+            if (op && (od < d || (od == d && other < j))) ++rank;
+        }
+        const int npush = group_sum(push ? 1 : 0);
+        if (sp + npush > kStackDepth) {
+            best.triangle = -1;
+            return;
+        }
+        if (push) {
+            const int at = sp + npush - 1 - rank;
+            stack_node[at] = ref;
+            stack_dist[at] = __double2float_rd(d);
+        }
+        sp += npush;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         node = -1;
         while (sp > 0) {
             --sp;
-            const int32_t c = stack.node[sp * kQueryThreads + lane];
-            const double d = static_cast<double>(stack.dist[sp * kQueryThreads + lane]);
-            if (d > best.d2 * kPruneSlack) continue;
-            if (c < 0) {
-                test_leaf(~c, p, corners, leaf_triangle, best);
-                continue;
-            }
+            const int32_t c = stack_node[sp];
+            const double dd = static_cast<double>(stack_dist[sp]);
+            if (dd > best.d2 * kPruneSlack) continue;
             node = c;
             break;
         }
@@ -422,38 +510,46 @@ __device__ inline double robust_weight(int kind, double s, double t) {
     return 1.0;
 }
 
-// One thread per query: mask, nearest triangle, interpolated normal, the
+// Eight lanes per query: mask, nearest triangle, interpolated normal, the
 // distance / normal / slope filters in the CPU route's order
 // (src/solver/correspondence.cpp:33-59), then the IRLS weight of
-// src/solver/route_a.cpp:19-38 for accepted rows.
+// src/solver/route_a.cpp:19-38 for accepted rows (written by the group's
+// first lane).
 __global__ void __launch_bounds__(kQueryThreads)
-query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __restrict__ corners,
+query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, const double* __restrict__ corners,
              const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
              const double* __restrict__ vnormals, const double* __restrict__ queries,
              const double* __restrict__ qnormals, const uint8_t* __restrict__ mask, int64_t count,
              QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
              double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
              int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
-             const int32_t* __restrict__ order) {
-    __shared__ int32_t stack_node[kStackDepth * kQueryThreads];
-    __shared__ float stack_dist[kStackDepth * kQueryThreads];
-    const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= count) return;
+             const int32_t* __restrict__ order, unsigned long long* __restrict__ stats) {
+    __shared__ int32_t stack_node[kGroupsPerBlock][kStackDepth];
+    __shared__ float stack_dist[kGroupsPerBlock][kStackDepth];
+    const int j = threadIdx.x % kLanes;
+    const int group = threadIdx.x / kLanes;
+    const int lane_base = (threadIdx.x % 64) & ~(kLanes - 1);
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
+    if (slot >= count) return;  // the whole group leaves together
     const int64_t i = order ? static_cast<int64_t>(order[slot]) : slot;
+    const bool writer = j == 0;
     if (mask && mask[i] == 0) {
-        st3(out_p, i, d3{0.0, 0.0, 0.0});
-        st3(out_n, i, d3{0.0, 0.0, 0.0});
-        out_d[i] = 0.0;
-        out_t[i] = -1;
-        out_rej[i] = 4;
-        if (out_u) out_u[i] = 1.0;
+        if (writer) {
+            st3(out_p, i, d3{0.0, 0.0, 0.0});
+            st3(out_n, i, d3{0.0, 0.0, 0.0});
+            out_d[i] = 0.0;
+            out_t[i] = -1;
+            out_rej[i] = 4;
+            if (out_u) out_u[i] = 1.0;
+        }
         return;
     }
     const d3 q = ld3(queries, i);
-    Best best;
+    Nearest best;
     best.d2 = __builtin_inf();
     best.leaf = -1;
     best.triangle = 0x7fffffff;
+    best.opens = best.tests = 0;
     // Warm start: the triangle this query matched last time is an exact
     // upper bound. The search still returns the lexicographic minimum of
     // (distance^2, triangle) -- boxes at exactly the bound are still visited --
@@ -465,12 +561,30 @@ query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __rest
         const d3 c = closest_on_triangle(q, ld3(tpoints, sf[0]), ld3(tpoints, sf[1]), ld3(tpoints, sf[2]), bary);
         best.d2 = norm2(c - q);
         best.triangle = seed;
-        best.point = c;
-        best.bary[0] = bary[0];
-        best.bary[1] = bary[1];
-        best.bary[2] = bary[2];
     }
-    nearest_triangle(q, nodes, nt, corners, leaf_triangle, LaneStack{stack_node, stack_dist}, best);
+    if (wide_count > 0) {
+        nearest_wide(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group], j, lane_base,
+                     best);
+    } else if (nt == 1) {
+        const double* c = corners;
+        double bary[3];
+        const d3 cp = closest_on_triangle(q, d3{c[0], c[1], c[2]}, d3{c[3], c[4], c[5]}, d3{c[6], c[7], c[8]}, bary);
+        const double d2 = norm2(cp - q);
+        const int32_t tri = leaf_triangle[0];
+        if (d2 < best.d2 || (d2 == best.d2 && tri < best.triangle)) {
+            best.d2 = d2;
+            best.triangle = tri;
+        }
+    }
+    if (stats) {
+        const int opens = best.opens, tests = group_sum(static_cast<int>(best.tests));
+        if (writer) {
+            atomicAdd(stats, static_cast<unsigned long long>(opens));
+            atomicAdd(stats + 1, static_cast<unsigned long long>(tests));
+            atomicMax(stats + 2, static_cast<unsigned long long>(opens));
+        }
+    }
+    if (!writer) return;
     if (best.triangle < 0 || best.triangle >= nt) {
         // Only a malformed hierarchy gets here: report the row as unusable.
         st3(out_p, i, q);
@@ -481,9 +595,13 @@ query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __rest
         if (out_u) out_u[i] = 1.0;
         return;
     }
+    // The winner's point and barycentrics, recomputed from its corners in
+    // the original face order (the same operations as the test that chose it).
     const int32_t* f = faces + 3 * static_cast<int64_t>(best.triangle);
+    double bary[3];
+    const d3 point = closest_on_triangle(q, ld3(tpoints, f[0]), ld3(tpoints, f[1]), ld3(tpoints, f[2]), bary);
     const d3 na = ld3(vnormals, f[0]), nb = ld3(vnormals, f[1]), nc = ld3(vnormals, f[2]);
-    const d3 normal = unit((na * best.bary[0] + nb * best.bary[1]) + nc * best.bary[2]);
+    const d3 normal = unit((na * bary[0] + nb * bary[1]) + nc * bary[2]);
     const double distance = sqrt(best.d2);
     uint8_t rej = 0;
     if (distance > s.distance_threshold) {
@@ -491,16 +609,16 @@ query_kernel(const BvhNode* __restrict__ nodes, int64_t nt, const double* __rest
     } else if (qnormals && dot(ld3(qnormals, i), normal) < s.min_normal_cosine) {
         rej = 2;
     } else if (distance > 0.0) {
-        const double slope = fabs(dot(normal, q - best.point)) / distance;
+        const double slope = fabs(dot(normal, q - point)) / distance;
         if (slope < s.min_slope) rej = 3;
     }
-    st3(out_p, i, best.point);
+    st3(out_p, i, point);
     st3(out_n, i, normal);
     out_d[i] = distance;
     out_t[i] = best.triangle;
     out_rej[i] = rej;
     if (out_u) {
-        const d3 off = q - best.point;
+        const d3 off = q - point;
         out_u[i] = (rej == 0 && s.robust_threshold > 0.0)
                        ? robust_weight(s.robust_kernel, norm2(off), s.robust_threshold)
                        : 1.0;
@@ -626,11 +744,29 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
         karras_kernel<<<blocks_for(nt - 1, 256), 256, 0, stream>>>(bvh.keys.get(), nt, bvh.nodes.get(),
                                                                     bvh.leaf_parent.get());
         NRICP_LAUNCH_CHECK("karras_kernel");
-        bvh.flags.resize(nt - 1);
+        bvh.flags.resize(nt);  // refit arrivals, then the wide-root flags
         bvh.flags.zero(stream);
         refit_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.leaf_corners.get(), nt, bvh.leaf_parent.get(),
                                                                bvh.nodes.get(), bvh.flags.get());
         NRICP_LAUNCH_CHECK("refit_kernel");
+        // Collapse into the eight-wide query tree.
+        const int64_t ni = nt - 1;
+        bvh.depth.resize(ni);
+        bvh.wide_index.resize(ni + 1);
+        depth_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(), bvh.flags.get());
+        NRICP_LAUNCH_CHECK("depth_kernel");
+        exclusive_scan32(bvh.flags.get(), bvh.wide_index.get(), ni, stream);
+        int32_t wide_count = 0;
+        NRICP_CHECK(hipMemcpyAsync(&wide_count, bvh.wide_index.get() + ni, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   stream));
+        NRICP_CHECK(hipStreamSynchronize(stream));
+        bvh.wide_count = wide_count;
+        bvh.wide.resize(std::max<int64_t>(wide_count, 1));
+        collapse_kernel<<<blocks_for(ni, 128), 128, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(),
+                                                                 bvh.wide_index.get(), bvh.wide.get());
+        NRICP_LAUNCH_CHECK("collapse_kernel");
+    } else {
+        bvh.wide_count = 0;
     }
     NRICP_CHECK(hipStreamSynchronize(stream));  // `cross` is released on return
 }
@@ -657,12 +793,14 @@ void morton_order(const double* points, int64_t n, int32_t* out, SortScratch& sc
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
-                    hipStream_t stream, const int32_t* warm, const int32_t* order) {
+                    hipStream_t stream, const int32_t* warm, const int32_t* order, uint64_t* stats) {
     if (count <= 0) return;
-    query_kernel<<<blocks_for(count, kQueryThreads), kQueryThreads, 0, stream>>>(
-        bvh.nodes.get(), bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(), bvh.faces.get(),
+    query_kernel<<<blocks_for(count * kLanes, kQueryThreads), kQueryThreads, 0, stream>>>(
+        bvh.wide.get(), bvh.wide_count, bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(),
+        bvh.faces.get(),
         bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm, out_points,
-        out_normals, out_distance, out_triangle, out_rejection, out_robust, order);
+        out_normals, out_distance, out_triangle, out_rejection, out_robust, order,
+        reinterpret_cast<unsigned long long*>(stats));
     NRICP_LAUNCH_CHECK("query_kernel");
 }
 

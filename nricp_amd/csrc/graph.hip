@@ -12,49 +12,12 @@
 #include <cfloat>
 
 #include "kernels.hpp"
+#include "so3.hpp"
 
 namespace nricp_hip {
 namespace {
 
 // ------------------------------------------------------------ rotations ----
-
-struct m3 {
-    double m[9];
-};
-
-__device__ inline m3 mat_mul(const m3& a, const m3& b) {
-    m3 o;
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-            double s = 0.0;
-            for (int k = 0; k < 3; ++k) s += a.m[r * 3 + k] * b.m[k * 3 + c];
-            o.m[r * 3 + c] = s;
-        }
-    return o;
-}
-
-__device__ inline m3 skew(d3 w) {
-    return m3{{0.0, -w.z, w.y, w.z, 0.0, -w.x, -w.y, w.x, 0.0}};
-}
-
-// Rodrigues with the second-order series below theta^2 = 1e-24.
-__device__ inline m3 exp_so3(d3 w) {
-    const double t2 = dot(w, w);
-    const m3 k = skew(w);
-    const m3 k2 = mat_mul(k, k);
-    double a, b;
-    if (t2 < 1e-24) {
-        a = 1.0;
-        b = 0.5;
-    } else {
-        const double t = sqrt(t2);
-        a = sin(t) / t;
-        b = (1.0 - cos(t)) / t2;
-    }
-    m3 o;
-    for (int i = 0; i < 9; ++i) o.m[i] = (i % 4 == 0 ? 1.0 : 0.0) + (a * k.m[i] + b * k2.m[i]);
-    return o;
-}
 
 struct quat {
     double w, x, y, z;

@@ -40,8 +40,25 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "node layout");
 
+// The query tree: the binary tree collapsed three levels at a time into
+// nodes of up to eight children, stored as structure-of-arrays so that eight
+// lanes read their child's box and reference in unit-stride rows.
+constexpr int kWideArity = 8;
+constexpr int32_t kWideEmpty = 0x7fffffff;  // unused child slot
+struct WideNode {
+    float lo[3][kWideArity];  // child boxes (outward-rounded FP32)
+    float hi[3][kWideArity];
+    int32_t child[kWideArity];  // >= 0 wide node, < 0 leaf ~index, kWideEmpty
+    int32_t pad[kWideArity];
+};
+static_assert(sizeof(WideNode) == 256, "wide node layout");
+
 struct Lbvh {
     int64_t triangle_count = 0;
+    int64_t wide_count = 0;
+    DeviceBuffer<WideNode> wide;        // wide nodes, root = 0
+    DeviceBuffer<int32_t> depth;        // binary depth of each internal node
+    DeviceBuffer<int32_t> wide_index;   // wide index of each internal node whose depth % 3 == 0
     DeviceBuffer<BvhNode> nodes;        // T - 1 internal nodes, root = 0
     DeviceBuffer<int32_t> leaf_parent;  // T
     DeviceBuffer<int32_t> leaf_triangle;  // T: triangle id of each leaf (Morton order)
@@ -73,11 +90,14 @@ struct QuerySettings {
 // given, holds each query's previous triangle (may alias out_triangle).
 // `order`, when given, is a permutation of the queries in which they are
 // processed (spatially coherent orders keep a wave's traversals together);
-// results are written at each query's own index either way.
+// results are written at each query's own index either way. `stats`, when
+// given, accumulates [nodes opened, triangles tested] and the maximum steps
+// of one query into its three entries.
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
-                    hipStream_t stream, const int32_t* warm = nullptr, const int32_t* order = nullptr);
+                    hipStream_t stream, const int32_t* warm = nullptr, const int32_t* order = nullptr,
+                    uint64_t* stats = nullptr);
 // Permutation of `count` points sorted by the Morton code (10 bits per axis)
 // of their position in the points' bounding box, ties by index.
 void morton_order(const double* points, int64_t count, int32_t* out_order, SortScratch& scratch,
@@ -189,6 +209,26 @@ void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStre
 template <typename Scalar>
 void assemble(const AssemblyInputs& in, const LevelLayout& layout, AssemblyScratch& s, Scalar* bsr, double* rhs,
               double* energies_out, hipStream_t stream);
+
+// The right-hand side -J^T r of the state with its rotation half zero: the
+// translation system of the local/global route (the energies of the same
+// state go to energies_out).
+void translation_rhs(const AssemblyInputs& in, const LevelLayout& layout, AssemblyScratch& s, double* rhs,
+                     double* energies_out, hipStream_t stream);
+
+// ------------------------------------------------------- local/global ------
+
+// Keeps the translation corner of every assembled block: diagonal blocks get
+// the identity in their rotation half, every rotation row and column of an
+// off-diagonal block is zeroed.
+template <typename Scalar>
+void restrict_to_translations(const LevelLayout& layout, Scalar* bsr, hipStream_t stream);
+// The local step: per node, the weighted rotation fit (Horn's quaternion) of
+// its bound vertices' correspondences, landmarks and first-endpoint edges,
+// taken under-relaxed in the tangent space: R <- exp(relaxation log(R_fit R^T)) R.
+// Writes the rotations in place (the fit reads no rotation but the node's own).
+void fit_rotations(const AssemblyInputs& in, const LevelLayout& layout, double relaxation, double* rotations,
+                   hipStream_t stream);
 
 // -------------------------------------------------------------- PCG --------
 

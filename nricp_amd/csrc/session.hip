@@ -50,6 +50,10 @@ __global__ void scatter_landmarks(const int32_t* __restrict__ vertices, const do
     target[3 * v + 2] = targets[3 * i + 2];
 }
 
+// Share of the fitted rotation the local step takes
+// (src/cuda/local_global.cu:10-18).
+constexpr double kRotationRelaxation = 0.2;
+
 double relative_change(double before, double after) {
     return (before - after) / std::max(std::abs(before), 1e-12);
 }
@@ -239,7 +243,6 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         throw InvalidArgument("unknown route");
     if (s.precision != NRICP_HIP_PRECISION_DOUBLE && s.precision != NRICP_HIP_PRECISION_MIXED)
         throw InvalidArgument("unknown precision");
-    if (s.route == NRICP_HIP_ROUTE_C) throw InvalidArgument("route_c is not available in this build");
     report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
     report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
     hipStream_t st = stream_.get();
@@ -299,6 +302,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     rejection_.resize(nv_);
     energy_cur_.resize(5);
     energy_cand_.resize(5);
+    energy_fit_.resize(5);
     summary_.resize(6);
     quantiles_.resize(2);
     NRICP_CHECK(hipStreamSynchronize(st));
@@ -341,6 +345,13 @@ AssemblyInputs Session::inputs(const double* R, const double* T) const {
 void Session::sync() {
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     clock_.collect();
+    if (stats_pending_) {
+        const std::vector<uint64_t> h = query_stats_.to_host(stream_.get());
+        counters_[kQueryNodeOpens] += static_cast<double>(h[0]);
+        counters_[kQueryLeafTests] += static_cast<double>(h[1]);
+        counters_[kQueryMaxSteps] = std::max(counters_[kQueryMaxSteps], static_cast<double>(h[2]));
+        stats_pending_ = false;
+    }
 }
 
 // Device-side preparation of one wrap: the template's vertex -> face lists,
@@ -426,9 +437,18 @@ void Session::refresh() {
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, ls.robust_kernel,
                     ls.robust_threshold};
     clock_.begin("closest_points", st);
+    uint64_t* stats = nullptr;
+    if (clock_.enabled()) {
+        if (!stats_pending_) {
+            query_stats_.resize(3);
+            query_stats_.zero(st);
+            stats_pending_ = true;
+        }
+        stats = query_stats_.get();
+    }
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
-                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get());
+                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get(), stats);
     have_warm_ = true;
     clock_.end(st);
     counters_[kQueryLaunches] += 1.0;
@@ -459,14 +479,20 @@ bool Session::inner_step() {
     hipStream_t st = stream_.get();
     const nricp_hip_level_settings& ls = level_settings();
     const int64_t nn = graph_->levels[level_].node_count;
+    const bool local_global = settings_.route == NRICP_HIP_ROUTE_C;
     if (!assembled_) {
         const AssemblyInputs in = inputs(rot_.get(), trans_.get());
         clock_.begin("assemble", st);
-        if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
+        if (settings_.precision == NRICP_HIP_PRECISION_MIXED) {
             assemble<float>(in, layout_, assembly_, bsr32_.get(), rhs_.get(), energy_cur_.get(), st);
-        else
+            if (local_global) restrict_to_translations<float>(layout_, bsr32_.get(), st);
+        } else {
             assemble<double>(in, layout_, assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st);
+            if (local_global) restrict_to_translations<double>(layout_, bsr_.get(), st);
+        }
         clock_.end(st);
+        // The local/global route keeps this matrix for the whole
+        // correspondence set (src/cuda/session.cu:259-267,325-327).
         assembled_ = true;
         // Compulsory traffic of one assembly: per vertex its rest point,
         // binding, correspondence, normal, weight and flags (134 B); per node
@@ -480,6 +506,21 @@ bool Session::inner_step() {
     bool accepted = false;
     double cur = 0.0, cand = 0.0;
     for (int a = 0; a < attempts; ++a) {
+        if (local_global) {
+            // Local step in place, then the translation right-hand side of the
+            // fitted rotations. A failed solve keeps the fitted rotations, and
+            // the next attempt fits again from them; a rejected step restores
+            // the saved state (src/cuda/session.cu:281-314).
+            NRICP_CHECK(hipMemcpyAsync(rot_prev_.get(), rot_.get(), 9 * nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                       st));
+            NRICP_CHECK(hipMemcpyAsync(trans_prev_.get(), trans_.get(), 3 * nn * sizeof(double),
+                                       hipMemcpyDeviceToDevice, st));
+            clock_.begin("local_step", st);
+            const AssemblyInputs in = inputs(rot_.get(), trans_.get());
+            fit_rotations(in, layout_, kRotationRelaxation, rot_.get(), st);
+            translation_rhs(inputs(rot_.get(), trans_.get()), layout_, assembly_, rhs_.get(), energy_fit_.get(), st);
+            clock_.end(st);
+        }
         solve(damping_);
         retract(rot_.get(), trans_.get(), delta_.get(), nn, rot_cand_.get(), trans_cand_.get(), st);
         clock_.begin("energy", st);
@@ -522,13 +563,19 @@ bool Session::inner_step() {
             for (int k = 0; k < 5; ++k) last_energy_[k] = h[5 + k];
             break;
         }
+        if (local_global) {
+            NRICP_CHECK(hipMemcpyAsync(rot_.get(), rot_prev_.get(), 9 * nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                       st));
+            NRICP_CHECK(hipMemcpyAsync(trans_.get(), trans_prev_.get(), 3 * nn * sizeof(double),
+                                       hipMemcpyDeviceToDevice, st));
+        }
         damping_ *= settings_.damping_increase;
     }
     if (!accepted) return false;
     std::swap(rot_, rot_cand_);
     std::swap(trans_, trans_cand_);
     std::swap(energy_cur_, energy_cand_);
-    assembled_ = false;
+    if (!local_global) assembled_ = false;
     current_energy_ = cand;
     damping_ = std::max(damping_ * settings_.damping_decrease, 1e-12);
     inner_ += 1;

@@ -91,6 +91,9 @@ enum Counter : int {
     kQueryLaunches,
     kQueryRows,
     kGaussNewtonSteps,
+    kQueryNodeOpens,
+    kQueryLeafTests,
+    kQueryMaxSteps,
     kCounterCount
 };
 
@@ -155,12 +158,14 @@ private:
     DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
     DeviceBuffer<int32_t> corr_tri_, query_order_;
     DeviceBuffer<uint64_t> order_keys_;
+    DeviceBuffer<uint64_t> query_stats_;  // opens, tests, max steps (profiling only)
+    bool stats_pending_ = false;
     DeviceBuffer<double> order_bounds_;
     DeviceBuffer<uint8_t> rejection_;
     DeviceBuffer<double> bsr_;
     DeviceBuffer<float> bsr32_;
     DeviceBuffer<double> rhs_;
-    DeviceBuffer<double> energy_cur_, energy_cand_, summary_, summary_partials_, quantiles_;
+    DeviceBuffer<double> energy_cur_, energy_cand_, energy_fit_, summary_, summary_partials_, quantiles_;
     DeviceBuffer<uint64_t> quantile_keys_;
     PinnedBuffer<double> staging_{32};
     // schedule

@@ -328,5 +328,13 @@ RouteResult solve_route_a(const Mesh& normalized_template, const TargetBvh& targ
                           const TemplateGraph& graph, const RouteSettings& s,
                           const std::vector<unsigned char>* mask,
                           const std::vector<Landmark>& landmarks);
+// Local/global route (src/cuda/local_global.cu, src/cuda/session.cu route C),
+// restated in FP64 with a direct solve of the translation system.
+void fit_rotations(Deformation& d, const EnergyInputs& in, const Weights& w, double relaxation);
+System restrict_to_translations(const System& s);
+RouteResult solve_route_c(const Mesh& normalized_template, const TargetBvh& target,
+                          const TemplateGraph& graph, const RouteSettings& s,
+                          const std::vector<unsigned char>* mask,
+                          const std::vector<Landmark>& landmarks, double relaxation = 0.2);
 
 }  // namespace oracle

@@ -272,8 +272,8 @@ LevelSettings level_settings(const py::dict& d) {
     return l;
 }
 
-// prepare_solve + route A + result dict (src/python/bindings.cpp:584-710, 1426-1492).
-py::dict solve_a(const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& levels,
+// prepare_solve + route A or C + result dict (src/python/bindings.cpp:584-710, 1426-1492).
+py::dict solve_route(bool local_global, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& levels,
                  const py::object& mask, const py::object& lmi, const py::object& lmt, double dist,
                  double mincos, double slope, double d0, double ddec, double dinc, int attempts,
                  bool diag, int graph_levels, int n0, int growth, double, int, int neighbours) {
@@ -314,7 +314,8 @@ py::dict solve_a(const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, con
     RouteResult r;
     {
         py::gil_scoped_release release;
-        r = solve_route_a(nt, bvh, graph, s, m.empty() ? nullptr : &m, lms);
+        r = local_global ? solve_route_c(nt, bvh, graph, s, m.empty() ? nullptr : &m, lms)
+                         : solve_route_a(nt, bvh, graph, s, m.empty() ? nullptr : &m, lms);
     }
     const auto end = std::chrono::steady_clock::now();
     r.report.setup_seconds = std::chrono::duration<double>(mid - start).count();
@@ -342,6 +343,17 @@ py::dict solve_a(const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, con
 }
 
 }  // namespace
+
+#define ROUTE_ARGS                                                                                              \
+    py::arg("template_points"), py::arg("template_faces"), py::arg("target_points"), py::arg("target_faces"),   \
+        py::arg("level_settings"), py::arg("mask") = py::none(), py::arg("landmark_indices") = py::none(),      \
+        py::arg("landmark_targets") = py::none(),                                                              \
+        py::arg("distance_threshold") = std::numeric_limits<double>::infinity(),                               \
+        py::arg("min_normal_cosine") = -1.0, py::arg("min_slope") = 0.0, py::arg("initial_damping") = 1e-3,    \
+        py::arg("damping_decrease") = 0.3, py::arg("damping_increase") = 4.0,                                  \
+        py::arg("max_damping_attempts") = 6, py::arg("diagonal_damping") = true, py::arg("graph_levels") = 4,  \
+        py::arg("nodes_initial") = 100, py::arg("growth") = 2, py::arg("pcg_tolerance") = 1e-5,                \
+        py::arg("pcg_max_iterations") = 60, py::arg("graph_neighbours") = 4
 
 PYBIND11_MODULE(nricp_oracle, m) {
     m.doc() = "FP64 CPU oracle of the NRICP inner loop (test infrastructure only).";
@@ -651,12 +663,21 @@ PYBIND11_MODULE(nricp_oracle, m) {
         std::copy(a.begin(), a.end(), o.mutable_data());
         return o;
     }, py::arg("diagonal"), py::arg("edges"), py::arg("edge_blocks"), py::arg("rhs"));
-    m.def("solve_route_a", &solve_a, py::arg("template_points"), py::arg("template_faces"), py::arg("target_points"),
-          py::arg("target_faces"), py::arg("level_settings"), py::arg("mask") = py::none(),
-          py::arg("landmark_indices") = py::none(), py::arg("landmark_targets") = py::none(),
-          py::arg("distance_threshold") = std::numeric_limits<double>::infinity(), py::arg("min_normal_cosine") = -1.0,
-          py::arg("min_slope") = 0.0, py::arg("initial_damping") = 1e-3, py::arg("damping_decrease") = 0.3,
-          py::arg("damping_increase") = 4.0, py::arg("max_damping_attempts") = 6, py::arg("diagonal_damping") = true,
-          py::arg("graph_levels") = 4, py::arg("nodes_initial") = 100, py::arg("growth") = 2,
-          py::arg("pcg_tolerance") = 1e-5, py::arg("pcg_max_iterations") = 60, py::arg("graph_neighbours") = 4);
+    m.def("solve_route_a",
+          [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l, const py::object& mk,
+             const py::object& li, const py::object& lt, double dist, double mincos, double slope, double d0,
+             double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double tol, int maxit, int nb) {
+              return solve_route(false, tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag,
+                                 gl, n0, g, tol, maxit, nb);
+          },
+          ROUTE_ARGS);
+    // The local/global route's FP64 restatement (test checker for route C).
+    m.def("solve_route_c",
+          [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l, const py::object& mk,
+             const py::object& li, const py::object& lt, double dist, double mincos, double slope, double d0,
+             double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double tol, int maxit, int nb) {
+              return solve_route(true, tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag,
+                                 gl, n0, g, tol, maxit, nb);
+          },
+          ROUTE_ARGS);
 }

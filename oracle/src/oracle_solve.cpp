@@ -735,4 +735,259 @@ RouteResult solve_route_a(const Mesh& tmpl, const TargetBvh& target, const Templ
     return res;
 }
 
+// ------------------------------------------------------- route C ---------
+namespace {
+
+// Horn's quaternion of the correlation S = sum w x y^T by cyclic Jacobi on
+// the symmetric 4x4, at most 12 sweeps in a fixed order, largest eigenvalue's
+// eigenvector (src/cuda/local_global.cu:53-129, in FP64).
+Quat horn_quaternion(const double* S) {
+    const double sxx = S[0], sxy = S[1], sxz = S[2], syx = S[3], syy = S[4], syz = S[5], szx = S[6], szy = S[7],
+                 szz = S[8];
+    double a[4][4];
+    a[0][0] = sxx + syy + szz;
+    a[0][1] = syz - szy;
+    a[0][2] = szx - sxz;
+    a[0][3] = sxy - syx;
+    a[1][1] = sxx - syy - szz;
+    a[1][2] = sxy + syx;
+    a[1][3] = szx + sxz;
+    a[2][2] = -sxx + syy - szz;
+    a[2][3] = syz + szy;
+    a[3][3] = -sxx - syy + szz;
+    for (int r = 1; r < 4; ++r)
+        for (int c = 0; c < r; ++c) a[r][c] = a[c][r];
+    double v[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) v[r][c] = r == c ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        double off = 0.0;
+        for (int r = 0; r < 4; ++r)
+            for (int c = r + 1; c < 4; ++c) off += a[r][c] * a[r][c];
+        if (off <= 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double sign = theta >= 0.0 ? 1.0 : -1.0;
+                const double t = sign / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0);
+                const double sn = t * cs;
+                for (int k = 0; k < 4; ++k) {
+                    const double f = a[k][p], g = a[k][q];
+                    a[k][p] = cs * f - sn * g;
+                    a[k][q] = sn * f + cs * g;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double f = a[p][k], g = a[q][k];
+                    a[p][k] = cs * f - sn * g;
+                    a[q][k] = sn * f + cs * g;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double f = v[k][p], g = v[k][q];
+                    v[k][p] = cs * f - sn * g;
+                    v[k][q] = sn * f + cs * g;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (a[i][i] > a[best][best]) best = i;
+    return Quat{v[0][best], v[1][best], v[2][best], v[3][best]};
+}
+
+// Row-major rotation of the normalised quaternion (src/cuda/local_global.cu:134-157).
+M3 horn_rotation(const Quat& q) {
+    const double len = std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+    M3 r;
+    if (!(len > 0.0)) return r;
+    const double w = q.w / len, x = q.x / len, y = q.y / len, z = q.z / len;
+    r.m[0] = w * w + x * x - y * y - z * z;
+    r.m[1] = 2.0 * (x * y - w * z);
+    r.m[2] = 2.0 * (x * z + w * y);
+    r.m[3] = 2.0 * (x * y + w * z);
+    r.m[4] = w * w - x * x + y * y - z * z;
+    r.m[5] = 2.0 * (y * z - w * x);
+    r.m[6] = 2.0 * (x * z - w * y);
+    r.m[7] = 2.0 * (y * z + w * x);
+    r.m[8] = w * w - x * x - y * y + z * z;
+    return r;
+}
+
+void add_correlation(double* S, double w, V3 x, V3 y) {
+    const double xv[3] = {x.x, x.y, x.z}, yv[3] = {y.x, y.y, y.z};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) S[3 * r + c] += w * xv[r] * yv[c];
+}
+
+}  // namespace
+
+// The local step (src/cuda/local_global.cu:188-302): per node the weighted
+// fit of its bound vertices' correspondences ((lambda_c + lambda_p) u w_own),
+// landmarks (lambda_l w_own) and the edges it starts (lambda_s), then
+// R <- exp(relaxation log(R_fit R^T)) R.
+void fit_rotations(Deformation& d, const EnergyInputs& in, const Weights& w, double relaxation) {
+    const int n = static_cast<int>(d.x.size());
+    std::vector<double> S(9 * static_cast<size_t>(n), 0.0);
+    std::vector<int> lm_of(in.points.size(), -1);
+    for (size_t k = 0; k < in.landmarks.size(); ++k) {
+        const int v = in.landmarks[k].vertex;
+        if (v >= 0 && static_cast<size_t>(v) < lm_of.size()) lm_of[v] = static_cast<int>(k);
+    }
+    const double data_weight = w.p2p + w.p2plane;
+    for (size_t v = 0; v < in.points.size(); ++v) {
+        const Binding& b = in.bindings[v];
+        for (int s = 0; s < b.count; ++s) {
+            const int j = b.node[s];
+            if (j < 0 || j >= n || b.w[s] == 0.0) continue;
+            const double own = b.w[s];
+            const V3 g = d.rest[j], t = d.x[j].t;
+            const V3 source = in.points[v] - g;
+            const DataCorr& c = in.corr[v];
+            const bool active = c.accepted && participates(in.mask, v);
+            if (active) add_correlation(&S[9 * j], data_weight * c.robust * own, source, (c.point - g) - t);
+            if (w.landmark > 0.0 && lm_of[v] >= 0)
+                add_correlation(&S[9 * j], w.landmark * own, source, (in.landmarks[lm_of[v]].target - g) - t);
+        }
+    }
+    for (const auto& e : in.edges) {
+        const int a = e.first, b = e.second;
+        if (a < 0 || b < 0 || a >= n || b >= n) continue;
+        const V3 g = d.rest[a], t = d.x[a].t, gb = d.rest[b];
+        add_correlation(&S[9 * a], w.smooth, gb - g, ((gb + d.x[b].t) - g) - t);
+    }
+    for (int j = 0; j < n; ++j) {
+        const M3 fitted = horn_rotation(horn_quaternion(&S[9 * j]));
+        if (relaxation >= 1.0) {
+            d.x[j].r = fitted;
+            continue;
+        }
+        const M3 cur = d.x[j].r;
+        const V3 omega = log_so3(fitted * transpose(cur)) * relaxation;
+        d.x[j].r = exp_so3(omega) * cur;
+    }
+}
+
+// The (t, t) corner of the system: identity in each diagonal block's rotation
+// half, rotation rows and columns of the off-diagonal blocks zeroed, rotation
+// half of the right-hand side zero (src/cuda/local_global.cu:453-485,509-534).
+System restrict_to_translations(const System& s) {
+    System o = s;
+    for (auto& blk : o.diag)
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+                if (r < 3 || c < 3) blk.m[6 * r + c] = (r == c) ? 1.0 : 0.0;
+    for (auto& blk : o.off)
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+                if (r < 3 || c < 3) blk.m[6 * r + c] = 0.0;
+    for (int i = 0; i < o.n; ++i)
+        for (int k = 0; k < 3; ++k) o.rhs[6 * i + k] = 0.0;
+    return o;
+}
+
+// The local/global schedule of the device session with DeviceRoute::LocalGlobal
+// (src/cuda/session.cu:259-267,277-328): the translation matrix once per
+// correspondence set; per attempt a local step in place, the translation
+// right-hand side, the solve; a failed solve keeps the fitted rotations, a
+// rejected step restores the state before the attempt; after an accept the
+// matrix stays and the candidate energy becomes the current one.
+RouteResult solve_route_c(const Mesh& tmpl, const TargetBvh& target, const TemplateGraph& graph,
+                          const RouteSettings& s, const std::vector<unsigned char>* mask,
+                          const std::vector<Landmark>& landmarks, double relaxation) {
+    RouteResult res;
+    res.report.route = "route_c";
+    if (graph.levels.empty()) return res;
+    Deformation d = identity_deformation(graph.levels.front().nodes);
+    EnergyInputs in;
+    in.points = tmpl.v;
+    in.landmarks = landmarks;
+    in.mask = mask;
+    for (size_t li = 0; li < graph.levels.size(); ++li) {
+        const GraphLevel& L = graph.levels[li];
+        if (li > 0) d = transfer_level(graph.levels[li - 1].nodes, d, L.nodes);
+        const LevelSettings ls = level_for(s, li);
+        in.bindings = L.bindings;
+        in.nodes = L.nodes;
+        in.edges = L.edges;
+        LevelView view;
+        view.level = static_cast<int>(li);
+        view.nodes = static_cast<int>(L.nodes.size());
+        double damping = s.initial_damping;
+        CorrSet found;
+        double prev = 0.0;
+        const int outer_limit = std::max(1, ls.max_outer);
+        const int inner_limit = std::max(1, ls.max_inner);
+        const int attempts = std::max(1, s.max_damping_attempts);
+        for (int outer = 0; outer < outer_limit; ++outer) {
+            const std::vector<V3> def = evaluate(tmpl.v, L.bindings, d);
+            Mesh dm{def, tmpl.f};
+            found = find_correspondences(target, def, vertex_normals(dm), s.corr, mask);
+            in.corr = irls(def, found, ls);
+            const Fused f = fused_system(in, d, ls.weights, true);
+            const System matrix = restrict_to_translations(f.sys);
+            double cur = f.energies.total();
+            if (outer == 0) view.energy_start = cur;
+            for (int inner = 0; inner < inner_limit; ++inner) {
+                res.trace_energy.push_back(cur);
+                bool accepted = false;
+                double cand_e = cur;
+                Deformation cand = d;
+                for (int a = 0; a < attempts; ++a) {
+                    const Deformation saved = d;
+                    fit_rotations(d, in, ls.weights, relaxation);
+                    System sys = matrix;
+                    const System rhs = restrict_to_translations(fused_system(in, d, ls.weights, true).sys);
+                    sys.rhs = rhs.rhs;
+                    const SolveResult sol = solve_direct(damped(sys, damping, s.diagonal_damping));
+                    if (!sol.ok) {
+                        damping *= s.damping_increase;
+                        continue;
+                    }
+                    cand = d;
+                    retract(cand, sol.delta);
+                    cand_e = fused_system(in, cand, ls.weights, false).energies.total();
+                    if (cand_e < cur) {
+                        accepted = true;
+                        break;
+                    }
+                    d = saved;
+                    damping *= s.damping_increase;
+                }
+                if (!accepted) break;
+                d = cand;
+                damping = std::max(damping * s.damping_decrease, 1e-12);
+                view.inner += 1;
+                if (rel_change(cur, cand_e) <= ls.inner_tol) break;
+                cur = cand_e;
+            }
+            const double end = fused_system(in, d, ls.weights, false).energies.total();
+            view.energy_end = end;
+            view.corr_mean = found.counts.mean;
+            view.outer += 1;
+            if (outer > 0 && rel_change(prev, end) <= ls.outer_tol) {
+                view.converged = true;
+                break;
+            }
+            prev = end;
+        }
+        res.report.levels.push_back(view);
+        const std::string pre = "level" + std::to_string(li) + "_";
+        res.report.settings.push_back({pre + "point_to_point", ls.weights.p2p});
+        res.report.settings.push_back({pre + "point_to_plane", ls.weights.p2plane});
+        res.report.settings.push_back({pre + "smoothness", ls.weights.smooth});
+        res.report.settings.push_back({pre + "anchor", ls.weights.anchor});
+        res.report.settings.push_back({pre + "landmark", ls.weights.landmark});
+        res.report.settings.push_back({pre + "robust_threshold", ls.robust_threshold});
+    }
+    const GraphLevel& fin = graph.levels.back();
+    res.vertices = evaluate(tmpl.v, fin.bindings, d);
+    res.deformation = d;
+    Mesh dm{res.vertices, tmpl.f};
+    const CorrSet fin_found = find_correspondences(target, res.vertices, vertex_normals(dm), s.corr, mask);
+    res.report.corr = fin_found.counts;
+    res.report.energy = block_energies(build_blocks(in, d, level_for(s, graph.levels.size() - 1).weights));
+    return res;
+}
+
 }  // namespace oracle

@@ -211,3 +211,18 @@ def test_route_b_mixed_precision_matches_the_oracle(device, oracle):
     rms, worst = _rms_max(d["vertices"], o["vertices"], diag)
     assert rms <= 1e-5 and worst <= 1e-4, (rms, worst)
     assert json.loads(d["report"])["precision"] == "float"
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_route_c_matches_the_oracle(device, oracle, precision):
+    template, faces, target = _jittered_pair(seed=11)
+    kwargs = dict(graph_levels=2, nodes_initial=80, growth=2)
+    d = device.solve_route_c(template, faces, target, faces, route_settings(2), pcg_tolerance=1e-12,
+                             pcg_max_iterations=5000, precision=precision, **kwargs)
+    o = oracle.solve_route_c(template, faces, target, faces, route_settings(2), **kwargs)
+    diag = float(np.linalg.norm(template.max(0) - template.min(0)))
+    rms, worst = _rms_max(d["vertices"], o["vertices"], diag)
+    assert rms <= 1e-5 and worst <= 1e-4, (rms, worst)
+    rd, ro = json.loads(d["report"]), json.loads(o["report"])
+    assert rd["route"] == "route_c"
+    assert [l["inner_iterations"] for l in rd["levels"]] == [l["inner_iterations"] for l in ro["levels"]]
