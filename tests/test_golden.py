@@ -73,12 +73,24 @@ def test_build_node_edges(impl, k):
 
 
 def test_closest_points(impl):
+    """Identical answers. The one freedom: when several triangles are at
+    exactly the same distance (a query whose closest point is a vertex they
+    share), the reference keeps the first its hierarchy visits and the device
+    the lowest index -- the closest point, distance and normal are the same
+    bits either way."""
     g = fixture("closest_points")
     r = impl.closest_points(g["target_points"], g["target_faces"], g["queries"])
-    assert np.array_equal(np.asarray(r["triangles"]), g["triangles"])
     assert np.array_equal(np.asarray(r["points"]), g["points"])
     assert np.array_equal(np.asarray(r["distances"]), g["distances"])
     assert np.array_equal(np.asarray(r["normals"]), g["normals"])
+    tri = np.asarray(r["triangles"])
+    if impl.__name__ == "nricp_oracle":
+        assert np.array_equal(tri, g["triangles"])
+    else:
+        faces = g["target_faces"]
+        for i in np.nonzero(tri != g["triangles"])[0]:
+            shared = set(faces[tri[i]]) & set(faces[g["triangles"][i]])
+            assert any(np.array_equal(g["target_points"][v], g["points"][i]) for v in shared), i
 
 
 @pytest.mark.parametrize("tag", ["plain", "filtered"])
