@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 TOLERANCE = 1e-12
 MAX_PCG = 5000
 # Device-timed stages of a wrap (HIP events around each stage's launches).
-STAGES = ("graph", "target_bvh", "layout", "deform", "closest_points", "assemble", "pcg", "energy")
+STAGES = ("fps", "graph", "target_bvh", "layout", "deform", "closest_points", "assemble", "pcg", "energy", "local_step")
 
 
 def log(msg: str) -> None:

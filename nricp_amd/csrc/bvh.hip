@@ -677,21 +677,22 @@ __global__ void pick_quantiles(const uint64_t* __restrict__ keys, int64_t accept
 
 void build_vertex_faces(const int32_t* faces, int64_t nf, int64_t nv, VertexFaces& out, SortScratch& scratch,
                         hipStream_t stream) {
-    DeviceBuffer<uint64_t> keys(3 * nf);
-    DeviceBuffer<uint32_t> vals(3 * nf);
+    out.keys.resize(3 * nf);
+    out.vals.resize(3 * nf);
+    uint64_t* keys = out.keys.get();
+    uint32_t* vals = out.vals.get();
     if (nf > 0) {
-        vertex_face_entries<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(faces, nf, keys.get(), vals.get());
+        vertex_face_entries<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(faces, nf, keys, vals);
         NRICP_LAUNCH_CHECK("vertex_face_entries");
-        radix_sort_pairs(keys.get(), vals.get(), 3 * nf, bits_for(static_cast<uint64_t>(nv)), scratch, stream);
+        radix_sort_pairs(keys, vals, 3 * nf, bits_for(static_cast<uint64_t>(nv)), scratch, stream);
     }
     out.offsets.resize(nv + 1);
     out.faces.resize(3 * nf);
-    offsets_from_sorted(keys.get(), 3 * nf, 0, ~0ull, nv, out.offsets.get(), stream);
+    offsets_from_sorted(keys, 3 * nf, 0, ~0ull, nv, out.offsets.get(), stream);
     if (nf > 0) {
-        copy_u32_to_i32<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(vals.get(), 3 * nf, out.faces.get());
+        copy_u32_to_i32<<<blocks_for(3 * nf, 256), 256, 0, stream>>>(vals, 3 * nf, out.faces.get());
         NRICP_LAUNCH_CHECK("copy_u32_to_i32");
     }
-    NRICP_CHECK(hipStreamSynchronize(stream));  // keys/vals are released on return
 }
 
 void vertex_normals(const double* points, const int32_t* faces, int64_t nf, const VertexFaces& vf, int64_t nv,
@@ -714,8 +715,9 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     // Target vertex normals (face order per vertex, as the CPU route).
     build_vertex_faces(bvh.faces.get(), nt, nv, vf, scratch, stream);
     bvh.vertex_normals.resize(3 * nv);
-    DeviceBuffer<double> cross(3 * nt);
-    vertex_normals(bvh.points.get(), bvh.faces.get(), nt, vf, nv, cross.get(), bvh.vertex_normals.get(), stream);
+    bvh.face_cross.resize(3 * nt);
+    vertex_normals(bvh.points.get(), bvh.faces.get(), nt, vf, nv, bvh.face_cross.get(), bvh.vertex_normals.get(),
+                   stream);
     // Morton keys of the centroids, triangle index in the low bits.
     bvh.centroids.resize(3 * nt);
     centroid_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.points.get(), bvh.faces.get(), nt, bvh.centroids.get());
@@ -755,7 +757,7 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
         bvh.wide_index.resize(ni + 1);
         depth_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(), bvh.flags.get());
         NRICP_LAUNCH_CHECK("depth_kernel");
-        exclusive_scan32(bvh.flags.get(), bvh.wide_index.get(), ni, stream);
+        exclusive_scan32(bvh.flags.get(), bvh.wide_index.get(), ni, scratch, stream);
         int32_t wide_count = 0;
         NRICP_CHECK(hipMemcpyAsync(&wide_count, bvh.wide_index.get() + ni, sizeof(int32_t), hipMemcpyDeviceToHost,
                                    stream));
@@ -768,7 +770,6 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     } else {
         bvh.wide_count = 0;
     }
-    NRICP_CHECK(hipStreamSynchronize(stream));  // `cross` is released on return
 }
 
 void morton_order(const double* points, int64_t n, int32_t* out, SortScratch& scratch, DeviceBuffer<uint64_t>& keys,

@@ -239,11 +239,11 @@ int nricp_hip_farthest_point_seeds(const double* points, int64_t n, int32_t seed
         const int32_t count = static_cast<int32_t>(std::min<int64_t>(seeds, n));
         const int32_t f = static_cast<int32_t>(std::clamp<int64_t>(first, 0, n - 1));
         hipStream_t st = op_stream();
-        DeviceBuffer<double> p, partial;
-        DeviceBuffer<unsigned> barrier;
+        DeviceBuffer<double> p;
+        FpsWork work;
         DeviceBuffer<int32_t> o(count);
         p.upload(points, 3 * n, st);
-        farthest_point_seeds(p.get(), n, count, f, o.get(), partial, barrier, st);
+        farthest_point_seeds(p.get(), n, count, f, o.get(), work, st);
         o.download(out, count, st);
         NRICP_CHECK(hipStreamSynchronize(st));
     });

@@ -103,25 +103,38 @@ __device__ inline void block_sum(const double (&v)[Count], double* lds /* Count 
     __syncthreads();
 }
 
+// ------------------------------------------------------ persistent grids ----
+//
+// The sampling (FPS) and PCG kernels are persistent grids whose groups meet at
+// grid barriers, so every group of a launch must be resident at once. They are
+// ordinary launches -- a cooperative launch would serialise them with the work
+// of other streams, and the sampling runs beside the solve on its own stream --
+// and residency is guaranteed by construction instead:
+//   * a launch never has more groups than the device holds at once for that
+//     kernel (occupancy query), and
+//   * each of the two persistent kernels uses at most half of the compute
+//     units, so both fit together, and
+//   * every other kernel runs to completion without waiting on anything, so
+//     the groups of a persistent launch that are not yet resident become
+//     resident once the ordinary kernels ahead of them retire.
+int persistent_group_budget(const void* kernel, int threads);  // groups one persistent launch may use
+void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream);
+
 // --------------------------------------------------------- grid barrier ----
 
-// Sense-free generation barrier for persistent grids launched with
-// hipLaunchCooperativeKernel (which guarantees co-residency). `state[0]` is the
-// arrival counter, `state[1]` the generation. Agent-scope acquire/release
-// orders every global write before the barrier against every read after it.
+// Grid barrier for persistent grids (launched under the residency rules
+// above) on one monotonic arrival counter, zeroed before the launch: a group
+// arriving at barrier k draws a ticket in [kP, (k+1)P) and waits until the
+// counter reaches (k+1)P -- one atomic and a poll, no reset and no separate
+// generation word. Agent-scope acq_rel on the ticket and acquire on the poll
+// order every global write before the barrier against every read after it.
 __device__ inline void grid_barrier(unsigned* state, unsigned participants) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned gen = __hip_atomic_load(state + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived =
-            __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == participants - 1) {
-            __hip_atomic_store(state, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(state + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(state + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-            }
+        const unsigned ticket = __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = (ticket / participants + 1u) * participants;
+        while (static_cast<int>(__hip_atomic_load(state, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+            __builtin_amdgcn_s_sleep(1);
         }
     }
     __syncthreads();

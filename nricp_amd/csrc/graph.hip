@@ -95,7 +95,7 @@ __global__ void retract_kernel(const double* __restrict__ R, const double* __res
 
 // ---------------------------------------------------------- sampling -------
 
-constexpr int kFpsThreads = 256;
+constexpr int kFpsThreads = 512;
 constexpr int kFpsPerThread = 16;
 
 // A sampling candidate: its running nearest squared distance, its index and
@@ -119,81 +119,68 @@ __device__ inline ArgMax shfl_xor(const ArgMax& v, int m) {
                   d3{__shfl_xor(v.point.x, m, 64), __shfl_xor(v.point.y, m, 64), __shfl_xor(v.point.z, m, 64)}};
 }
 
-struct ArgMaxLds {
-    double value[kFpsThreads / 64], x[kFpsThreads / 64], y[kFpsThreads / 64], z[kFpsThreads / 64];
-    int64_t index[kFpsThreads / 64];
-};
-
-// Winner over the block; every thread receives it.
-__device__ inline ArgMax block_argmax(ArgMax v, ArgMaxLds& lds) {
+__device__ inline ArgMax wave_argmax(ArgMax v) {
+#pragma unroll
     for (int offset = 32; offset > 0; offset >>= 1) v = better(v, shfl_xor(v, offset));
-    const int wave = threadIdx.x / 64;
-    if (threadIdx.x % 64 == 0) {
-        lds.value[wave] = v.value;
-        lds.index[wave] = v.index;
-        lds.x[wave] = v.point.x;
-        lds.y[wave] = v.point.y;
-        lds.z[wave] = v.point.z;
-    }
-    __syncthreads();
-    ArgMax r{lds.value[0], lds.index[0], d3{lds.x[0], lds.y[0], lds.z[0]}};
-    for (int w = 1; w < kFpsThreads / 64; ++w)
-        r = better(r, ArgMax{lds.value[w], lds.index[w], d3{lds.x[w], lds.y[w], lds.z[w]}});
-    __syncthreads();
-    return r;
+    return v;
 }
+
+constexpr int kFpsWaves = kFpsThreads / 64;
 
 // Persistent FPS: each workgroup owns a contiguous chunk of points and keeps
 // them and their running nearest squared distances in registers. One grid
-// barrier per seed: every group publishes its chunk's winner (value, index,
-// position); after the barrier thread t of every group loads winner t and
-// the group reduces them, so every group picks the same seed.
+// barrier per seed: every wave publishes its winner (value, index, position)
+// to its slot; after the barrier every wave loads all slots and reduces them
+// with shuffles, so every wave of every group picks the same seed (the order
+// (value, lower index) is total, so the reduction order cannot matter). A
+// launch computes seeds [begin, end); the running distances go back to memory
+// at its end, so the next launch resumes where this one stopped.
 __global__ void __launch_bounds__(kFpsThreads)
-fps_kernel(const double* __restrict__ p, int64_t n, int32_t seeds, int32_t first, int32_t* __restrict__ out,
-           double* partial_v /* 2 x groups x 4 */, int64_t* partial_i /* 2 x groups */, unsigned* barrier,
-           int64_t chunk) {
-    __shared__ ArgMaxLds lds;
+fps_kernel(const double* __restrict__ p, int64_t n, int32_t begin, int32_t end, int32_t first,
+           int32_t* __restrict__ out, double* __restrict__ nearest_g, double* partial_v /* 2 x slots x 4 */,
+           int64_t* partial_i /* 2 x slots */, unsigned* barrier, int64_t chunk) {
+    // The sampling is one long dependent chain running beside the solve's
+    // kernels: its waves take issue priority on the units they share.
+    __builtin_amdgcn_s_setprio(3);
     const unsigned groups = gridDim.x;
-    const int64_t begin = static_cast<int64_t>(blockIdx.x) * chunk;
+    const unsigned slots = groups * kFpsWaves;
+    const unsigned my_slot = blockIdx.x * kFpsWaves + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * chunk;
     double nearest[kFpsPerThread];
     d3 mine[kFpsPerThread];
-    const d3 c0 = ld3(p, first);
     const ArgMax none{-1.0, 0x7fffffffffffffffll, d3{0.0, 0.0, 0.0}};
     ArgMax local = none;
+    const d3 c0 = ld3(p, begin == 0 ? first : out[begin - 1]);
 #pragma unroll
     for (int k = 0; k < kFpsPerThread; ++k) {
-        const int64_t i = begin + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
-        if (i < n && i < begin + chunk) {
+        const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
+        if (i < n && i < base + chunk) {
             mine[k] = ld3(p, i);
-            nearest[k] = norm2(mine[k] - c0);
+            nearest[k] = begin == 0 ? norm2(mine[k] - c0) : nearest_g[i];
             local = better(local, ArgMax{nearest[k], i, mine[k]});
         } else {
             mine[k] = d3{0.0, 0.0, 0.0};
             nearest[k] = -1.0;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = first;
-    for (int32_t s = 1; s < seeds; ++s) {
-        const ArgMax g = block_argmax(local, lds);
-        ArgMax best = g;
-        if (groups > 1) {
-            double* pv = partial_v + (s & 1) * 4 * groups;
-            int64_t* pi = partial_i + (s & 1) * groups;
-            if (threadIdx.x == 0) {
-                pv[4 * blockIdx.x] = g.value;
-                pv[4 * blockIdx.x + 1] = g.point.x;
-                pv[4 * blockIdx.x + 2] = g.point.y;
-                pv[4 * blockIdx.x + 3] = g.point.z;
-                pi[blockIdx.x] = g.index;
-            }
-            grid_barrier(barrier, groups);
-            ArgMax mine_slot = none;
-            if (threadIdx.x < groups) {
-                const int64_t t = threadIdx.x;
-                mine_slot = ArgMax{pv[4 * t], pi[t], d3{pv[4 * t + 1], pv[4 * t + 2], pv[4 * t + 3]}};
-            }
-            best = block_argmax(mine_slot, lds);
+    if (begin == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = first;
+    for (int32_t s = begin > 0 ? begin : 1; s < end; ++s) {
+        const ArgMax w = wave_argmax(local);
+        double* pv = partial_v + (s & 1) * 4 * slots;
+        int64_t* pi = partial_i + (s & 1) * slots;
+        if (lane == 0) {
+            pv[4 * my_slot] = w.value;
+            pv[4 * my_slot + 1] = w.point.x;
+            pv[4 * my_slot + 2] = w.point.y;
+            pv[4 * my_slot + 3] = w.point.z;
+            pi[my_slot] = w.index;
         }
+        grid_barrier(barrier, groups);
+        ArgMax best = none;
+        for (unsigned t = lane; t < slots; t += 64)
+            best = better(best, ArgMax{pv[4 * t], pi[t], d3{pv[4 * t + 1], pv[4 * t + 2], pv[4 * t + 3]}});
+        best = wave_argmax(best);
         if (blockIdx.x == 0 && threadIdx.x == 0) out[s] = static_cast<int32_t>(best.index);
         const d3 c = best.point;
         local = none;
@@ -202,10 +189,15 @@ fps_kernel(const double* __restrict__ p, int64_t n, int32_t seeds, int32_t first
             if (nearest[k] >= 0.0) {
                 const double d = norm2(mine[k] - c);
                 nearest[k] = fmin(nearest[k], d);
-                const int64_t i = begin + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
+                const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
                 local = better(local, ArgMax{nearest[k], i, mine[k]});
             }
         }
+    }
+#pragma unroll
+    for (int k = 0; k < kFpsPerThread; ++k) {
+        const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
+        if (nearest[k] >= 0.0) nearest_g[i] = nearest[k];
     }
 }
 
@@ -430,38 +422,40 @@ void retract(const double* R, const double* T, const double* delta, int64_t nn, 
     NRICP_LAUNCH_CHECK("retract_kernel");
 }
 
-void farthest_point_seeds(const double* points, int64_t n, int32_t seeds, int32_t first, int32_t* out,
-                          DeviceBuffer<double>& partial, DeviceBuffer<unsigned>& barrier, hipStream_t stream) {
-    if (n <= 0 || seeds <= 0) return;
-    // Enough groups that each thread holds at most kFpsPerThread points, and
-    // no more than can be co-resident.
+void farthest_point_seeds_range(const double* points, int64_t n, int32_t first, int32_t begin, int32_t end,
+                                int32_t* out, FpsWork& w, hipStream_t stream) {
+    if (n <= 0 || end <= begin) return;
+    if (first < 0 || first >= n) throw InvalidArgument("farthest_point_seeds: the first seed is out of range");
+    // Enough groups that each thread holds at most kFpsPerThread points,
+    // within the persistent-launch budget.
     const int64_t per_group = static_cast<int64_t>(kFpsThreads) * kFpsPerThread;
     const int64_t groups64 = (n + per_group - 1) / per_group;
-    int device = 0;
-    NRICP_CHECK(hipGetDevice(&device));
-    int cus = 0;
-    NRICP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    int per_cu = 0;
-    NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fps_kernel, kFpsThreads, 0));
-    const int64_t resident = static_cast<int64_t>(cus) * std::max(per_cu, 1);
-    if (groups64 > resident)
-        throw InvalidArgument("farthest_point_seeds: " + std::to_string(n) + " points exceed the resident capacity");
-    if (groups64 > kFpsThreads)
+    const int64_t budget = persistent_group_budget(reinterpret_cast<const void*>(fps_kernel), kFpsThreads);
+    if (groups64 > budget)
         throw InvalidArgument("farthest_point_seeds: " + std::to_string(n) + " points exceed one persistent launch");
     const unsigned groups = static_cast<unsigned>(groups64);
     const int64_t chunk = per_group;
-    partial.resize(10 * static_cast<int64_t>(groups));  // 2 buffers x (value, x, y, z, index)
-    barrier.resize(2);
-    barrier.zero(stream);
-    double* pv = partial.get();
-    int64_t* pi = reinterpret_cast<int64_t*>(partial.get() + 8 * groups);
-    int32_t s = seeds;
-    int32_t f = first;
+    if (begin == 0) {
+        w.nearest.resize(n);
+        w.partial.resize(10 * static_cast<int64_t>(groups) * kFpsWaves);  // 2 buffers x slots x (value, x, y, z, index)
+        w.barrier.resize(2);
+        w.barrier.zero(stream);
+    }
+    double* pv = w.partial.get();
+    int64_t* pi = reinterpret_cast<int64_t*>(w.partial.get() + 8 * static_cast<int64_t>(groups) * kFpsWaves);
+    int32_t b = begin, e = end, f = first;
     int64_t nn = n;
-    unsigned* bar = barrier.get();
-    void* args[] = {const_cast<double**>(&points), &nn, &s, &f, &out, &pv, &pi, &bar, const_cast<int64_t*>(&chunk)};
-    NRICP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fps_kernel), dim3(groups), dim3(kFpsThreads),
-                                           args, 0, stream));
+    double* near = w.nearest.get();
+    unsigned* bar = w.barrier.get();
+    void* args[] = {const_cast<double**>(&points), &nn, &b, &e, &f, &out, &near, &pv, &pi, &bar,
+                    const_cast<int64_t*>(&chunk)};
+    launch_persistent(reinterpret_cast<const void*>(fps_kernel), static_cast<int>(groups), kFpsThreads, args, stream);
+}
+
+void farthest_point_seeds(const double* points, int64_t n, int32_t seeds, int32_t first, int32_t* out, FpsWork& w,
+                          hipStream_t stream) {
+    if (n <= 0 || seeds <= 0) return;
+    farthest_point_seeds_range(points, n, first, 0, seeds, out, w, stream);
 }
 
 void bind_vertices(const double* points, int64_t n, const double* nodes, int64_t nn, int32_t k, int32_t* on,
@@ -491,11 +485,12 @@ int64_t build_node_edges(const double* nodes, int64_t nn, int32_t k, DeviceBuffe
     else
         knn_kernel<16><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
     NRICP_LAUNCH_CHECK("knn_kernel");
-    radix_sort_pairs(keys.get(), nullptr, m, 64, scratch, stream);
+    radix_sort_pairs(keys.get(), nullptr, m, bits_for(static_cast<uint64_t>(nn) * static_cast<uint64_t>(nn)), scratch,
+                     stream);
     flags.resize(m + 1);
     pos.resize(m + 1);
     run_starts(keys.get(), m, flags.get(), stream);
-    exclusive_scan32(flags.get(), pos.get(), m, stream);
+    exclusive_scan32(flags.get(), pos.get(), m, scratch, stream);
     int32_t count = 0;
     NRICP_CHECK(hipMemcpyAsync(&count, pos.get() + m, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     NRICP_CHECK(hipStreamSynchronize(stream));

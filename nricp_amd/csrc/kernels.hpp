@@ -19,6 +19,8 @@ namespace nricp_hip {
 struct VertexFaces {
     DeviceBuffer<int32_t> offsets;  // V + 1
     DeviceBuffer<int32_t> faces;    // 3F
+    DeviceBuffer<uint64_t> keys;    // scratch: (vertex, face) entries
+    DeviceBuffer<uint32_t> vals;
 };
 void build_vertex_faces(const int32_t* faces, int64_t face_count, int64_t vertex_count, VertexFaces& out,
                         SortScratch& scratch, hipStream_t stream);
@@ -69,7 +71,8 @@ struct Lbvh {
     DeviceBuffer<uint64_t> keys;
     DeviceBuffer<int32_t> flags;
     DeviceBuffer<double> centroids;
-    DeviceBuffer<double> bounds;  // partial boxes
+    DeviceBuffer<double> bounds;      // partial boxes
+    DeviceBuffer<double> face_cross;  // scratch for the target normals
 };
 // Builds the hierarchy over a target mesh already on the device (points,
 // faces copied into the Lbvh). Morton codes (10 bits per axis) of the triangle
@@ -127,9 +130,19 @@ void retract(const double* rotations, const double* translations, const double* 
 
 // ------------------------------------------------------------- graph -------
 
-// Farthest-point seeds (first maximum wins), as a persistent cooperative grid.
+// Farthest-point seeds (first maximum wins), as a persistent grid.
+// The running state lives in FpsWork, so the seeds can be computed in ranges:
+// [0, a) then [a, b) gives the same seeds as [0, b) (coarser graph levels use
+// prefixes of the finest level's seeds and can start before it is done).
+struct FpsWork {
+    DeviceBuffer<double> nearest;  // running nearest squared distance per point
+    DeviceBuffer<double> partial;
+    DeviceBuffer<unsigned> barrier;
+};
 void farthest_point_seeds(const double* points, int64_t count, int32_t seeds, int32_t first, int32_t* out,
-                          DeviceBuffer<double>& partial, DeviceBuffer<unsigned>& barrier, hipStream_t stream);
+                          FpsWork& work, hipStream_t stream);
+void farthest_point_seeds_range(const double* points, int64_t count, int32_t first, int32_t begin, int32_t end,
+                                int32_t* out, FpsWork& work, hipStream_t stream);
 // Nearest `k` nodes by (d^2, index), weights (1 - d/dmax)^2 normalised.
 void bind_vertices(const double* points, int64_t count, const double* nodes, int64_t node_count, int32_t k,
                    int32_t* out_nodes, double* out_weights, int32_t* out_counts, hipStream_t stream);
@@ -163,8 +176,8 @@ struct LevelLayout {
     DeviceBuffer<int32_t> diag_position;   // N
     DeviceBuffer<int32_t> pair_position;   // P x 2: block (a,b), block (b,a)
     // scratch
-    DeviceBuffer<uint64_t> keys;
-    DeviceBuffer<uint32_t> vals;
+    DeviceBuffer<uint64_t> keys, block_keys;
+    DeviceBuffer<uint32_t> vals, block_vals;
     DeviceBuffer<int32_t> counts, offsets, flags, runs;
 };
 void build_level_layout(const int32_t* bind_nodes, const double* bind_weights, const int32_t* bind_counts,

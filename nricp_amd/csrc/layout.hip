@@ -107,28 +107,30 @@ __global__ void set_last(int32_t* __restrict__ a, int64_t index, int32_t value) 
     if (threadIdx.x == 0 && blockIdx.x == 0) a[index] = value;
 }
 
-__global__ void emit_blocks(const int32_t* __restrict__ pairs, int64_t np, int64_t nn, uint64_t* __restrict__ key,
-                            uint32_t* __restrict__ val) {
+// Block keys row << bits | column, so a sort on 2 * bits bits orders rows
+// and, within a row, columns.
+__global__ void emit_blocks(const int32_t* __restrict__ pairs, int64_t np, int64_t nn, int bits,
+                            uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < nn) {
-        key[i] = static_cast<uint64_t>(i) << 32 | static_cast<uint64_t>(i);
+        key[i] = static_cast<uint64_t>(i) << bits | static_cast<uint64_t>(i);
         val[i] = static_cast<uint32_t>(i << 2);
     }
     if (i < np) {
         const uint64_t a = static_cast<uint64_t>(pairs[2 * i]), b = static_cast<uint64_t>(pairs[2 * i + 1]);
-        key[nn + 2 * i] = a << 32 | b;
+        key[nn + 2 * i] = a << bits | b;
         val[nn + 2 * i] = static_cast<uint32_t>(i << 2 | 1);
-        key[nn + 2 * i + 1] = b << 32 | a;
+        key[nn + 2 * i + 1] = b << bits | a;
         val[nn + 2 * i + 1] = static_cast<uint32_t>(i << 2 | 2);
     }
 }
 
 __global__ void block_positions(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int64_t n,
-                                int32_t* __restrict__ columns, int32_t* __restrict__ diag_pos,
+                                uint64_t column_mask, int32_t* __restrict__ columns, int32_t* __restrict__ diag_pos,
                                 int32_t* __restrict__ pair_pos) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    columns[k] = static_cast<int32_t>(key[k] & 0xffffffffull);
+    columns[k] = static_cast<int32_t>(key[k] & column_mask);
     const uint32_t v = val[k];
     const uint32_t kind = v & 3u, index = v >> 2;
     if (kind == 0)
@@ -163,8 +165,8 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
         count_rows<<<blocks_for(nv, 256), 256, 0, stream>>>(bn, bw, bc, nv, nn, gcount, pcount);
         NRICP_LAUNCH_CHECK("count_rows");
     }
-    exclusive_scan32(gcount, gpos, nv, stream);
-    exclusive_scan32(pcount, ppos, nv, stream);
+    exclusive_scan32(gcount, gpos, nv, scratch, stream);
+    exclusive_scan32(pcount, ppos, nv, scratch, stream);
     const int64_t ng = read_i32(gpos + nv, stream);
     const int64_t nvp = read_i32(ppos + nv, stream);
     const int64_t np_entries = nvp + ne;
@@ -200,13 +202,17 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     L.node_edge_offsets.resize(nn + 1);
     L.node_edge_entries.resize(std::max<int64_t>(2 * ne, 1));
     if (ne > 0) {
-        radix_sort_pairs(ekey, eval, 2 * ne, 64, scratch, stream);
+        // Keys are node ids, or all ones for an invalid end: the low node_bits
+        // bits already order them (all ones exceeds every node id).
+        radix_sort_pairs(ekey, eval, 2 * ne, node_bits, scratch, stream);
         to_entries<<<blocks_for(2 * ne, 256), 256, 0, stream>>>(eval, 2 * ne, L.node_edge_entries.get());
         NRICP_LAUNCH_CHECK("to_entries");
     }
     offsets_from_sorted(ekey, 2 * ne, 0, ~0ull, nn, L.node_edge_offsets.get(), stream);
     // 5. pair groups. Invalid and self edges carry the key ~0 and sort last.
-    radix_sort_pairs(pkey, pval, np_entries, 64, scratch, stream);
+    // Keys are a * N + b < N^2, or all ones: the bits of N^2 order them.
+    radix_sort_pairs(pkey, pval, np_entries, bits_for(static_cast<uint64_t>(nn) * static_cast<uint64_t>(nn)), scratch,
+                     stream);
     L.flags.resize(np_entries + 1);
     L.runs.resize(np_entries + 1);
     int64_t valid_entries = np_entries;
@@ -223,7 +229,7 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
         }
     }
     run_starts(pkey, valid_entries, L.flags.get(), stream);
-    exclusive_scan32(L.flags.get(), L.runs.get(), valid_entries, stream);
+    exclusive_scan32(L.flags.get(), L.runs.get(), valid_entries, scratch, stream);
     const int64_t np = valid_entries > 0 ? read_i32(L.runs.get() + valid_entries, stream) : 0;
     L.group_count = np;
     L.group_offsets.resize(np + 1);
@@ -241,20 +247,23 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     // 6. block-sparse rows of the symmetric matrix.
     const int64_t nnzb = nn + 2 * np;
     L.nnzb = nnzb;
-    DeviceBuffer<uint64_t> bkey(nnzb);
-    DeviceBuffer<uint32_t> bval(nnzb);
-    emit_blocks<<<blocks_for(std::max(nn, np), 256), 256, 0, stream>>>(L.group_pairs.get(), np, nn, bkey.get(), bval.get());
+    L.block_keys.resize(nnzb);
+    L.block_vals.resize(nnzb);
+    uint64_t* bkey = L.block_keys.get();
+    uint32_t* bval = L.block_vals.get();
+    emit_blocks<<<blocks_for(std::max(nn, np), 256), 256, 0, stream>>>(L.group_pairs.get(), np, nn, node_bits, bkey,
+                                                                        bval);
     NRICP_LAUNCH_CHECK("emit_blocks");
-    radix_sort_pairs(bkey.get(), bval.get(), nnzb, 32 + node_bits, scratch, stream);
+    radix_sort_pairs(bkey, bval, nnzb, 2 * node_bits, scratch, stream);
     L.row_offsets.resize(nn + 1);
     L.columns.resize(nnzb);
     L.diag_position.resize(nn);
     L.pair_position.resize(std::max<int64_t>(2 * np, 1));
-    offsets_from_sorted(bkey.get(), nnzb, 32, 0xffffffffull, nn, L.row_offsets.get(), stream);
-    block_positions<<<blocks_for(nnzb, 256), 256, 0, stream>>>(bkey.get(), bval.get(), nnzb, L.columns.get(),
+    const uint64_t column_mask = (1ull << node_bits) - 1;
+    offsets_from_sorted(bkey, nnzb, node_bits, column_mask, nn, L.row_offsets.get(), stream);
+    block_positions<<<blocks_for(nnzb, 256), 256, 0, stream>>>(bkey, bval, nnzb, column_mask, L.columns.get(),
                                                                 L.diag_position.get(), L.pair_position.get());
     NRICP_LAUNCH_CHECK("block_positions");
-    NRICP_CHECK(hipStreamSynchronize(stream));  // bkey/bval are released on return
 }
 
 }  // namespace nricp_hip

@@ -1,5 +1,5 @@
 // Block-Jacobi preconditioned conjugate gradients on the damped block system,
-// as one persistent cooperative launch per solve.
+// as one persistent launch per solve.
 //
 // Reference behaviour: replaces src/cuda/pcg_solver.cu:15-326 (block-Jacobi
 // PCG from x0 = 0, stop when |r|/|b| <= tol, ok <=> converged). Differences,
@@ -30,6 +30,7 @@ namespace {
 
 constexpr int kPcgThreads = 512;
 constexpr int kNodesPerPass = kPcgThreads / 8;  // 8 lanes per node, 6 active
+constexpr int kBarrierGroups = 32;
 
 // Inverse of the symmetric 6x6 block `a` (lower triangle read) plus `jitter`
 // on its diagonal, through its Cholesky factor: L L^T = a, X = L^-1,
@@ -127,24 +128,32 @@ __device__ inline void sync_all(unsigned* barrier) {
         __syncthreads();
 }
 
-// Reproducible sums over the whole launch. publish(): the block's sum of the
-// per-thread partials goes to the block's slot. After a grid barrier, fold():
-// thread t loads slot t and the block adds them in its fixed tree order, so
-// every block obtains the same bits. Needs gridDim.x <= kPcgThreads.
+// Reproducible sums over the whole launch. publish(): each wave's sum of the
+// per-thread partials goes to the wave's slot. After a grid barrier, fold():
+// every wave loads all slots (lane-strided) and finishes with a butterfly, so
+// every wave of every group obtains the same bits without any LDS round.
+constexpr int kPcgWaves = kPcgThreads / kWave;
+
 template <int Count>
-__device__ inline void publish(const double (&v)[Count], double* slots, double* lds) {
-    double blk[Count];
-    block_sum<Count, kPcgThreads>(v, lds, blk);
-    if (threadIdx.x == 0)
+__device__ inline void publish(const double (&v)[Count], double* slots) {
+    const unsigned slot = blockIdx.x * kPcgWaves + threadIdx.x / kWave;
 #pragma unroll
-        for (int k = 0; k < Count; ++k) slots[Count * blockIdx.x + k] = blk[k];
+    for (int k = 0; k < Count; ++k) {
+        const double w = wave_sum(v[k]);
+        if (threadIdx.x % kWave == 0) slots[Count * slot + k] = w;
+    }
 }
 template <int Count>
-__device__ inline void fold(const double* slots, double* lds, double (&out)[Count]) {
-    double v[Count];
+__device__ inline void fold(const double* slots, double (&out)[Count]) {
+    const unsigned nslots = gridDim.x * kPcgWaves;
+    double acc[Count];
 #pragma unroll
-    for (int k = 0; k < Count; ++k) v[k] = threadIdx.x < gridDim.x ? slots[Count * threadIdx.x + k] : 0.0;
-    block_sum<Count, kPcgThreads>(v, lds, out);
+    for (int k = 0; k < Count; ++k) acc[k] = 0.0;
+    for (unsigned t = threadIdx.x % kWave; t < nslots; t += kWave)
+#pragma unroll
+        for (int k = 0; k < Count; ++k) acc[k] += slots[Count * t + k];
+#pragma unroll
+    for (int k = 0; k < Count; ++k) out[k] = wave_sum(acc[k]);
 }
 
 // One launch-wide sum with its own synchronisation (used where no vector
@@ -156,9 +165,9 @@ __device__ inline void launch_sum(const double (&v)[Count], double* slots, doubl
         block_sum<Count, kPcgThreads>(v, lds, out);
         return;
     }
-    publish<Count>(v, slots, lds);
+    publish<Count>(v, slots);
     grid_barrier(barrier, gridDim.x);
-    fold<Count>(slots, lds, out);
+    fold<Count>(slots, out);
 }
 
 // Row `lane` of the node's preconditioner block times the node's 6-vector,
@@ -236,7 +245,7 @@ __global__ void __launch_bounds__(kPcgThreads)
 pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
            const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ rhs, int64_t nn,
            double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
-           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
+           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x waves */,
            unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
     __shared__ double lds[3 * (kPcgThreads / kWave)];
     const int lane = threadIdx.x & 7;
@@ -267,7 +276,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     {
         const double v[1] = {bb};
         double o[1];
-        launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+        launch_sum<1, Grid>(v, slots + phase * 3 * kPcgWaves * gridDim.x, lds, barrier, o);
         phase ^= 1;
         red[0] = o[0];
     }
@@ -313,9 +322,9 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
                 part[2] += rk * rk;
             }
             if (Grid) {
-                publish<3>(part, slots + phase * 3 * gridDim.x, lds);
+                publish<3>(part, slots + phase * 3 * kPcgWaves * gridDim.x);
                 grid_barrier(barrier, gridDim.x);
-                fold<3>(slots + phase * 3 * gridDim.x, lds, red);
+                fold<3>(slots + phase * 3 * kPcgWaves * gridDim.x, red);
             } else {
                 block_sum<3, kPcgThreads>(part, lds, red);
             }
@@ -381,7 +390,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         {
             const double v[1] = {rr};
             double o[1];
-            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            launch_sum<1, Grid>(v, slots + phase * 3 * kPcgWaves * gridDim.x, lds, barrier, o);
             phase ^= 1;
             red[0] = o[0];
         }
@@ -410,16 +419,6 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     }
 }
 
-int resident_groups(const void* kernel) {
-    int device = 0;
-    NRICP_CHECK(hipGetDevice(&device));
-    int cus = 0;
-    NRICP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    int per_cu = 0;
-    NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kPcgThreads, 0));
-    return cus * std::max(per_cu, 1);
-}
-
 // Node slots per lane group K: at most 2 (register budget: every CG vector
 // and the preconditioner rows stay in registers). Systems that fit one workgroup run without grid barriers; larger
 // ones spread one slot per lane group over as many groups as needed (at most
@@ -442,10 +441,12 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
         while (k < passes) k *= 2;
         grid = false;
     } else {
-        const int limit = std::min(resident_groups(reinterpret_cast<const void*>(pcg_kernel<1, S, true>)),
-                                   kPcgThreads);
+        const int limit = std::min(
+            persistent_group_budget(reinterpret_cast<const void*>(pcg_kernel<2, S, true>), kPcgThreads), kPcgThreads);
         int64_t need = passes;
-        while (need > limit && k < kmax) {
+        // A grid barrier costs more the more groups arrive at it; past
+        // kBarrierGroups groups two node slots per lane group pay off.
+        while ((need > limit || need > kBarrierGroups) && k < kmax) {
             k *= 2;
             need = (passes + k - 1) / k;
         }
@@ -460,7 +461,7 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
         kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, false>)
                         : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
     }
-    ws.reduce.resize(6 * static_cast<int64_t>(groups));
+    ws.reduce.resize(6 * static_cast<int64_t>(kPcgWaves) * groups);
     ws.barrier.resize(2);
     ws.barrier.zero(stream);
     ws.outcome.resize(4);
@@ -478,7 +479,7 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
                     &inner_tol, &max_iter, &vb, &xbuf, &slots, &bar, &solution, &out};
     if (ws.kernel_start) NRICP_CHECK(hipEventRecord(ws.kernel_start, stream));
     if (grid) {
-        NRICP_CHECK(hipLaunchCooperativeKernel(kernel, dim3(groups), dim3(kPcgThreads), args, 0, stream));
+        launch_persistent(kernel, groups, kPcgThreads, args, stream);
     } else {
         NRICP_CHECK(hipLaunchKernel(kernel, dim3(1), dim3(kPcgThreads), args, 0, stream));
     }

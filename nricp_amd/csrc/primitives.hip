@@ -1,11 +1,14 @@
 // Stable LSD radix sort, exclusive scan, CSR offsets and run flags for gfx950.
 //
-// Sort: 8-bit digits. Each pass is three launches -- a per-tile digit
-// histogram in LDS, one exclusive scan over the digit-major table of tile
-// counts, and a stable scatter in which every wave ranks its keys with eight
+// Sort: 8-bit digits. Each pass is a per-tile digit histogram in LDS, one
+// exclusive scan over the digit-major table of tile counts, and a stable
+// scatter in which every wave ranks its keys with eight
 // ballots (the lanes holding the same digit) and waves are ordered through LDS.
 // A tile is 256 threads x 8 rounds = 2048 keys.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
 
 #include "primitives.hpp"
 
@@ -30,31 +33,6 @@ digit_histogram(const uint64_t* __restrict__ keys, int64_t n, int shift, uint32_
     }
     __syncthreads();
     counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of n uint32 in place, one workgroup of 1024 threads: each
-// thread scans a contiguous chunk, the chunk totals are scanned in LDS.
-__global__ void __launch_bounds__(1024) scan_u32_inplace(uint32_t* data, int64_t n) {
-    __shared__ uint32_t totals[1024];
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t begin = threadIdx.x * chunk;
-    const int64_t end = begin + chunk < n ? begin + chunk : n;
-    uint32_t sum = 0;
-    for (int64_t i = begin; i < end; ++i) sum += data[i];
-    totals[threadIdx.x] = sum;
-    __syncthreads();
-    for (int offset = 1; offset < 1024; offset <<= 1) {
-        const uint32_t v = threadIdx.x >= offset ? totals[threadIdx.x - offset] : 0u;
-        __syncthreads();
-        totals[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x > 0 ? totals[threadIdx.x - 1] : 0u;
-    for (int64_t i = begin; i < end; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
 }
 
 __device__ inline uint64_t lanemask_lt() {
@@ -114,29 +92,120 @@ __global__ void copy_pairs(const uint64_t* __restrict__ k, const uint32_t* __res
     if (v) vo[i] = v[i];
 }
 
+// ------------------------------------------------------------------ scan --
+//
+// Exclusive scan in three launches: per-tile sums, a scan of the tile sums
+// (one workgroup, chunked with a carry), and a per-tile scan that adds the
+// tile's offset. A tile is 256 threads x 16 items, loaded striped (coalesced)
+// into LDS and scanned in contiguous runs of 16 per thread. Every sum has a
+// fixed order, and each tile is read whole before it is written, so the scan
+// may run in place.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one value per thread over the block; returns the block
+// total through `total`.
 template <typename T>
-__global__ void __launch_bounds__(1024) scan_exclusive(const T* __restrict__ in, T* __restrict__ out, int64_t n) {
-    __shared__ T totals[1024];
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t begin = threadIdx.x * chunk;
-    const int64_t end = begin + chunk < n ? begin + chunk : n;
-    T sum = 0;
-    for (int64_t i = begin; i < end; ++i) sum += in[i];
-    totals[threadIdx.x] = sum;
+__device__ inline T block_exclusive(T v, T* wave_totals, T& total) {
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    T incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wave_totals[wave] = incl;
     __syncthreads();
-    for (int offset = 1; offset < 1024; offset <<= 1) {
-        const T v = threadIdx.x >= offset ? totals[threadIdx.x - offset] : T(0);
-        __syncthreads();
-        totals[threadIdx.x] += v;
-        __syncthreads();
+    T before = 0;
+    total = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) before += wave_totals[w];
+        total += wave_totals[w];
     }
-    T run = threadIdx.x > 0 ? totals[threadIdx.x - 1] : T(0);
-    for (int64_t i = begin; i < end; ++i) {
-        const T v = in[i];
-        out[i] = run;
-        run += v;
+    __syncthreads();
+    return before + incl - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads) scan_tile_sums(const T* __restrict__ in, int64_t n,
+                                                               T* __restrict__ partials) {
+    __shared__ T wave_totals[kScanThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kScanThreads + threadIdx.x;
+        if (i < n) sum += in[i];
     }
-    if (threadIdx.x == 1023) out[n] = totals[1023];
+    T total;
+    block_exclusive(sum, wave_totals, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// In-place exclusive scan of the tile sums by one workgroup.
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads) scan_partials(T* __restrict__ partials, int64_t tiles) {
+    __shared__ T wave_totals[kScanThreads / 64];
+    T carry = 0;
+    for (int64_t base = 0; base < tiles; base += kScanThreads) {
+        const int64_t i = base + threadIdx.x;
+        const T v = i < tiles ? partials[i] : T(0);
+        T total;
+        const T ex = block_exclusive(v, wave_totals, total);
+        if (i < tiles) partials[i] = carry + ex;
+        carry += total;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads)
+scan_tiles(const T* in, T* out, int64_t n, const T* __restrict__ partials, T* total_out) {
+    __shared__ T tile[kScanTile];
+    __shared__ T wave_totals[kScanThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kScanThreads + threadIdx.x;
+        tile[k * kScanThreads + threadIdx.x] = i < n ? in[i] : T(0);
+    }
+    __syncthreads();
+    T run[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        run[k] = sum;
+        sum += tile[threadIdx.x * kScanItems + k];
+    }
+    T total;
+    const T offset = partials[blockIdx.x] + block_exclusive(sum, wave_totals, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) tile[threadIdx.x * kScanItems + k] = offset + run[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kScanThreads + threadIdx.x;
+        if (i < n) out[i] = tile[k * kScanThreads + threadIdx.x];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = partials[blockIdx.x] + total;
+}
+
+template <typename T>
+void scan_exclusive(const T* in, T* out, int64_t n, T* total_out, DeviceBuffer<int64_t>& scratch,
+                    hipStream_t stream) {
+    if (n <= 0) {
+        if (total_out) NRICP_CHECK(hipMemsetAsync(total_out, 0, sizeof(T), stream));
+        return;
+    }
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    scratch.resize((tiles * static_cast<int64_t>(sizeof(T)) + 7) / 8);
+    T* partials = reinterpret_cast<T*>(scratch.get());
+    scan_tile_sums<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, stream>>>(in, n, partials);
+    NRICP_LAUNCH_CHECK("scan_tile_sums");
+    scan_partials<T><<<1, kScanThreads, 0, stream>>>(partials, tiles);
+    NRICP_LAUNCH_CHECK("scan_partials");
+    scan_tiles<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, stream>>>(in, out, n, partials, total_out);
+    NRICP_LAUNCH_CHECK("scan_tiles");
 }
 
 __global__ void sorted_offsets(const uint64_t* __restrict__ keys, int64_t n, int shift, uint64_t mask,
@@ -162,6 +231,23 @@ __global__ void run_start_flags(const uint64_t* __restrict__ keys, int64_t n, in
 
 }  // namespace
 
+int persistent_group_budget(const void* kernel, int threads) {
+    int device = 0;
+    NRICP_CHECK(hipGetDevice(&device));
+    int cus = 0;
+    NRICP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    int per_cu = 0;
+    NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    if (per_cu < 1) throw DeviceError("a persistent kernel does not fit on a compute unit");
+    return std::max(1, cus / 2);
+}
+
+void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream) {
+    if (groups < 1 || groups > persistent_group_budget(kernel, threads))
+        throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
+    NRICP_CHECK(hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream));
+}
+
 int bits_for(uint64_t v) {
     int b = 0;
     while (b < 64 && (v >> b) != 0) ++b;
@@ -185,8 +271,8 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, S
         const int shift = 8 * p;
         digit_histogram<<<tiles, kSortThreads, 0, stream>>>(kin, n, shift, s.counts.get(), tiles);
         NRICP_LAUNCH_CHECK("digit_histogram");
-        scan_u32_inplace<<<1, 1024, 0, stream>>>(s.counts.get(), static_cast<int64_t>(kRadix) * tiles);
-        NRICP_LAUNCH_CHECK("scan_u32_inplace");
+        scan_exclusive<uint32_t>(s.counts.get(), s.counts.get(), static_cast<int64_t>(kRadix) * tiles, nullptr,
+                                 s.scan_partials, stream);
         digit_scatter<<<tiles, kSortThreads, 0, stream>>>(kin, vin, n, shift, s.counts.get(), tiles, kout, vout);
         NRICP_LAUNCH_CHECK("digit_scatter");
         std::swap(kin, kout);
@@ -198,14 +284,12 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, S
     }
 }
 
-void exclusive_scan(const int64_t* in, int64_t* out, int64_t n, hipStream_t stream) {
-    scan_exclusive<int64_t><<<1, 1024, 0, stream>>>(in, out, n);
-    NRICP_LAUNCH_CHECK("scan_exclusive64");
+void exclusive_scan(const int64_t* in, int64_t* out, int64_t n, SortScratch& s, hipStream_t stream) {
+    scan_exclusive<int64_t>(in, out, n, out + n, s.scan_partials, stream);
 }
 
-void exclusive_scan32(const int32_t* in, int32_t* out, int64_t n, hipStream_t stream) {
-    scan_exclusive<int32_t><<<1, 1024, 0, stream>>>(in, out, n);
-    NRICP_LAUNCH_CHECK("scan_exclusive32");
+void exclusive_scan32(const int32_t* in, int32_t* out, int64_t n, SortScratch& s, hipStream_t stream) {
+    scan_exclusive<int32_t>(in, out, n, out + n, s.scan_partials, stream);
 }
 
 void offsets_from_sorted(const uint64_t* keys, int64_t n, int shift, uint64_t mask, int64_t rows,

@@ -16,6 +16,7 @@ struct SortScratch {
     DeviceBuffer<uint64_t> keys_alt;
     DeviceBuffer<uint32_t> vals_alt;
     DeviceBuffer<uint32_t> counts;
+    DeviceBuffer<int64_t> scan_partials;  // tile sums of the scans
 };
 
 // Sorts keys[0..n) (and vals alongside, when non-null) by the low `key_bits`
@@ -23,10 +24,10 @@ struct SortScratch {
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, SortScratch& scratch,
                       hipStream_t stream);
 
-// out[i] = sum of in[0..i), out[n] = total; int64 throughout. `out` has n + 1
-// entries. Single pass, single workgroup: meant for the CSR sizes of a level.
-void exclusive_scan(const int64_t* in, int64_t* out, int64_t n, hipStream_t stream);
-void exclusive_scan32(const int32_t* in, int32_t* out, int64_t n, hipStream_t stream);
+// out[i] = sum of in[0..i), out[n] = total. `out` has n + 1 entries and may
+// alias `in`. Multi-workgroup, fixed summation order.
+void exclusive_scan(const int64_t* in, int64_t* out, int64_t n, SortScratch& scratch, hipStream_t stream);
+void exclusive_scan32(const int32_t* in, int32_t* out, int64_t n, SortScratch& scratch, hipStream_t stream);
 
 // offsets[r] = first index i with (keys[i] >> shift) >= r, r in [0, rows];
 // keys sorted ascending.

@@ -90,6 +90,25 @@ std::string escape(const std::string& t) {
 
 // ------------------------------------------------------------------ graph --
 
+void build_graph_level(const double* points, int64_t nv, const int32_t* seeds, int64_t node_count, int k,
+                       int neighbours, GraphLevel& L, SortScratch& scratch, hipStream_t stream) {
+    L.node_count = node_count;
+    L.node_indices.resize(node_count);
+    NRICP_CHECK(hipMemcpyAsync(L.node_indices.get(), seeds, node_count * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                               stream));
+    L.node_positions.resize(3 * node_count);
+    gather_rows3<<<blocks_for(node_count, 256), 256, 0, stream>>>(points, L.node_indices.get(), node_count,
+                                                                   L.node_positions.get());
+    NRICP_LAUNCH_CHECK("gather_rows3");
+    L.bind_nodes.resize(4 * nv);
+    L.bind_weights.resize(4 * nv);
+    L.bind_counts.resize(nv);
+    bind_vertices(points, nv, L.node_positions.get(), node_count, k, L.bind_nodes.get(), L.bind_weights.get(),
+                  L.bind_counts.get(), stream);
+    L.edge_count = build_node_edges(L.node_positions.get(), node_count, neighbours, L.edges, L.edge_keys,
+                                    L.edge_flags, L.edge_pos, scratch, stream);
+}
+
 void build_device_graph(const double* points, int64_t nv, int levels, int n0, int growth, int k, int neighbours,
                         DeviceGraph& g, SortScratch& scratch, hipStream_t stream) {
     if (nv <= 0) throw InvalidArgument("the template has no vertices");
@@ -100,29 +119,11 @@ void build_device_graph(const double* points, int64_t nv, int levels, int n0, in
     int64_t finest = 0;
     for (int l = 0; l < lv; ++l) finest = std::max(finest, level_count_for(lv, n0, growth, nv, l));
     DeviceBuffer<int32_t> seeds(finest);
-    DeviceBuffer<double> partial;
-    DeviceBuffer<unsigned> barrier;
-    farthest_point_seeds(points, nv, static_cast<int32_t>(finest), 0, seeds.get(), partial, barrier, stream);
-    DeviceBuffer<uint64_t> keys;
-    DeviceBuffer<int32_t> flags, pos;
-    for (int l = 0; l < lv; ++l) {
-        GraphLevel& L = g.levels[l];
-        L.node_count = level_count_for(lv, n0, growth, nv, l);
-        L.node_indices.resize(L.node_count);
-        NRICP_CHECK(hipMemcpyAsync(L.node_indices.get(), seeds.get(), L.node_count * sizeof(int32_t),
-                                   hipMemcpyDeviceToDevice, stream));
-        L.node_positions.resize(3 * L.node_count);
-        gather_rows3<<<blocks_for(L.node_count, 256), 256, 0, stream>>>(points, L.node_indices.get(), L.node_count,
-                                                                         L.node_positions.get());
-        NRICP_LAUNCH_CHECK("gather_rows3");
-        L.bind_nodes.resize(4 * nv);
-        L.bind_weights.resize(4 * nv);
-        L.bind_counts.resize(nv);
-        bind_vertices(points, nv, L.node_positions.get(), L.node_count, k, L.bind_nodes.get(), L.bind_weights.get(),
-                      L.bind_counts.get(), stream);
-        L.edge_count = build_node_edges(L.node_positions.get(), L.node_count, neighbours, L.edges, keys, flags, pos,
-                                        scratch, stream);
-    }
+    FpsWork work;
+    farthest_point_seeds(points, nv, static_cast<int32_t>(finest), 0, seeds.get(), work, stream);
+    for (int l = 0; l < lv; ++l)
+        build_graph_level(points, nv, seeds.get(), level_count_for(lv, n0, growth, nv, l), k, neighbours, g.levels[l],
+                          scratch, stream);
     NRICP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -206,8 +207,19 @@ void KernelClock::end(hipStream_t s) {
     NRICP_CHECK(hipEventRecord(pending_.back().stop, s));
 }
 
-void KernelClock::collect() {
+void KernelClock::collect(bool wait) {
+    // Entries whose stop has not been reached yet (work still running on
+    // another stream) stay pending unless the caller waits for them.
+    std::vector<Pending> still;
     for (Pending& p : pending_) {
+        if (!wait) {
+            const hipError_t q = hipEventQuery(p.stop);
+            if (q == hipErrorNotReady) {
+                still.push_back(p);
+                continue;
+            }
+            NRICP_CHECK(q);
+        }
         float ms = 0.0f;
         NRICP_CHECK(hipEventSynchronize(p.stop));
         NRICP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
@@ -217,7 +229,7 @@ void KernelClock::collect() {
         pool_.push_back(p.start);
         pool_.push_back(p.stop);
     }
-    pending_.clear();
+    pending_.swap(still);
 }
 
 KernelClock::~KernelClock() {
@@ -286,6 +298,11 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         const int lv = std::max(1, s.graph_levels);
         for (int l = 0; l < lv; ++l) nmax = std::max(nmax, level_count_for(lv, s.nodes_initial, s.growth, nv_, l));
         graph_ = &own_graph_;
+        own_graph_.vertex_count = nv_;
+        own_graph_.levels.resize(lv);
+        seeds_.resize(nmax);
+        fps_ready_.resize(lv);
+        for (auto& e : fps_ready_) NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     if (s.max_nodes_per_vertex < 1) throw InvalidArgument("max_nodes_per_vertex must be at least 1");
     if (s.graph_neighbours < 0 || s.graph_neighbours > 16) throw InvalidArgument("graph_neighbours must be in [0, 16]");
@@ -307,6 +324,20 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     quantiles_.resize(2);
     NRICP_CHECK(hipStreamSynchronize(st));
     report_.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+Session::~Session() {
+    (void)hipStreamSynchronize(fps_stream_.get());
+    (void)hipStreamSynchronize(stream_.get());
+    for (hipEvent_t e : fps_ready_) (void)hipEventDestroy(e);
+}
+
+KernelClock& Session::clock() {
+    // Everything timed so far, including work still on the sampling stream.
+    NRICP_CHECK(hipStreamSynchronize(fps_stream_.get()));
+    NRICP_CHECK(hipStreamSynchronize(stream_.get()));
+    clock_.collect(true);
+    return clock_;
 }
 
 const nricp_hip_level_settings& Session::level_settings() const {
@@ -344,7 +375,7 @@ AssemblyInputs Session::inputs(const double* R, const double* T) const {
 
 void Session::sync() {
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
-    clock_.collect();
+    clock_.collect(false);
     if (stats_pending_) {
         const std::vector<uint64_t> h = query_stats_.to_host(stream_.get());
         counters_[kQueryNodeOpens] += static_cast<double>(h[0]);
@@ -358,6 +389,26 @@ void Session::sync() {
 // the target hierarchy with its vertex normals, and the template graph.
 void Session::prepare() {
     hipStream_t st = stream_.get();
+    if (!shared_graph_) {
+        // Farthest-point seeds run ahead on their own stream, one launch per
+        // level's prefix, each marked by an event; a level waits only for the
+        // seeds it uses, so the finer levels' sampling overlaps the coarser
+        // levels' solves.
+        const int lv = static_cast<int>(own_graph_.levels.size());
+        clock_.begin("fps", fps_stream_.get());
+        int32_t done = 0;
+        for (int l = 0; l < lv; ++l) {
+            const int32_t want = static_cast<int32_t>(level_count_for(lv, settings_.nodes_initial, settings_.growth,
+                                                                      nv_, l));
+            if (want > done) {
+                farthest_point_seeds_range(points_.get(), nv_, 0, done, want, seeds_.get(), fps_work_,
+                                           fps_stream_.get());
+                done = want;
+            }
+            NRICP_CHECK(hipEventRecord(fps_ready_[l], fps_stream_.get()));
+        }
+        clock_.end(fps_stream_.get());
+    }
     clock_.begin("target_bvh", st);
     build_vertex_faces(faces_.get(), nf_, nv_, template_vf_, sort_, st);
     build_lbvh(bvh_, target_vertex_count_, sort_, target_vf_, st);
@@ -365,12 +416,6 @@ void Session::prepare() {
     // is smooth, so neighbouring queries stay neighbours.
     morton_order(points_.get(), nv_, query_order_.get(), sort_, order_keys_, order_bounds_, st);
     clock_.end(st);
-    if (!shared_graph_) {
-        clock_.begin("graph", st);
-        build_device_graph(points_.get(), nv_, settings_.graph_levels, settings_.nodes_initial, settings_.growth,
-                           settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_, sort_, st);
-        clock_.end(st);
-    }
     level_ = 0;
 }
 
@@ -396,6 +441,16 @@ void Session::reset() {
 
 void Session::begin_level() {
     hipStream_t st = stream_.get();
+    if (!shared_graph_) {
+        NRICP_CHECK(hipStreamWaitEvent(st, fps_ready_[level_], 0));
+        clock_.begin("graph", st);
+        const int lv = static_cast<int>(own_graph_.levels.size());
+        build_graph_level(points_.get(), nv_, seeds_.get(),
+                          level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, static_cast<int>(level_)),
+                          settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_.levels[level_], sort_,
+                          st);
+        clock_.end(st);
+    }
     const GraphLevel& L = graph_->levels[level_];
     if (level_ == 0) {
         identity_state<<<blocks_for(L.node_count, 256), 256, 0, st>>>(rot_.get(), trans_.get(), L.node_count);
@@ -724,7 +779,7 @@ int Session::level_count() const {
 void Session::level_nodes(int64_t* out) const {
     const int lv = level_count();
     for (int l = 0; l < lv; ++l)
-        out[l] = static_cast<size_t>(l) < graph_->levels.size()
+        out[l] = (static_cast<size_t>(l) < graph_->levels.size() && graph_->levels[l].node_count > 0)
                      ? graph_->levels[l].node_count
                      : level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, l);
 }

@@ -23,6 +23,8 @@ struct GraphLevel {
     DeviceBuffer<double> bind_weights;
     DeviceBuffer<int32_t> bind_counts;
     DeviceBuffer<int32_t> edges;
+    DeviceBuffer<uint64_t> edge_keys;  // scratch of the edge build
+    DeviceBuffer<int32_t> edge_flags, edge_pos;
 };
 
 struct DeviceGraph {
@@ -35,6 +37,11 @@ struct DeviceGraph {
 void build_device_graph(const double* points, int64_t vertex_count, int levels, int nodes_initial, int growth,
                         int max_nodes_per_vertex, int neighbours, DeviceGraph& graph, SortScratch& scratch,
                         hipStream_t stream);
+// One level from the first `node_count` seeds: node positions, bindings and
+// kNN edges (synchronises the stream for the edge count).
+void build_graph_level(const double* points, int64_t vertex_count, const int32_t* seeds, int64_t node_count,
+                       int max_nodes_per_vertex, int neighbours, GraphLevel& level, SortScratch& scratch,
+                       hipStream_t stream);
 
 // Report of a run (RunReport, src/core/report.h:12-59).
 struct LevelView {
@@ -63,7 +70,7 @@ public:
     // A pending entry whose two events the caller records itself; returns
     // false (and no events) when the clock is off.
     bool external(const char* name, hipEvent_t* start, hipEvent_t* stop);
-    void collect();  // after a stream synchronisation
+    void collect(bool wait);  // wait: block on entries not yet reached
     std::map<std::string, std::pair<double, int64_t>> totals;  // name -> (ms, launches)
     ~KernelClock();
 
@@ -100,6 +107,9 @@ enum Counter : int {
 class Session {
 public:
     Session(const nricp_hip_problem& problem, const nricp_hip_settings& settings, const DeviceGraph* graph);
+    ~Session();
+    Session(const Session&) = delete;
+    Session& operator=(const Session&) = delete;
     nricp_hip_progress step();
     void run();
     // Rewind to the start of the wrap, keeping the uploaded inputs; the next
@@ -114,8 +124,7 @@ public:
     void state(double* rotations, double* translations, double* rest);
     std::string report_text();
     const std::vector<double>& trace() const { return trace_; }
-    KernelClock& clock() { return clock_; }
-    const KernelClock& clock() const { return clock_; }
+    KernelClock& clock();  // synchronises the session's streams and collects
 
 private:
     enum class Phase { Prepare, BeginLevel, Refresh, InnerStep, CloseOuter, EndLevel, Done };
@@ -134,6 +143,10 @@ private:
     nricp_hip_settings settings_;
     std::vector<nricp_hip_level_settings> level_settings_;
     Stream stream_;
+    Stream fps_stream_;                    // farthest-point sampling, ahead of the levels
+    FpsWork fps_work_;
+    DeviceBuffer<int32_t> seeds_;          // finest level's seeds; coarser levels use prefixes
+    std::vector<hipEvent_t> fps_ready_;    // per level: its seeds are written
     SortScratch sort_;
     int64_t nv_ = 0, nf_ = 0;
     DeviceBuffer<double> points_;
