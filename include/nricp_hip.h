@@ -292,9 +292,11 @@ int nricp_hip_session_report(nricp_hip_session* session, char* buffer, size_t ca
 /* Energy at the start of every inner iteration so far (decision trace). */
 int nricp_hip_session_trace(const nricp_hip_session* session, double* out, int64_t capacity,
                             int64_t* out_count);
-/* Device-time accounting with HIP events on the session's stream (off by
- * default): enable, then read the total milliseconds and launch count of a
- * stage -- "closest_points", "assemble", "pcg", "energy", "layout". */
+/* Device-time accounting with HIP events (off by default): enable, then read
+ * the total milliseconds and launch count of a stage -- "fps" (on the sampling
+ * stream, overlapping the rest), "graph", "target_bvh", "layout", "deform",
+ * "closest_points", "assemble", "pcg", "energy", "local_step". Reading
+ * synchronises the session's streams. */
 int nricp_hip_session_profile(nricp_hip_session* session, int32_t enable);
 int nricp_hip_session_kernel_time(const nricp_hip_session* session, const char* stage, double* total_ms,
                                   int64_t* launches);

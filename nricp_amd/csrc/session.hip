@@ -593,15 +593,17 @@ bool Session::inner_step() {
             trace_.push_back(cur);
             if (outer_ == 0 && inner_ == 0) view_.energy_start = cur;
         }
-        // Algorithmic traffic of the solve (matrix with each symmetric pair
-        // counted once, the block preconditioner, and u written and read
-        // once per iteration) and of the candidate energy pass.
+        // Algorithmic traffic of the solve: per iteration the symmetric
+        // matrix once (N diagonal and P pair blocks of 36 values) and the
+        // exchanged 6-vector written and read once; per solve the
+        // preconditioner written and read once, the rhs read and the solution
+        // written (FP64). Candidate energy pass: its inputs once.
         {
             const double s = settings_.precision == NRICP_HIP_PRECISION_MIXED ? 4.0 : 8.0;
             const double P = static_cast<double>(layout_.group_count), N = static_cast<double>(nn);
             counters_[kPcgLaunches] += 1.0;
             counters_[kPcgIterations] += h[10];
-            counters_[kPcgBytes] += h[10] * s * (36.0 * (2.0 * N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
+            counters_[kPcgBytes] += h[10] * s * (36.0 * (N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
             counters_[kEnergyLaunches] += 1.0;
             counters_[kEnergyBytes] += 134.0 * static_cast<double>(nv_) + 120.0 * N +
                                        8.0 * static_cast<double>(graph_->levels[level_].edge_count) + 40.0;
