@@ -523,7 +523,7 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
              QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
              double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
              int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
-             const int32_t* __restrict__ order, unsigned long long* __restrict__ stats) {
+             const int32_t* __restrict__ order, uint32_t* __restrict__ stats) {
     __shared__ int32_t stack_node[kGroupsPerBlock][kStackDepth];
     __shared__ float stack_dist[kGroupsPerBlock][kStackDepth];
     const int j = threadIdx.x % kLanes;
@@ -535,6 +535,7 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
     const bool writer = j == 0;
     if (mask && mask[i] == 0) {
         if (writer) {
+            if (stats) stats[2 * slot] = stats[2 * slot + 1] = 0;
             st3(out_p, i, d3{0.0, 0.0, 0.0});
             st3(out_n, i, d3{0.0, 0.0, 0.0});
             out_d[i] = 0.0;
@@ -577,11 +578,10 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
         }
     }
     if (stats) {
-        const int opens = best.opens, tests = group_sum(static_cast<int>(best.tests));
+        const uint32_t tests = static_cast<uint32_t>(group_sum(static_cast<int>(best.tests)));
         if (writer) {
-            atomicAdd(stats, static_cast<unsigned long long>(opens));
-            atomicAdd(stats + 1, static_cast<unsigned long long>(tests));
-            atomicMax(stats + 2, static_cast<unsigned long long>(opens));
+            stats[2 * slot] = best.opens;
+            stats[2 * slot + 1] = tests;
         }
     }
     if (!writer) return;
@@ -622,6 +622,38 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
         out_u[i] = (rej == 0 && s.robust_threshold > 0.0)
                        ? robust_weight(s.robust_kernel, norm2(off), s.robust_threshold)
                        : 1.0;
+    }
+}
+
+// Traversal statistics of one launch (profiling): per-query (opens, tests)
+// folded into running totals [sum opens, sum tests, max opens].
+__global__ void __launch_bounds__(1024) query_stats_fold(const uint32_t* __restrict__ rows, int64_t count,
+                                                        unsigned long long* __restrict__ totals) {
+    __shared__ unsigned long long lds[3][1024];
+    unsigned long long o = 0, t = 0, m = 0;
+    for (int64_t i = threadIdx.x; i < count; i += 1024) {
+        const unsigned long long a = rows[2 * i], b = rows[2 * i + 1];
+        o += a;
+        t += b;
+        m = a > m ? a : m;
+    }
+    lds[0][threadIdx.x] = o;
+    lds[1][threadIdx.x] = t;
+    lds[2][threadIdx.x] = m;
+    __syncthreads();
+    for (int span = 512; span > 0; span >>= 1) {
+        if (threadIdx.x < span) {
+            lds[0][threadIdx.x] += lds[0][threadIdx.x + span];
+            lds[1][threadIdx.x] += lds[1][threadIdx.x + span];
+            const unsigned long long other = lds[2][threadIdx.x + span];
+            if (other > lds[2][threadIdx.x]) lds[2][threadIdx.x] = other;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[0] += lds[0][0];
+        totals[1] += lds[1][0];
+        if (lds[2][0] > totals[2]) totals[2] = lds[2][0];
     }
 }
 
@@ -794,15 +826,22 @@ void morton_order(const double* points, int64_t n, int32_t* out, SortScratch& sc
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
-                    hipStream_t stream, const int32_t* warm, const int32_t* order, uint64_t* stats) {
+                    hipStream_t stream, const int32_t* warm, const int32_t* order, QueryStats* stats) {
     if (count <= 0) return;
     query_kernel<<<blocks_for(count * kLanes, kQueryThreads), kQueryThreads, 0, stream>>>(
         bvh.wide.get(), bvh.wide_count, bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(),
         bvh.faces.get(),
         bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm, out_points,
         out_normals, out_distance, out_triangle, out_rejection, out_robust, order,
-        reinterpret_cast<unsigned long long*>(stats));
+        stats ? stats->rows.get() : nullptr);
     NRICP_LAUNCH_CHECK("query_kernel");
+}
+
+void fold_query_stats(QueryStats& stats, int64_t count, hipStream_t stream) {
+    if (count <= 0) return;
+    query_stats_fold<<<1, 1024, 0, stream>>>(stats.rows.get(), count,
+                                             reinterpret_cast<unsigned long long*>(stats.totals.get()));
+    NRICP_LAUNCH_CHECK("query_stats_fold");
 }
 
 void summarize_correspondences(const uint8_t* rejection, const double* distance, int64_t count,

@@ -94,13 +94,19 @@ struct QuerySettings {
 // `order`, when given, is a permutation of the queries in which they are
 // processed (spatially coherent orders keep a wave's traversals together);
 // results are written at each query's own index either way. `stats`, when
-// given, accumulates [nodes opened, triangles tested] and the maximum steps
-// of one query into its three entries.
+// given, receives each query's (wide nodes opened, triangles tested) in
+// stats->rows (profiling; 2 x count entries); fold_query_stats then adds
+// them to stats->totals [opens, tests, most opens of one query].
+struct QueryStats {
+    DeviceBuffer<uint32_t> rows;    // per query: opens, tests
+    DeviceBuffer<uint64_t> totals;  // 3
+};
 void closest_points(const Lbvh& bvh, const double* queries, const double* query_normals, const uint8_t* mask,
                     int64_t count, const QuerySettings& s, double* out_points, double* out_normals,
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
                     hipStream_t stream, const int32_t* warm = nullptr, const int32_t* order = nullptr,
-                    uint64_t* stats = nullptr);
+                    QueryStats* stats = nullptr);
+void fold_query_stats(QueryStats& stats, int64_t count, hipStream_t stream);
 // Permutation of `count` points sorted by the Morton code (10 bits per axis)
 // of their position in the points' bounding box, ties by index.
 void morton_order(const double* points, int64_t count, int32_t* out_order, SortScratch& scratch,

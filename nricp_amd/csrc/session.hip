@@ -377,7 +377,7 @@ void Session::sync() {
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     clock_.collect(false);
     if (stats_pending_) {
-        const std::vector<uint64_t> h = query_stats_.to_host(stream_.get());
+        const std::vector<uint64_t> h = query_stats_.totals.to_host(stream_.get());
         counters_[kQueryNodeOpens] += static_cast<double>(h[0]);
         counters_[kQueryLeafTests] += static_cast<double>(h[1]);
         counters_[kQueryMaxSteps] = std::max(counters_[kQueryMaxSteps], static_cast<double>(h[2]));
@@ -492,20 +492,22 @@ void Session::refresh() {
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, ls.robust_kernel,
                     ls.robust_threshold};
     clock_.begin("closest_points", st);
-    uint64_t* stats = nullptr;
+    QueryStats* stats = nullptr;
     if (clock_.enabled()) {
         if (!stats_pending_) {
-            query_stats_.resize(3);
-            query_stats_.zero(st);
+            query_stats_.totals.resize(3);
+            query_stats_.totals.zero(st);
             stats_pending_ = true;
         }
-        stats = query_stats_.get();
+        query_stats_.rows.resize(2 * nv_);
+        stats = &query_stats_;
     }
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
                    robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get(), stats);
     have_warm_ = true;
     clock_.end(st);
+    if (stats) fold_query_stats(*stats, nv_, st);
     counters_[kQueryLaunches] += 1.0;
     counters_[kQueryRows] += static_cast<double>(nv_);
     inner_ = 0;

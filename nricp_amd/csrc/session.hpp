@@ -171,7 +171,7 @@ private:
     DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
     DeviceBuffer<int32_t> corr_tri_, query_order_;
     DeviceBuffer<uint64_t> order_keys_;
-    DeviceBuffer<uint64_t> query_stats_;  // opens, tests, max steps (profiling only)
+    QueryStats query_stats_;  // traversal statistics (profiling only)
     bool stats_pending_ = false;
     DeviceBuffer<double> order_bounds_;
     DeviceBuffer<uint8_t> rejection_;
