@@ -195,48 +195,66 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int64_t n, BvhN
     if (i == 0) nodes[0].parent = -1;
 }
 
-// Bottom-up refit: a leaf writes its triangle box into its parent's slot;
-// the second child to arrive at a node merges both slots and carries the
-// union one level up. Release/acquire at agent scope orders the slot writes
-// against the arrival counter across compute dies.
-__global__ void refit_kernel(const double* __restrict__ corners, int64_t n, const int32_t* __restrict__ leaf_parent,
-                             BvhNode* nodes, int32_t* arrivals) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-        const double a = corners[9 * i + k], b = corners[9 * i + 3 + k], c = corners[9 * i + 6 + k];
-        lo[k] = fmin(fmin(a, b), c);
-        hi[k] = fmax(fmax(a, b), c);
-    }
-    // Outward rounding to FP32: the box still contains the triangle exactly;
-    // unions of FP32 boxes (min / max) are exact from here on.
-    float flo[3], fhi[3];
-    for (int k = 0; k < 3; ++k) {
-        flo[k] = __double2float_rd(lo[k]);
-        fhi[k] = __double2float_ru(hi[k]);
-    }
-    int32_t child = ~static_cast<int32_t>(i);
-    int32_t node = leaf_parent[i];
-    while (node >= 0) {
-        BvhNode* nd = nodes + node;
-        const int side = nd->child[0] == child ? 0 : 1;
+// Bottom-up refit, one launch per binary depth from the deepest up: a node
+// at depth d stores its two children's boxes, and its internal children (at
+// depth d + 1) were completed by the previous launch, so no node waits on
+// another inside a launch. A leaf's box is its triangle's FP64 box rounded
+// outward to FP32 (it still contains the triangle); unions of FP32 boxes
+// (min / max) are exact from there on.
+__device__ inline void child_box(const BvhNode* __restrict__ nodes, const double* __restrict__ corners, int32_t c,
+                                 float* lo, float* hi) {
+    if (c < 0) {
+        const double* t = corners + 9 * static_cast<int64_t>(~c);
         for (int k = 0; k < 3; ++k) {
-            nd->box[6 * side + k] = flo[k];
-            nd->box[6 * side + 3 + k] = fhi[k];
+            lo[k] = __double2float_rd(fmin(fmin(t[k], t[3 + k]), t[6 + k]));
+            hi[k] = __double2float_ru(fmax(fmax(t[k], t[3 + k]), t[6 + k]));
         }
-        const int32_t prior =
-            __hip_atomic_fetch_add(arrivals + node, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prior == 0) return;
-        const int other = 1 - side;
+    } else {
+        const float* b = nodes[c].box;
         for (int k = 0; k < 3; ++k) {
-            flo[k] = fminf(flo[k], __hip_atomic_load(&nd->box[6 * other + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            fhi[k] = fmaxf(fhi[k], __hip_atomic_load(&nd->box[6 * other + 3 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            lo[k] = fminf(b[k], b[6 + k]);
+            hi[k] = fmaxf(b[3 + k], b[9 + k]);
         }
-        child = node;
-        node = nd->parent;
     }
 }
+
+__global__ void refit_level_kernel(const uint32_t* __restrict__ order, int64_t begin, int64_t end,
+                                   const double* __restrict__ corners, BvhNode* nodes) {
+    const int64_t t = begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= end) return;
+    BvhNode* nd = nodes + order[t];
+    for (int side = 0; side < 2; ++side) {
+        float lo[3], hi[3];
+        child_box(nodes, corners, nd->child[side], lo, hi);
+        for (int k = 0; k < 3; ++k) {
+            nd->box[6 * side + k] = lo[k];
+            nd->box[6 * side + 3 + k] = hi[k];
+        }
+    }
+}
+
+// (depth, node) pairs for the level lists.
+__global__ void depth_keys_kernel(const int32_t* __restrict__ depth, int64_t ni, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    keys[i] = static_cast<uint64_t>(depth[i]);
+    vals[i] = static_cast<uint32_t>(i);
+}
+
+// [start, end) of each depth's run in the depth-sorted list.
+__global__ void level_runs_kernel(const uint64_t* __restrict__ keys, int64_t ni, int32_t* __restrict__ runs) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const uint64_t d = keys[i];
+    if (i == 0 || keys[i - 1] != d) runs[2 * d] = static_cast<int32_t>(i);
+    if (i == ni - 1 || keys[i + 1] != d) runs[2 * d + 1] = static_cast<int32_t>(i + 1);
+}
+
+// A Karras tree over unique keys of B bits is at most B + 1 levels deep
+// (every level consumes a bit of common prefix); keys here have at most 61.
+constexpr int kDepthBits = 7;
+constexpr int kMaxDepth = 1 << kDepthBits;
 
 // Binary depth of every internal node (parent links up to the root); the
 // nodes at depths 0, 3, 6, ... root the wide nodes.
@@ -246,7 +264,7 @@ __global__ void depth_kernel(const BvhNode* __restrict__ nodes, int64_t ni, int3
     if (i >= ni) return;
     int32_t d = 0;
     int32_t n = static_cast<int32_t>(i);
-    while (n > 0 && d < 128) {
+    while (n > 0 && d < kMaxDepth - 1) {
         n = nodes[n].parent;
         ++d;
     }
@@ -778,22 +796,39 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
         karras_kernel<<<blocks_for(nt - 1, 256), 256, 0, stream>>>(bvh.keys.get(), nt, bvh.nodes.get(),
                                                                     bvh.leaf_parent.get());
         NRICP_LAUNCH_CHECK("karras_kernel");
-        bvh.flags.resize(nt);  // refit arrivals, then the wide-root flags
-        bvh.flags.zero(stream);
-        refit_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.leaf_corners.get(), nt, bvh.leaf_parent.get(),
-                                                               bvh.nodes.get(), bvh.flags.get());
-        NRICP_LAUNCH_CHECK("refit_kernel");
-        // Collapse into the eight-wide query tree.
+        // Depths (from the parent links) give the wide roots (depth % 3 == 0)
+        // and the refit's level lists; one read-back brings the wide count
+        // and the level runs.
         const int64_t ni = nt - 1;
+        bvh.flags.resize(ni);
         bvh.depth.resize(ni);
         bvh.wide_index.resize(ni + 1);
         depth_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(), bvh.flags.get());
         NRICP_LAUNCH_CHECK("depth_kernel");
         exclusive_scan32(bvh.flags.get(), bvh.wide_index.get(), ni, scratch, stream);
-        int32_t wide_count = 0;
-        NRICP_CHECK(hipMemcpyAsync(&wide_count, bvh.wide_index.get() + ni, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                   stream));
+        bvh.level_order.resize(ni);
+        depth_keys_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.depth.get(), ni, bvh.keys.get(),
+                                                                    bvh.level_order.get());
+        NRICP_LAUNCH_CHECK("depth_keys_kernel");
+        radix_sort_pairs(bvh.keys.get(), bvh.level_order.get(), ni, kDepthBits, scratch, stream);
+        bvh.level_runs.resize(2 * kMaxDepth);
+        bvh.level_runs.zero(stream);
+        level_runs_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.keys.get(), ni, bvh.level_runs.get());
+        NRICP_LAUNCH_CHECK("level_runs_kernel");
+        std::vector<int32_t> runs(2 * kMaxDepth + 1);
+        NRICP_CHECK(hipMemcpyAsync(runs.data(), bvh.level_runs.get(), 2 * kMaxDepth * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, stream));
+        NRICP_CHECK(hipMemcpyAsync(runs.data() + 2 * kMaxDepth, bvh.wide_index.get() + ni, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, stream));
         NRICP_CHECK(hipStreamSynchronize(stream));
+        for (int d = kMaxDepth - 1; d >= 0; --d) {
+            const int64_t begin = runs[2 * d], end = runs[2 * d + 1];
+            if (end <= begin) continue;
+            refit_level_kernel<<<blocks_for(end - begin, 256), 256, 0, stream>>>(
+                bvh.level_order.get(), begin, end, bvh.leaf_corners.get(), bvh.nodes.get());
+            NRICP_LAUNCH_CHECK("refit_level_kernel");
+        }
+        const int32_t wide_count = runs[2 * kMaxDepth];
         bvh.wide_count = wide_count;
         bvh.wide.resize(std::max<int64_t>(wide_count, 1));
         collapse_kernel<<<blocks_for(ni, 128), 128, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(),

@@ -94,110 +94,246 @@ __global__ void retract_kernel(const double* __restrict__ R, const double* __res
 }
 
 // ---------------------------------------------------------- sampling -------
+//
+// Farthest-point sampling is one long dependent chain: each seed is the point
+// farthest from all earlier seeds, so seed s + 1 can only be chosen once seed
+// s has updated every running distance. The chain runs in ONE workgroup, so a
+// step costs a workgroup barrier instead of a grid barrier, and it only
+// touches the points a new seed can change:
+//
+//   * The points are stored in the Morton order of their positions, in blocks
+//     of 64 x ppl. Each block keeps an outward-rounded box and the maximum of
+//     its points' running distances (with that point's index and position).
+//   * Lane l of wave w owns block l * 16 + w (neighbouring blocks go to
+//     different waves) and keeps its summary in registers.
+//   * Step: every wave reduces its lanes' block maxima and publishes the
+//     winner; after one __syncthreads every wave reduces the 16 winners and
+//     so knows the seed. Each lane then tests its block: if the seed's squared
+//     distance to the box is at least the block's maximum (with a relative
+//     margin far above rounding), no point of the block can get closer, so
+//     the block is skipped. The wave sweeps the blocks that remain, updates
+//     their points' distances and recomputes their maxima.
+//
+// Late in the run a seed reaches a few blocks, so a step is one sweep of a few
+// hundred points plus two reductions. Distances are the CPU route's own
+// expression (no contraction), ties go to the lower index, so the seeds are
+// the CPU route's seeds bit for bit.
 
-constexpr int kFpsThreads = 512;
-constexpr int kFpsPerThread = 16;
-
-// A sampling candidate: its running nearest squared distance, its index and
-// its position (carried along so the winner's position needs no extra load).
-struct ArgMax {
-    double value;
-    int64_t index;
-    d3 point;
-};
+constexpr int kSampleThreads = 1024;
+constexpr int kSampleWaves = kSampleThreads / 64;
+constexpr int kSampleBlocks = kSampleThreads;  // one block per lane
+constexpr double kSampleMargin = 1.0 - 1e-9;
 
 // The larger value wins; equal values go to the lower index (the first
 // maximum of an in-order sweep). A total order, so any reduction order gives
 // the same winner.
-__device__ inline ArgMax better(ArgMax a, ArgMax b) {
-    if (b.value > a.value || (b.value == a.value && b.index < a.index)) return b;
-    return a;
+__device__ inline bool beats(double v, int32_t i, double bv, int32_t bi) {
+    return v > bv || (v == bv && i < bi);
 }
 
-__device__ inline ArgMax shfl_xor(const ArgMax& v, int m) {
-    return ArgMax{__shfl_xor(v.value, m, 64), __shfl_xor(v.index, m, 64),
-                  d3{__shfl_xor(v.point.x, m, 64), __shfl_xor(v.point.y, m, 64), __shfl_xor(v.point.z, m, 64)}};
-}
-
-__device__ inline ArgMax wave_argmax(ArgMax v) {
+// Butterfly arg-max over the wave: every lane ends with the winner.
+__device__ inline void wave_argmax(double& v, int32_t& i) {
 #pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) v = better(v, shfl_xor(v, offset));
-    return v;
-}
-
-constexpr int kFpsWaves = kFpsThreads / 64;
-
-// Persistent FPS: each workgroup owns a contiguous chunk of points and keeps
-// them and their running nearest squared distances in registers. One grid
-// barrier per seed: every wave publishes its winner (value, index, position)
-// to its slot; after the barrier every wave loads all slots and reduces them
-// with shuffles, so every wave of every group picks the same seed (the order
-// (value, lower index) is total, so the reduction order cannot matter). A
-// launch computes seeds [begin, end); the running distances go back to memory
-// at its end, so the next launch resumes where this one stopped.
-__global__ void __launch_bounds__(kFpsThreads)
-fps_kernel(const double* __restrict__ p, int64_t n, int32_t begin, int32_t end, int32_t first,
-           int32_t* __restrict__ out, double* __restrict__ nearest_g, double* partial_v /* 2 x slots x 4 */,
-           int64_t* partial_i /* 2 x slots */, unsigned* barrier, int64_t chunk) {
-    // The sampling is one long dependent chain running beside the solve's
-    // kernels: its waves take issue priority on the units they share.
-    __builtin_amdgcn_s_setprio(3);
-    const unsigned groups = gridDim.x;
-    const unsigned slots = groups * kFpsWaves;
-    const unsigned my_slot = blockIdx.x * kFpsWaves + threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * chunk;
-    double nearest[kFpsPerThread];
-    d3 mine[kFpsPerThread];
-    const ArgMax none{-1.0, 0x7fffffffffffffffll, d3{0.0, 0.0, 0.0}};
-    ArgMax local = none;
-    const d3 c0 = ld3(p, begin == 0 ? first : out[begin - 1]);
-#pragma unroll
-    for (int k = 0; k < kFpsPerThread; ++k) {
-        const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
-        if (i < n && i < base + chunk) {
-            mine[k] = ld3(p, i);
-            nearest[k] = begin == 0 ? norm2(mine[k] - c0) : nearest_g[i];
-            local = better(local, ArgMax{nearest[k], i, mine[k]});
-        } else {
-            mine[k] = d3{0.0, 0.0, 0.0};
-            nearest[k] = -1.0;
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        const double ov = __shfl_xor(v, offset, 64);
+        const int32_t oi = __shfl_xor(i, offset, 64);
+        if (beats(ov, oi, v, i)) {
+            v = ov;
+            i = oi;
         }
     }
-    if (begin == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = first;
-    for (int32_t s = begin > 0 ? begin : 1; s < end; ++s) {
-        const ArgMax w = wave_argmax(local);
-        double* pv = partial_v + (s & 1) * 4 * slots;
-        int64_t* pi = partial_i + (s & 1) * slots;
-        if (lane == 0) {
-            pv[4 * my_slot] = w.value;
-            pv[4 * my_slot + 1] = w.point.x;
-            pv[4 * my_slot + 2] = w.point.y;
-            pv[4 * my_slot + 3] = w.point.z;
-            pi[my_slot] = w.index;
+}
+
+// The lane holding index `i` (indices are unique within a reduction).
+__device__ inline int owner_lane(int32_t mine, int32_t i) {
+    return __ffsll(static_cast<unsigned long long>(__ballot(mine == i))) - 1;
+}
+
+__device__ inline d3 shfl_d3(d3 p, int lane) {
+    return d3{__shfl(p.x, lane, 64), __shfl(p.y, lane, 64), __shfl(p.z, lane, 64)};
+}
+
+// Builds the blocked layout: records (x, y, z, running distance to the first
+// seed) in Morton order, the original indices, and each block's box and
+// maximum. One wave per block; padding slots hold distance -1 (never chosen,
+// never changed) and index INT32_MAX.
+__global__ void __launch_bounds__(64)
+sample_init_kernel(const double* __restrict__ p, int64_t n, const int32_t* __restrict__ order, int32_t first,
+                   int ppl, double4* __restrict__ rec, int32_t* __restrict__ ids, float* __restrict__ box,
+                   double4* __restrict__ best, int32_t* __restrict__ best_index, int32_t* __restrict__ out) {
+    const int lane = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * 64 * ppl;
+    const d3 c0 = ld3(p, first);
+    d3 lo{DBL_MAX, DBL_MAX, DBL_MAX}, hi{-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double v = -1.0;
+    int32_t vi = 0x7fffffff;
+    d3 vp{0.0, 0.0, 0.0};
+    for (int k = 0; k < ppl; ++k) {
+        const int64_t j = base + static_cast<int64_t>(k) * 64 + lane;
+        if (j < n) {
+            const int32_t i = order[j];
+            const d3 q = ld3(p, i);
+            const double d = norm2(q - c0);
+            rec[j] = make_double4(q.x, q.y, q.z, d);
+            ids[j] = i;
+            lo = d3{fmin(lo.x, q.x), fmin(lo.y, q.y), fmin(lo.z, q.z)};
+            hi = d3{fmax(hi.x, q.x), fmax(hi.y, q.y), fmax(hi.z, q.z)};
+            if (beats(d, i, v, vi)) {
+                v = d;
+                vi = i;
+                vp = q;
+            }
+        } else {
+            rec[j] = make_double4(0.0, 0.0, 0.0, -1.0);
+            ids[j] = 0x7fffffff;
         }
-        grid_barrier(barrier, groups);
-        ArgMax best = none;
-        for (unsigned t = lane; t < slots; t += 64)
-            best = better(best, ArgMax{pv[4 * t], pi[t], d3{pv[4 * t + 1], pv[4 * t + 2], pv[4 * t + 3]}});
-        best = wave_argmax(best);
-        if (blockIdx.x == 0 && threadIdx.x == 0) out[s] = static_cast<int32_t>(best.index);
-        const d3 c = best.point;
-        local = none;
+    }
+    const int32_t own = vi;
+    wave_argmax(v, vi);
+    vp = shfl_d3(vp, owner_lane(own, vi));
 #pragma unroll
-        for (int k = 0; k < kFpsPerThread; ++k) {
-            if (nearest[k] >= 0.0) {
-                const double d = norm2(mine[k] - c);
-                nearest[k] = fmin(nearest[k], d);
-                const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
-                local = better(local, ArgMax{nearest[k], i, mine[k]});
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        lo = d3{fmin(lo.x, __shfl_xor(lo.x, offset, 64)), fmin(lo.y, __shfl_xor(lo.y, offset, 64)),
+                fmin(lo.z, __shfl_xor(lo.z, offset, 64))};
+        hi = d3{fmax(hi.x, __shfl_xor(hi.x, offset, 64)), fmax(hi.y, __shfl_xor(hi.y, offset, 64)),
+                fmax(hi.z, __shfl_xor(hi.z, offset, 64))};
+    }
+    if (lane == 0) {
+        float* b = box + 6 * static_cast<int64_t>(blockIdx.x);
+        b[0] = __double2float_rd(lo.x);
+        b[1] = __double2float_rd(lo.y);
+        b[2] = __double2float_rd(lo.z);
+        b[3] = __double2float_ru(hi.x);
+        b[4] = __double2float_ru(hi.y);
+        b[5] = __double2float_ru(hi.z);
+        best[blockIdx.x] = make_double4(v, vp.x, vp.y, vp.z);
+        best_index[blockIdx.x] = vi;
+        if (blockIdx.x == 0) out[0] = first;
+    }
+}
+
+// Sweeps one block for seed c: U records per lane per round, `rounds`
+// rounds. Returns the block's new maximum (every lane holds it).
+template <int U>
+__device__ inline void sample_sweep(double4* __restrict__ rec, const int32_t* __restrict__ ids, int64_t base,
+                                    int rounds, d3 c, int lane, double& v, int32_t& vi, d3& vp) {
+    v = -1.0;
+    vi = 0x7fffffff;
+    vp = d3{0.0, 0.0, 0.0};
+    for (int r = 0; r < rounds; ++r) {
+        double4 q[U];
+        int32_t id[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + static_cast<int64_t>(r * U + u) * 64 + lane;
+            q[u] = rec[j];
+            id[u] = ids[j];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (q[u].w >= 0.0) {
+                const double d = norm2(d3{q[u].x, q[u].y, q[u].z} - c);
+                if (d < q[u].w) {
+                    q[u].w = d;
+                    reinterpret_cast<double*>(rec + base + static_cast<int64_t>(r * U + u) * 64 + lane)[3] = d;
+                }
+                if (beats(q[u].w, id[u], v, vi)) {
+                    v = q[u].w;
+                    vi = id[u];
+                    vp = d3{q[u].x, q[u].y, q[u].z};
+                }
             }
         }
     }
+    const int32_t own = vi;
+    wave_argmax(v, vi);
+    vp = shfl_d3(vp, owner_lane(own, vi));
+}
+
+// Seeds [begin, end) in one workgroup; the running state (records and block
+// summaries) stays in memory between launches, so ranges resume exactly.
+template <int U>
+__global__ void __launch_bounds__(kSampleThreads)
+sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const float* __restrict__ box,
+              double4* __restrict__ best, int32_t* __restrict__ best_index, int blocks, int rounds, int32_t begin,
+              int32_t end, int32_t* __restrict__ out) {
+    // A long dependent chain beside the solve's kernels: take issue priority
+    // on the unit it shares.
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ double slot_v[2][kSampleWaves];
+    __shared__ int32_t slot_i[2][kSampleWaves];
+    __shared__ double slot_p[2][kSampleWaves][3];
+    const int wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int block = lane * kSampleWaves + wave;
+    const bool owned = block < blocks;
+    const int64_t points_per_block = static_cast<int64_t>(64) * U * rounds;
+    d3 lo{0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0}, bp{0.0, 0.0, 0.0};
+    double bv = -2.0;  // below the padding's -1: an unowned lane never wins
+    int32_t bi = 0x7fffffff;
+    if (owned) {
+        const float* b = box + 6 * static_cast<int64_t>(block);
+        lo = d3{b[0], b[1], b[2]};
+        hi = d3{b[3], b[4], b[5]};
+        const double4 q = best[block];
+        bv = q.x;
+        bp = d3{q.y, q.z, q.w};
+        bi = best_index[block];
+    }
+    for (int32_t s = begin; s < end; ++s) {
+        const int buf = s & 1;
+        double v = bv;
+        int32_t vi = bi;
+        wave_argmax(v, vi);
+        const d3 vp = shfl_d3(bp, owner_lane(bi, vi));
+        if (lane == 0) {
+            slot_v[buf][wave] = v;
+            slot_i[buf][wave] = vi;
+            slot_p[buf][wave][0] = vp.x;
+            slot_p[buf][wave][1] = vp.y;
+            slot_p[buf][wave][2] = vp.z;
+        }
+        __syncthreads();
+        double sv = slot_v[buf][lane % kSampleWaves];
+        int32_t si = slot_i[buf][lane % kSampleWaves];
+        const int32_t mine = si;
 #pragma unroll
-    for (int k = 0; k < kFpsPerThread; ++k) {
-        const int64_t i = base + static_cast<int64_t>(k) * kFpsThreads + threadIdx.x;
-        if (nearest[k] >= 0.0) nearest_g[i] = nearest[k];
+        for (int offset = kSampleWaves / 2; offset > 0; offset >>= 1) {
+            const double ov = __shfl_xor(sv, offset, 64);
+            const int32_t oi = __shfl_xor(si, offset, 64);
+            if (beats(ov, oi, sv, si)) {
+                sv = ov;
+                si = oi;
+            }
+        }
+        const int from = owner_lane(mine, si) % kSampleWaves;
+        const d3 c{slot_p[buf][from][0], slot_p[buf][from][1], slot_p[buf][from][2]};
+        if (threadIdx.x == 0) out[s] = si;
+        // Blocks the seed may reach: squared distance from c to the box below
+        // the block's maximum.
+        const double dx = fmax(fmax(lo.x - c.x, c.x - hi.x), 0.0);
+        const double dy = fmax(fmax(lo.y - c.y, c.y - hi.y), 0.0);
+        const double dz = fmax(fmax(lo.z - c.z, c.z - hi.z), 0.0);
+        const double reach = dx * dx + dy * dy + dz * dz;
+        unsigned long long todo = __ballot(owned && reach * kSampleMargin < bv);
+        while (todo) {
+            const int l = __ffsll(todo) - 1;
+            todo &= todo - 1;
+            const int64_t base = static_cast<int64_t>(l * kSampleWaves + wave) * points_per_block;
+            double nv;
+            int32_t ni;
+            d3 np;
+            sample_sweep<U>(rec, ids, base, rounds, c, lane, nv, ni, np);
+            if (lane == l) {
+                bv = nv;
+                bi = ni;
+                bp = np;
+            }
+        }
+    }
+    if (owned) {
+        best[block] = make_double4(bv, bp.x, bp.y, bp.z);
+        best_index[block] = bi;
     }
 }
 
@@ -426,30 +562,57 @@ void farthest_point_seeds_range(const double* points, int64_t n, int32_t first, 
                                 int32_t* out, FpsWork& w, hipStream_t stream) {
     if (n <= 0 || end <= begin) return;
     if (first < 0 || first >= n) throw InvalidArgument("farthest_point_seeds: the first seed is out of range");
-    // Enough groups that each thread holds at most kFpsPerThread points,
-    // within the persistent-launch budget.
-    const int64_t per_group = static_cast<int64_t>(kFpsThreads) * kFpsPerThread;
-    const int64_t groups64 = (n + per_group - 1) / per_group;
-    const int64_t budget = persistent_group_budget(reinterpret_cast<const void*>(fps_kernel), kFpsThreads);
-    if (groups64 > budget)
-        throw InvalidArgument("farthest_point_seeds: " + std::to_string(n) + " points exceed one persistent launch");
-    const unsigned groups = static_cast<unsigned>(groups64);
-    const int64_t chunk = per_group;
+    if (n >= (1ll << 31) - 1) throw InvalidArgument("farthest_point_seeds: too many points");
+    if (end > n) throw InvalidArgument("farthest_point_seeds: more seeds than points");
+    // ppl points per lane per block, so that at most kSampleBlocks blocks
+    // cover the points; beyond 4 per lane, whole rounds of 4.
+    int ppl = static_cast<int>((n + 64 * kSampleBlocks - 1) / (64 * kSampleBlocks));
+    if (ppl > 4) ppl = (ppl + 3) / 4 * 4;
+    const int64_t per_block = static_cast<int64_t>(64) * ppl;
+    const int blocks = static_cast<int>((n + per_block - 1) / per_block);
     if (begin == 0) {
-        w.nearest.resize(n);
-        w.partial.resize(10 * static_cast<int64_t>(groups) * kFpsWaves);  // 2 buffers x slots x (value, x, y, z, index)
-        w.barrier.resize(2);
-        w.barrier.zero(stream);
+        w.order.resize(n);
+        morton_order(points, n, w.order.get(), w.sort, w.keys, w.bounds, stream);
+        w.records.resize(static_cast<int64_t>(blocks) * per_block);
+        w.ids.resize(static_cast<int64_t>(blocks) * per_block);
+        w.box.resize(6 * static_cast<int64_t>(blocks));
+        w.best.resize(blocks);
+        w.best_index.resize(blocks);
+        w.ppl = ppl;
+        sample_init_kernel<<<blocks, 64, 0, stream>>>(points, n, w.order.get(), first, ppl, w.records.get(),
+                                                      w.ids.get(), w.box.get(), w.best.get(), w.best_index.get(),
+                                                      out);
+        NRICP_LAUNCH_CHECK("sample_init_kernel");
+        begin = 1;
+        if (end <= begin) return;
+    } else if (w.ppl != ppl || static_cast<int64_t>(w.best.size()) != blocks) {
+        throw InvalidArgument("farthest_point_seeds: a range resumed with different points");
     }
-    double* pv = w.partial.get();
-    int64_t* pi = reinterpret_cast<int64_t*>(w.partial.get() + 8 * static_cast<int64_t>(groups) * kFpsWaves);
-    int32_t b = begin, e = end, f = first;
-    int64_t nn = n;
-    double* near = w.nearest.get();
-    unsigned* bar = w.barrier.get();
-    void* args[] = {const_cast<double**>(&points), &nn, &b, &e, &f, &out, &near, &pv, &pi, &bar,
-                    const_cast<int64_t*>(&chunk)};
-    launch_persistent(reinterpret_cast<const void*>(fps_kernel), static_cast<int>(groups), kFpsThreads, args, stream);
+    const int unroll = ppl < 4 ? ppl : 4;
+    const int rounds = ppl / unroll;
+    switch (unroll) {
+        case 1:
+            sample_kernel<1><<<1, kSampleThreads, 0, stream>>>(w.records.get(), w.ids.get(), w.box.get(),
+                                                               w.best.get(), w.best_index.get(), blocks, rounds,
+                                                               begin, end, out);
+            break;
+        case 2:
+            sample_kernel<2><<<1, kSampleThreads, 0, stream>>>(w.records.get(), w.ids.get(), w.box.get(),
+                                                               w.best.get(), w.best_index.get(), blocks, rounds,
+                                                               begin, end, out);
+            break;
+        case 3:
+            sample_kernel<3><<<1, kSampleThreads, 0, stream>>>(w.records.get(), w.ids.get(), w.box.get(),
+                                                               w.best.get(), w.best_index.get(), blocks, rounds,
+                                                               begin, end, out);
+            break;
+        default:
+            sample_kernel<4><<<1, kSampleThreads, 0, stream>>>(w.records.get(), w.ids.get(), w.box.get(),
+                                                               w.best.get(), w.best_index.get(), blocks, rounds,
+                                                               begin, end, out);
+            break;
+    }
+    NRICP_LAUNCH_CHECK("sample_kernel");
 }
 
 void farthest_point_seeds(const double* points, int64_t n, int32_t seeds, int32_t first, int32_t* out, FpsWork& w,

@@ -70,6 +70,8 @@ struct Lbvh {
     DeviceBuffer<double> points;          // Vt x 3
     DeviceBuffer<uint64_t> keys;
     DeviceBuffer<int32_t> flags;
+    DeviceBuffer<uint32_t> level_order;  // internal nodes sorted by depth (refit)
+    DeviceBuffer<int32_t> level_runs;    // [start, end) of each depth in level_order
     DeviceBuffer<double> centroids;
     DeviceBuffer<double> bounds;      // partial boxes
     DeviceBuffer<double> face_cross;  // scratch for the target normals
@@ -77,7 +79,8 @@ struct Lbvh {
 // Builds the hierarchy over a target mesh already on the device (points,
 // faces copied into the Lbvh). Morton codes (10 bits per axis) of the triangle
 // centroids, radix sorted with the triangle index as tie-break, Karras (2012)
-// topology, bottom-up refit of exact FP64 boxes.
+// topology, level-by-level bottom-up refit of outward-rounded FP32 boxes,
+// collapsed to an eight-wide tree.
 void build_lbvh(Lbvh& bvh, int64_t vertex_count, SortScratch& scratch, VertexFaces& vf, hipStream_t stream);
 
 struct QuerySettings {
@@ -136,14 +139,23 @@ void retract(const double* rotations, const double* translations, const double* 
 
 // ------------------------------------------------------------- graph -------
 
-// Farthest-point seeds (first maximum wins), as a persistent grid.
+// Farthest-point seeds (first maximum wins), in one workgroup over the points
+// in Morton order, skipping the blocks a new seed cannot reach.
 // The running state lives in FpsWork, so the seeds can be computed in ranges:
 // [0, a) then [a, b) gives the same seeds as [0, b) (coarser graph levels use
 // prefixes of the finest level's seeds and can start before it is done).
+// `order` is left holding the Morton order of the points.
 struct FpsWork {
-    DeviceBuffer<double> nearest;  // running nearest squared distance per point
-    DeviceBuffer<double> partial;
-    DeviceBuffer<unsigned> barrier;
+    DeviceBuffer<int32_t> order;       // Morton order of the points
+    DeviceBuffer<double4> records;     // (x, y, z, running nearest squared distance), Morton order
+    DeviceBuffer<int32_t> ids;         // original index per record
+    DeviceBuffer<float> box;           // per block: outward-rounded box
+    DeviceBuffer<double4> best;        // per block: (maximum, x, y, z)
+    DeviceBuffer<int32_t> best_index;  // per block: index of the maximum
+    SortScratch sort;
+    DeviceBuffer<uint64_t> keys;
+    DeviceBuffer<double> bounds;
+    int ppl = 0;
 };
 void farthest_point_seeds(const double* points, int64_t count, int32_t seeds, int32_t first, int32_t* out,
                           FpsWork& work, hipStream_t stream);

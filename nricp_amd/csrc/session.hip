@@ -413,8 +413,11 @@ void Session::prepare() {
     build_vertex_faces(faces_.get(), nf_, nv_, template_vf_, sort_, st);
     build_lbvh(bvh_, target_vertex_count_, sort_, target_vf_, st);
     // Queries run in the Morton order of the rest template: the deformation
-    // is smooth, so neighbouring queries stay neighbours.
-    morton_order(points_.get(), nv_, query_order_.get(), sort_, order_keys_, order_bounds_, st);
+    // is smooth, so neighbouring queries stay neighbours. The sampling
+    // computes that order first on its own stream (a level's queries come
+    // after its wait on the sampling); with a given graph it is built here.
+    if (shared_graph_)
+        morton_order(points_.get(), nv_, query_order_.get(), sort_, order_keys_, order_bounds_, st);
     clock_.end(st);
     level_ = 0;
 }
@@ -504,7 +507,7 @@ void Session::refresh() {
     }
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
-                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get(), stats);
+                   robust_.get(), st, have_warm_ ? corr_tri_.get() : nullptr, query_order(), stats);
     have_warm_ = true;
     clock_.end(st);
     if (stats) fold_query_stats(*stats, nv_, st);
@@ -694,7 +697,7 @@ void Session::final_statistics() {
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, 0, 0.0};
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
-                   nullptr, st, have_warm_ ? corr_tri_.get() : nullptr, query_order_.get());
+                   nullptr, st, have_warm_ ? corr_tri_.get() : nullptr, query_order());
     summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
     NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     sync();

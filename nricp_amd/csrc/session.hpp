@@ -170,6 +170,9 @@ private:
     DeviceBuffer<double> rot_, trans_, rot_cand_, trans_cand_, rot_prev_, trans_prev_, delta_;
     DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
     DeviceBuffer<int32_t> corr_tri_, query_order_;
+    // Morton order of the rest template: the sampling's, or query_order_
+    // when the graph is given.
+    const int32_t* query_order() const { return shared_graph_ ? query_order_.get() : fps_work_.order.get(); }
     DeviceBuffer<uint64_t> order_keys_;
     QueryStats query_stats_;  // traversal statistics (profiling only)
     bool stats_pending_ = false;

@@ -68,6 +68,26 @@ def test_farthest_point_seeds_are_identical(device, oracle):
                               oracle.farthest_point_seeds(points, count, first))
 
 
+@pytest.mark.parametrize("n", [70_000, 150_000, 300_000])
+def test_farthest_point_seeds_are_identical_at_every_block_size(device, oracle, n):
+    # The sampler stores the points in blocks of 64 x ceil(n / 65536) (in
+    # rounds of 4 above 4 per lane): these sizes take 2, 3 and 8 per lane.
+    rng = np.random.default_rng(n)
+    points = rng.normal(size=(n, 3)) * np.array([1.0, 0.3, 2.0])
+    assert np.array_equal(device.farthest_point_seeds(points, 300, 5), oracle.farthest_point_seeds(points, 300, 5))
+
+
+def test_farthest_point_seeds_break_exact_ties_like_the_oracle(device, oracle):
+    # A lattice has many equal squared distances; the repeated points make
+    # the running distances all zero before the last seed, after which every
+    # pick is the lowest index.
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), np.arange(12), indexing="ij"), -1).reshape(-1, 3)
+    points = np.concatenate([g, g[::7]]).astype(np.float64)
+    for count in (50, len(points)):
+        assert np.array_equal(device.farthest_point_seeds(points, count, 3),
+                              oracle.farthest_point_seeds(points, count, 3))
+
+
 def test_bindings_and_edges_are_identical(device, oracle):
     rng = np.random.default_rng(5)
     points = rng.normal(size=(3000, 3))
