@@ -200,6 +200,32 @@ def cpu_baseline(config: int, job, settings, kwargs) -> dict:
     }
 
 
+def copy_calibration(mib: int) -> None:
+    """Three device-to-device copies of `mib` MiB through the HIP runtime the
+    library uses (its blit copy kernel): a plain copy of known size, against
+    which rocprofv3's FETCH_SIZE / WRITE_SIZE counts are scaled."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    size = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), size) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, size) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(dst, src, size, 3) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    log(f"copied {mib} MiB device to device 3 times (counter calibration)")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +236,9 @@ def main(argv=None) -> int:
     ap.add_argument("--forms", default="mixed,double")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device to device three times: a plain copy of known size, against "
+                         "which rocprofv3 FETCH_SIZE / WRITE_SIZE counts are scaled")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -226,6 +255,8 @@ def main(argv=None) -> int:
         print(f"no HIP device: {info['device']}", file=sys.stderr)
         return 1
     log(f"device: {info['device']}")
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     t0 = time.perf_counter()
     job, settings, kwargs = build_job(args.config, args.seed)
     log(f"job {job.name}: template {len(job.template_points)} v / {len(job.template_faces)} f, target "

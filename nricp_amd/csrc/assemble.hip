@@ -398,14 +398,8 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
         }
     }
 #pragma unroll
-    for (int k = 0; k < 36; ++k) {
-        acc[k] += __shfl_xor(acc[k], 1, kWave);
-        acc[k] += __shfl_xor(acc[k], 2, kWave);
-        acc[k] += __shfl_xor(acc[k], 4, kWave);
-    }
-    hit |= __shfl_xor(hit, 1, kWave);
-    hit |= __shfl_xor(hit, 2, kWave);
-    hit |= __shfl_xor(hit, 4, kWave);
+    for (int k = 0; k < 36; ++k) acc[k] = octet_sum(acc[k]);
+    hit = octet_sum(hit);
     if (!valid || sub != 0) return;
     Scalar* ab = bsr + 36 * static_cast<int64_t>(pos[2 * g]);
     Scalar* ba = bsr + 36 * static_cast<int64_t>(pos[2 * g + 1]);
@@ -416,7 +410,7 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             ab[6 * x + y] = static_cast<Scalar>(acc[6 * x + y]);
             ba[6 * y + x] = static_cast<Scalar>(acc[6 * x + y]);
         }
-    if (touched) touched[g] = hit;
+    if (touched) touched[g] = hit != 0 ? 1 : 0;
 }
 
 void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream) {

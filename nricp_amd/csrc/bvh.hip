@@ -90,12 +90,25 @@ __global__ void __launch_bounds__(256) bounds_partial(const double* __restrict__
         }
 }
 
+// One wave folds the partial boxes (min / max are exact in any order) into
+// part[0..6).
 __global__ void bounds_final(double* part, int blocks) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int b = 1; b < blocks; ++b)
+    const int lane = threadIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = lane; b < blocks; b += 64)
         for (int k = 0; k < 3; ++k) {
-            part[k] = fmin(part[k], part[6 * b + k]);
-            part[3 + k] = fmax(part[3 + k], part[6 * b + 3 + k]);
+            lo[k] = fmin(lo[k], part[6 * b + k]);
+            hi[k] = fmax(hi[k], part[6 * b + 3 + k]);
+        }
+    for (int offset = 32; offset > 0; offset >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], offset, 64));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], offset, 64));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            part[k] = lo[k];
+            part[3 + k] = hi[k];
         }
 }
 
@@ -108,20 +121,26 @@ __device__ inline uint32_t spread10(uint32_t v) {
     return v;
 }
 
-__global__ void morton_kernel(const double* __restrict__ c, int64_t nt, const double* __restrict__ box,
-                              int index_bits, uint64_t* __restrict__ keys) {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
+// 30-bit Morton code of p on a 1024^3 grid over `box` (lo xyz, hi xyz),
+// clamped to the grid.
+__device__ inline uint64_t morton_code(d3 p, const double* __restrict__ box) {
+    const double c[3] = {p.x, p.y, p.z};
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {
         const double extent = box[3 + k] - box[k];
-        const double u = extent > 0.0 ? (c[3 * t + k] - box[k]) / extent : 0.0;
+        const double u = extent > 0.0 ? (c[k] - box[k]) / extent : 0.0;
         double s = u * 1024.0;
         s = s < 0.0 ? 0.0 : (s > 1023.0 ? 1023.0 : s);
         q[k] = static_cast<uint32_t>(s);
     }
-    const uint64_t code = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-    keys[t] = (code << index_bits) | static_cast<uint64_t>(t);
+    return (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+}
+
+__global__ void morton_kernel(const double* __restrict__ c, int64_t nt, const double* __restrict__ box,
+                              int index_bits, uint64_t* __restrict__ keys) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    keys[t] = (morton_code(ld3(c, t), box) << index_bits) | static_cast<uint64_t>(t);
 }
 
 __global__ void leaf_kernel(const uint64_t* __restrict__ keys, int64_t nt, uint64_t index_mask,
@@ -779,6 +798,7 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     bounds_final<<<1, 64, 0, stream>>>(bvh.bounds.get(), parts);
     NRICP_LAUNCH_CHECK("bounds_final");
     const int index_bits = std::max(1, bits_for(static_cast<uint64_t>(nt - 1)));
+    bvh.index_bits = index_bits;
     bvh.keys.resize(nt);
     morton_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.centroids.get(), nt, bvh.bounds.get(), index_bits,
                                                             bvh.keys.get());
@@ -807,13 +827,14 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
         NRICP_LAUNCH_CHECK("depth_kernel");
         exclusive_scan32(bvh.flags.get(), bvh.wide_index.get(), ni, scratch, stream);
         bvh.level_order.resize(ni);
-        depth_keys_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.depth.get(), ni, bvh.keys.get(),
+        bvh.level_keys.resize(ni);
+        depth_keys_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.depth.get(), ni, bvh.level_keys.get(),
                                                                     bvh.level_order.get());
         NRICP_LAUNCH_CHECK("depth_keys_kernel");
-        radix_sort_pairs(bvh.keys.get(), bvh.level_order.get(), ni, kDepthBits, scratch, stream);
+        radix_sort_pairs(bvh.level_keys.get(), bvh.level_order.get(), ni, kDepthBits, scratch, stream);
         bvh.level_runs.resize(2 * kMaxDepth);
         bvh.level_runs.zero(stream);
-        level_runs_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.keys.get(), ni, bvh.level_runs.get());
+        level_runs_kernel<<<blocks_for(ni, 256), 256, 0, stream>>>(bvh.level_keys.get(), ni, bvh.level_runs.get());
         NRICP_LAUNCH_CHECK("level_runs_kernel");
         std::vector<int32_t> runs(2 * kMaxDepth + 1);
         NRICP_CHECK(hipMemcpyAsync(runs.data(), bvh.level_runs.get(), 2 * kMaxDepth * sizeof(int32_t),

@@ -71,13 +71,57 @@ __device__ inline d3 mul3(const double* r, d3 v) {
 
 // ----------------------------------------------------------- reductions ----
 
-__device__ inline double wave_sum(double v) {
-    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
+// Wave-wide sums with DPP lane moves (register-to-register, no LDS round
+// trip): pairs within quads, then the two quads of each half-row
+// (half-mirror), then the two halves of each row (mirror) -- after which all
+// sixteen lanes of a row hold the row's sum, since a + b == b + a bitwise --
+// and the four row sums, read as scalars, in row order. Every lane gets the
+// same bits, and the order is fixed, so sums are reproducible.
+template <int Ctrl>
+__device__ inline int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, Ctrl, 0xF, 0xF, false);
+}
+template <int Ctrl>
+__device__ inline double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<Ctrl>(static_cast<int>(b));
+    const int hi = dpp_move<Ctrl>(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int Ctrl>
+__device__ inline float dpp_move(float v) {
+    return __int_as_float(dpp_move<Ctrl>(__float_as_int(v)));
+}
+constexpr int kDppQuadSwap1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadSwap2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowHalfMirror = 0x141;
+
+__device__ inline double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ inline float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Sum over each group of eight lanes (lanes 8g .. 8g + 7); every lane of the
+// group gets it.
+template <typename T>
+__device__ inline T octet_sum(T v) {
+    v += dpp_move<kDppQuadSwap1>(v);
+    v += dpp_move<kDppQuadSwap2>(v);
+    v += dpp_move<kDppRowHalfMirror>(v);
     return v;
 }
-__device__ inline float wave_sum(float v) {
-    for (int offset = kWave / 2; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
-    return v;
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+    v = octet_sum(v);
+    v += dpp_move<kDppRowMirror>(v);
+    return ((read_lane(v, 0) + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
 }
 
 // Block-wide sum of `count` values per thread into out[0..count), fixed order
@@ -128,15 +172,19 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
 // counter reaches (k+1)P -- one atomic and a poll, no reset and no separate
 // generation word. Agent-scope acq_rel on the ticket and acquire on the poll
 // order every global write before the barrier against every read after it.
+// grid_arrive_wait is the part thread 0 runs between the two workgroup
+// barriers.
+__device__ inline void grid_arrive_wait(unsigned* state, unsigned participants) {
+    const unsigned ticket = __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (ticket / participants + 1u) * participants;
+    while (static_cast<int>(__hip_atomic_load(state, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 __device__ inline void grid_barrier(unsigned* state, unsigned participants) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned ticket = __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned target = (ticket / participants + 1u) * participants;
-        while (static_cast<int>(__hip_atomic_load(state, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
+    if (threadIdx.x == 0) grid_arrive_wait(state, participants);
     __syncthreads();
 }
 

@@ -68,8 +68,10 @@ struct Lbvh {
     DeviceBuffer<int32_t> faces;        // T x 3 (original order)
     DeviceBuffer<double> vertex_normals;  // Vt x 3
     DeviceBuffer<double> points;          // Vt x 3
-    DeviceBuffer<uint64_t> keys;
+    DeviceBuffer<uint64_t> keys;        // sorted Morton keys of the leaves (triangle index in the low bits)
+    int index_bits = 0;
     DeviceBuffer<int32_t> flags;
+    DeviceBuffer<uint64_t> level_keys;   // depth keys (refit)
     DeviceBuffer<uint32_t> level_order;  // internal nodes sorted by depth (refit)
     DeviceBuffer<int32_t> level_runs;    // [start, end) of each depth in level_order
     DeviceBuffer<double> centroids;

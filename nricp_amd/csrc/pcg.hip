@@ -128,28 +128,41 @@ __device__ inline void sync_all(unsigned* barrier) {
         __syncthreads();
 }
 
-// Reproducible sums over the whole launch. publish(): each wave's sum of the
-// per-thread partials goes to the wave's slot. After a grid barrier, fold():
-// every wave loads all slots (lane-strided) and finishes with a butterfly, so
-// every wave of every group obtains the same bits without any LDS round.
+// Reproducible sums over the whole launch. Each wave sums its threads'
+// partials, thread 0 adds the group's wave sums in wave order (through LDS)
+// and publishes them in the group's slot just before its barrier arrival, so
+// the partial sums ride on the barrier. After it, fold(): every wave loads the
+// group slots lane-strided and sums them, so every wave of every group
+// obtains the same bits.
 constexpr int kPcgWaves = kPcgThreads / kWave;
 
 template <int Count>
-__device__ inline void publish(const double (&v)[Count], double* slots) {
-    const unsigned slot = blockIdx.x * kPcgWaves + threadIdx.x / kWave;
+__device__ inline void publish_and_wait(const double (&v)[Count], double* lds /* Count x waves */, double* slots,
+                                        unsigned* barrier) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
     for (int k = 0; k < Count; ++k) {
         const double w = wave_sum(v[k]);
-        if (threadIdx.x % kWave == 0) slots[Count * slot + k] = w;
+        if (lane == 0) lds[k * kPcgWaves + wave] = w;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < Count; ++k) {
+            double s = lds[k * kPcgWaves];
+            for (int w = 1; w < kPcgWaves; ++w) s += lds[k * kPcgWaves + w];
+            slots[Count * blockIdx.x + k] = s;
+        }
+        grid_arrive_wait(barrier, gridDim.x);
+    }
+    __syncthreads();
 }
 template <int Count>
 __device__ inline void fold(const double* slots, double (&out)[Count]) {
-    const unsigned nslots = gridDim.x * kPcgWaves;
     double acc[Count];
 #pragma unroll
     for (int k = 0; k < Count; ++k) acc[k] = 0.0;
-    for (unsigned t = threadIdx.x % kWave; t < nslots; t += kWave)
+    for (unsigned t = threadIdx.x % kWave; t < gridDim.x; t += kWave)
 #pragma unroll
         for (int k = 0; k < Count; ++k) acc[k] += slots[Count * t + k];
 #pragma unroll
@@ -165,8 +178,7 @@ __device__ inline void launch_sum(const double (&v)[Count], double* slots, doubl
         block_sum<Count, kPcgThreads>(v, lds, out);
         return;
     }
-    publish<Count>(v, slots);
-    grid_barrier(barrier, gridDim.x);
+    publish_and_wait<Count>(v, lds, slots, barrier);
     fold<Count>(slots, out);
 }
 
@@ -182,48 +194,116 @@ __device__ inline S precondition_row(const S (&mrow)[6], S v, int lane_base) {
     return static_cast<S>(acc);
 }
 
+// This lane's share of one node's block row, fixed for the whole solve and
+// kept in registers: its first block (the row's block `lane`, or -1 past the
+// row's end) with that block's column, and the row's end (further blocks of
+// the lane are every eighth after the first).
+struct RowShare {
+    int32_t first, column, end;
+};
+
+__device__ inline RowShare row_share(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
+                                     int64_t node, bool valid, int lane) {
+    RowShare r{-1, 0, 0};
+    if (valid) {
+        r.end = row_off[node + 1];
+        const int32_t b = row_off[node] + lane;
+        if (b < r.end) {
+            r.first = b;
+            r.column = cols[b];
+        }
+    }
+    return r;
+}
+
+// The matrix a workgroup reads, staged in LDS. A group's nodes are
+// consecutive, so their block rows are one contiguous range of the block
+// array; the group copies as much of it as the LDS holds once per solve (the
+// matrix is fixed for the solve) and reads the rest from memory.
+constexpr int kMatrixLdsBytes = 156 * 1024;
+
+template <typename S>
+struct MatrixView {
+    const S* global;  // all blocks
+    const S* local;   // LDS copy of blocks [begin, begin + staged)
+    int32_t begin, staged;
+};
+
+// One 6x6 block as whole 16-byte loads (blocks are 16-byte aligned in both
+// copies), then its product with the column node's 6-vector into acc. An
+// FP32 block is nine loads; an FP64 block goes row by row (three loads each)
+// to bound the registers it holds.
+template <typename S, typename V, typename Acc>
+__device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&acc)[6]) {
+    Acc xv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xv[c] = static_cast<Acc>(x[c]);
+    if constexpr (sizeof(S) == 4) {
+        S v[36];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const float4 q = reinterpret_cast<const float4*>(blk)[c];
+            v[4 * c] = q.x;
+            v[4 * c + 1] = q.y;
+            v[4 * c + 2] = q.z;
+            v[4 * c + 3] = q.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            Acc t = Acc(0);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) t += static_cast<Acc>(v[6 * r + c]) * xv[c];
+            acc[r] += t;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const double2* row = reinterpret_cast<const double2*>(blk + 6 * r);
+            const double2 a = row[0], b = row[1], c = row[2];
+            const S v[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+            Acc t = Acc(0);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t += static_cast<Acc>(v[k]) * xv[k];
+            acc[r] += t;
+        }
+    }
+}
+
+template <typename S, typename V, typename Acc>
+__device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* __restrict__ x, Acc (&acc)[6]) {
+    const int32_t rel = b - m.begin;
+    if (rel >= 0 && rel < m.staged)
+        add_block<S, V, Acc>(m.local + 36 * rel, x, acc);
+    else
+        add_block<S, V, Acc>(m.global + 36 * static_cast<int64_t>(b), x, acc);
+}
+
 // (A + diag(damp)) u for the six rows of one node, evaluated by the node's
 // eight lanes together: lane j multiplies blocks j, j + 8, ... of the node's
 // block row (the full 6x6 block times the column node's 6-vector), and the
-// eight partial
-// 6-vectors are summed across the lanes in a fixed butterfly order. Every lane
-// of the wave must call it (the sum is a wave shuffle); `valid` false adds
+// eight partial 6-vectors are summed across the lanes in a fixed order. The
+// lane's first block and its column come from registers, so a row of at most
+// eight blocks costs one round of loads. `self` is this lane's entry of u
+// (the row's own entry, which the lane holds) and `damp` its damping. Every
+// lane of the wave must call it (the sum crosses lanes); `valid` false adds
 // zeros. Returns row `lane` of the product (lanes 6 and 7 get row 0). Acc is
 // the accumulation precision (the matrix precision for the iteration, FP64
 // for the refinement residual).
 template <typename S, typename V, typename Acc>
-__device__ inline Acc node_product(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
-                                   const S* __restrict__ bsr, const S* __restrict__ damp, const V* __restrict__ u,
-                                   int64_t node, bool valid, int lane) {
+__device__ inline Acc node_product(const RowShare& share, const int32_t* __restrict__ cols, const MatrixView<S>& m,
+                                   S damp, V self, const V* __restrict__ u, bool valid, int lane) {
     Acc acc[6] = {Acc(0), Acc(0), Acc(0), Acc(0), Acc(0), Acc(0)};
-    if (valid) {
-        const int32_t end = row_off[node + 1];
-        for (int32_t b = row_off[node] + lane; b < end; b += 8) {
-            const S* blk = bsr + 36 * static_cast<int64_t>(b);
-            const V* x = u + 6 * static_cast<int64_t>(cols[b]);
-            Acc xv[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) xv[c] = static_cast<Acc>(x[c]);
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                Acc t = Acc(0);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) t += static_cast<Acc>(blk[6 * r + c]) * xv[c];
-                acc[r] += t;
-            }
-        }
+    if (share.first >= 0) {
+        add_block_at<S, V, Acc>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
+        for (int32_t b = share.first + 8; b < share.end; b += 8)
+            add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
     }
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-        acc[r] += __shfl_xor(acc[r], 1, 64);
-        acc[r] += __shfl_xor(acc[r], 2, 64);
-        acc[r] += __shfl_xor(acc[r], 4, 64);
-    }
+    for (int r = 0; r < 6; ++r) acc[r] = octet_sum(acc[r]);
     Acc out = acc[0];
 #pragma unroll
     for (int r = 1; r < 6; ++r) out = lane == r ? acc[r] : out;
-    if (valid && lane < 6)
-        out += static_cast<Acc>(damp[6 * node + lane]) * static_cast<Acc>(u[6 * node + lane]);
+    if (valid && lane < 6) out += static_cast<Acc>(damp) * static_cast<Acc>(self);
     return out;
 }
 
@@ -245,15 +325,18 @@ __global__ void __launch_bounds__(kPcgThreads)
 pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
            const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ rhs, int64_t nn,
            double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
-           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x waves */,
+           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
            unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
     __shared__ double lds[3 * (kPcgThreads / kWave)];
+    __shared__ float4 matrix_lds[kMatrixLdsBytes / 16];
     const int lane = threadIdx.x & 7;
     const int lane_base = (threadIdx.x & 63) & ~7;
     const bool row_lane = lane < 6;
     const int64_t stride = 6 * nn;
     int64_t node[K];
     bool own[K], live[K];
+    RowShare share[K];
+    S dampk[K];
     S mrow[K][6];
     double b64[K], x64[K];
     S x[K], r[K], u[K], w[K], p[K], s[K], q[K], z[K];
@@ -263,6 +346,8 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         node[k] = (static_cast<int64_t>(blockIdx.x) * K + k) * kNodesPerPass + (threadIdx.x >> 3);
         live[k] = node[k] < nn;
         own[k] = row_lane && live[k];
+        share[k] = row_share(row_off, cols, node[k], live[k], lane);
+        dampk[k] = own[k] ? damp[6 * node[k] + lane] : S(0);
 #pragma unroll
         for (int c = 0; c < 6; ++c) mrow[k][c] = own[k] ? minv[36 * node[k] + 6 * lane + c] : S(0);
         b64[k] = own[k] ? rhs[6 * node[k] + lane] : 0.0;
@@ -271,12 +356,28 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         x[k] = u[k] = w[k] = p[k] = s[k] = q[k] = z[k] = S(0);
         bb += b64[k] * b64[k];
     }
+    // The group's block rows into LDS, as far as they fit.
+    MatrixView<S> mat;
+    {
+        const int64_t first = static_cast<int64_t>(blockIdx.x) * K * kNodesPerPass;
+        const int64_t last = first + K * kNodesPerPass < nn ? first + K * kNodesPerPass : nn;
+        mat.global = bsr;
+        mat.local = reinterpret_cast<const S*>(matrix_lds);
+        mat.begin = first < nn ? row_off[first] : 0;
+        const int32_t blocks = first < nn ? row_off[last] - mat.begin : 0;
+        constexpr int32_t capacity = kMatrixLdsBytes / (36 * static_cast<int32_t>(sizeof(S)));
+        mat.staged = blocks < capacity ? blocks : capacity;
+        const float4* src = reinterpret_cast<const float4*>(bsr + 36 * static_cast<int64_t>(mat.begin));
+        const int32_t chunks = mat.staged * 36 * static_cast<int32_t>(sizeof(S)) / 16;
+        for (int32_t i = threadIdx.x; i < chunks; i += kPcgThreads) matrix_lds[i] = src[i];
+        __syncthreads();
+    }
     int phase = 0;
     double red[3];
     {
         const double v[1] = {bb};
         double o[1];
-        launch_sum<1, Grid>(v, slots + phase * 3 * kPcgWaves * gridDim.x, lds, barrier, o);
+        launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
         phase ^= 1;
         red[0] = o[0];
     }
@@ -300,7 +401,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         sync_all<Grid>(barrier);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const S v = node_product<S, S, S>(row_off, cols, bsr, damp, ex, node[k], live[k], lane);
+            const S v = node_product<S, S, S>(share[k], cols, mat, dampk[k], u[k], ex, live[k], lane);
             w[k] = own[k] ? v : S(0);
         }
         phase ^= 1;
@@ -322,9 +423,8 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
                 part[2] += rk * rk;
             }
             if (Grid) {
-                publish<3>(part, slots + phase * 3 * kPcgWaves * gridDim.x);
-                grid_barrier(barrier, gridDim.x);
-                fold<3>(slots + phase * 3 * kPcgWaves * gridDim.x, red);
+                publish_and_wait<3>(part, lds, slots + phase * 3 * gridDim.x, barrier);
+                fold<3>(slots + phase * 3 * gridDim.x, red);
             } else {
                 block_sum<3, kPcgThreads>(part, lds, red);
             }
@@ -338,7 +438,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             S n[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const S v = node_product<S, S, S>(row_off, cols, bsr, damp, ex, node[k], live[k], lane);
+                const S v = node_product<S, S, S>(share[k], cols, mat, dampk[k], m[k], ex, live[k], lane);
                 n[k] = own[k] ? v : S(0);
             }
             phase ^= 1;
@@ -383,14 +483,15 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         double r64[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double v = node_product<S, double, double>(row_off, cols, bsr, damp, xbuf, node[k], live[k], lane);
+            const double v =
+                node_product<S, double, double>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
             r64[k] = own[k] ? b64[k] - v : 0.0;
             rr += r64[k] * r64[k];
         }
         {
             const double v[1] = {rr};
             double o[1];
-            launch_sum<1, Grid>(v, slots + phase * 3 * kPcgWaves * gridDim.x, lds, barrier, o);
+            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
             phase ^= 1;
             red[0] = o[0];
         }
@@ -461,7 +562,7 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
         kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, false>)
                         : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
     }
-    ws.reduce.resize(6 * static_cast<int64_t>(kPcgWaves) * groups);
+    ws.reduce.resize(6 * static_cast<int64_t>(groups));
     ws.barrier.resize(2);
     ws.barrier.zero(stream);
     ws.outcome.resize(4);
