@@ -273,7 +273,10 @@ int nricp_hip_bind_vertices(const double* points, int64_t n, const double* nodes
         b.resize(4 * n);
         w.resize(4 * n);
         c.resize(n);
-        bind_vertices(p.get(), n, g.get(), nn, k, b.get(), w.get(), c.get(), st);
+        SortScratch sort;
+        NodeGrid grid;
+        build_node_grid(g.get(), nn, grid, sort, st);
+        bind_vertices(p.get(), n, g.get(), nn, grid, k, b.get(), w.get(), c.get(), st);
         b.download(on, 4 * n, st);
         w.download(ow, 4 * n, st);
         c.download(oc, n, st);
@@ -294,7 +297,9 @@ int nricp_hip_build_node_edges(const double* nodes, int64_t nn, int32_t k, int32
         DeviceBuffer<int32_t> e, flags, pos;
         DeviceBuffer<uint64_t> keys;
         g.upload(nodes, 3 * nn, st);
-        const int64_t m = build_node_edges(g.get(), nn, k, e, keys, flags, pos, sort, st);
+        NodeGrid grid;
+        build_node_grid(g.get(), nn, grid, sort, st);
+        const int64_t m = build_node_edges(g.get(), nn, grid, k, e, keys, flags, pos, sort, st);
         *count = m;
         if (out && capacity > 0) e.download(out, 2 * std::min(m, capacity), st);
         NRICP_CHECK(hipStreamSynchronize(st));
@@ -361,7 +366,10 @@ int nricp_hip_transfer_level(const double* from, int64_t nf, const double* R, co
         r.upload(R, 9 * nf, st);
         t.upload(T, 3 * nf, st);
         g.upload(to, 3 * nt, st);
-        transfer_level(f.get(), nf, r.get(), t.get(), g.get(), nt, k, ro.get(), tout.get(), st);
+        SortScratch sort;
+        NodeGrid grid;
+        build_node_grid(f.get(), nf, grid, sort, st);
+        transfer_level(f.get(), nf, grid, r.get(), t.get(), g.get(), nt, k, ro.get(), tout.get(), st);
         ro.download(Ro, 9 * nt, st);
         tout.download(To, 3 * nt, st);
         NRICP_CHECK(hipStreamSynchronize(st));

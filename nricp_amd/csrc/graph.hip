@@ -337,6 +337,130 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
     }
 }
 
+// ------------------------------------------------------------ node grid ----
+//
+// Exact nearest-node searches over a uniform grid of cells. Nodes are sorted
+// by cell; a search visits the cells in cubic shells around the query's cell
+// and stops once its k-th best is closer than the nearest face of the visited
+// cube that still has cells beyond it (less a margin far above the rounding
+// of the cell assignment), or every cell has been visited. The visit order
+// does not matter: candidates are kept in the total (distance, index) order,
+// so the answer is the brute-force answer.
+
+struct GridView {
+    const double* frame;  // origin xyz, cell size
+    int cells;            // per axis
+    const int32_t* runs;  // per cell [start, end) in the sorted order
+    const double4* sorted;  // (x, y, z, node index) in cell order
+};
+
+__device__ inline int grid_axis_cell(double x, double origin, double h, int cells) {
+    const double t = floor((x - origin) / h);
+    return t < 0.0 ? 0 : (t > static_cast<double>(cells - 1) ? cells - 1 : static_cast<int>(t));
+}
+
+// Visits every node of the shells around x's cell, shell by shell, calling
+// visit(index, position) for each, until done(bound) holds after a shell
+// (bound: every node not yet visited is at least that far from x) or no cell
+// is left.
+template <class Visit, class Done>
+__device__ inline void grid_search(const GridView& g, d3 x, Visit visit, Done done) {
+    const double o[3] = {g.frame[0], g.frame[1], g.frame[2]};
+    const double h = g.frame[3];
+    const double xv[3] = {x.x, x.y, x.z};
+    int c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = grid_axis_cell(xv[k], o[k], h, g.cells);
+    const double margin = 1e-9 * h;
+    for (int r = 0;; ++r) {
+        const int z0 = max(c[2] - r, 0), z1 = min(c[2] + r, g.cells - 1);
+        const int y0 = max(c[1] - r, 0), y1 = min(c[1] + r, g.cells - 1);
+        const int x0 = max(c[0] - r, 0), x1 = min(c[0] + r, g.cells - 1);
+        for (int cz = z0; cz <= z1; ++cz) {
+            const bool zface = cz == c[2] - r || cz == c[2] + r;
+            for (int cy = y0; cy <= y1; ++cy) {
+                const bool yface = zface || cy == c[1] - r || cy == c[1] + r;
+                for (int cx = x0; cx <= x1; ++cx) {
+                    if (!yface && cx != c[0] - r && cx != c[0] + r) continue;  // inside: an earlier shell
+                    const int64_t cell = (static_cast<int64_t>(cz) * g.cells + cy) * g.cells + cx;
+                    const int32_t e = g.runs[2 * cell + 1];
+                    for (int32_t t = g.runs[2 * cell]; t < e; ++t) {
+                        const double4 q = g.sorted[t];
+                        visit(static_cast<int32_t>(q.w), d3{q.x, q.y, q.z});
+                    }
+                }
+            }
+        }
+        // Distance from x to the nearest face of the visited cube that has
+        // cells beyond it.
+        double bound = __builtin_inf();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (c[k] - r > 0) bound = fmin(bound, xv[k] - (o[k] + static_cast<double>(c[k] - r) * h));
+            if (c[k] + r < g.cells - 1) bound = fmin(bound, (o[k] + static_cast<double>(c[k] + r + 1) * h) - xv[k]);
+        }
+        if (bound == __builtin_inf()) return;
+        if (done(bound - margin)) return;
+    }
+}
+
+// Box of the nodes and the grid frame: origin = box low corner, cell size =
+// largest extent / cells (a little larger, so the far corner is in the last
+// cell; 1 for a degenerate box). One block.
+__global__ void __launch_bounds__(256) grid_frame_kernel(const double* __restrict__ p, int64_t n, int cells,
+                                                         double* __restrict__ frame) {
+    __shared__ double lo[3][256], hi[3][256];
+    double l[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, u[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = threadIdx.x; i < n; i += 256)
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fmin(l[k], p[3 * i + k]);
+            u[k] = fmax(u[k], p[3 * i + k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+        lo[k][threadIdx.x] = l[k];
+        hi[k][threadIdx.x] = u[k];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                lo[k][threadIdx.x] = fmin(lo[k][threadIdx.x], lo[k][threadIdx.x + s]);
+                hi[k][threadIdx.x] = fmax(hi[k][threadIdx.x], hi[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double extent = 0.0;
+        for (int k = 0; k < 3; ++k) extent = fmax(extent, hi[k][0] - lo[k][0]);
+        for (int k = 0; k < 3; ++k) frame[k] = lo[k][0];
+        frame[3] = extent > 0.0 ? extent / cells * (1.0 + 1e-6) : 1.0;
+    }
+}
+
+__global__ void grid_keys_kernel(const double* __restrict__ p, int64_t n, const double* __restrict__ frame,
+                                 int cells, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double h = frame[3];
+    const int cx = grid_axis_cell(p[3 * i], frame[0], h, cells);
+    const int cy = grid_axis_cell(p[3 * i + 1], frame[1], h, cells);
+    const int cz = grid_axis_cell(p[3 * i + 2], frame[2], h, cells);
+    keys[i] = (static_cast<uint64_t>(cz) * cells + cy) * cells + cx;
+    vals[i] = static_cast<uint32_t>(i);
+}
+
+__global__ void grid_fill_kernel(const double* __restrict__ p, int64_t n, const uint64_t* __restrict__ keys,
+                                 const uint32_t* __restrict__ vals, int32_t* __restrict__ runs,
+                                 double4* __restrict__ sorted) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t c = keys[i];
+    if (i == 0 || keys[i - 1] != c) runs[2 * c] = static_cast<int32_t>(i);
+    if (i == n - 1 || keys[i + 1] != c) runs[2 * c + 1] = static_cast<int32_t>(i + 1);
+    const uint32_t j = vals[i];
+    sorted[i] = make_double4(p[3 * j], p[3 * j + 1], p[3 * j + 2], static_cast<double>(j));
+}
+
 // Sorted insertion of (d, j) into the KMAX best entries kept in ascending
 // (d, index) order: the candidate bubbles down the fully unrolled list, so
 // every index is a compile-time constant and the list stays in registers.
@@ -355,40 +479,26 @@ __device__ inline void insert_sorted(double d, int32_t j, double* dist, int32_t*
     }
 }
 
+__device__ inline double pick4(const double* v, int k) {
+    return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
+
 // Nearest 4 nodes by (d^2, index), the order the CPU route's keep_nearest
 // produces (candidates visited in increasing index, ties kept by the lower
-// index); callers use the first `support` of them. Nodes are staged through
-// LDS a tile at a time by the whole block, so every thread of the block must
-// call it, including those past the end of the rows.
-template <int Threads>
-__device__ inline void nearest4(d3 x, const double* __restrict__ nodes, int64_t nn, double* d2, int32_t* id,
-                                double* tile) {
+// index); callers use the first `need` (at most the node count) of them.
+__device__ inline void nearest4(d3 x, const GridView& g, int need, double* d2, int32_t* id) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         d2[m] = __builtin_inf();
         id[m] = 0x7fffffff;
     }
-    for (int64_t base = 0; base < nn; base += Threads) {
-        __syncthreads();
-        const int64_t j = base + threadIdx.x;
-        if (j < nn) {
-            tile[3 * threadIdx.x] = nodes[3 * j];
-            tile[3 * threadIdx.x + 1] = nodes[3 * j + 1];
-            tile[3 * threadIdx.x + 2] = nodes[3 * j + 2];
-        }
-        __syncthreads();
-        const int count = static_cast<int>(nn - base < Threads ? nn - base : Threads);
-        for (int t = 0; t < count; ++t) {
-            const d3 g{tile[3 * t], tile[3 * t + 1], tile[3 * t + 2]};
-            const double s = norm2(x - g);
-            if (s < d2[3] || (s == d2[3] && base + t < id[3]))
-                insert_sorted<4>(s, static_cast<int32_t>(base + t), d2, id);
-        }
-    }
-}
-
-__device__ inline double pick4(const double* v, int k) {
-    return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+    grid_search(
+        g, x,
+        [&](int32_t j, d3 p) {
+            const double s = norm2(x - p);
+            if (s < d2[3] || (s == d2[3] && j < id[3])) insert_sorted<4>(s, j, d2, id);
+        },
+        [&](double bound) { return bound > 0.0 && pick4(d2, need - 1) < bound * bound; });
 }
 
 // (1 - d/dmax)^2 over the first `support` entries, normalised (uniform when
@@ -415,15 +525,14 @@ __device__ inline void distance_weights(const double* d2, int support, double* w
 constexpr int kBindThreads = 128;
 
 __global__ void __launch_bounds__(kBindThreads)
-bind_kernel(const double* __restrict__ p, int64_t n, const double* __restrict__ nodes, int64_t nn, int support,
-            int32_t* __restrict__ on, double* __restrict__ ow, int32_t* __restrict__ oc) {
-    __shared__ double tile[3 * kBindThreads];
+bind_kernel(const double* __restrict__ p, int64_t n, GridView grid, int support, int32_t* __restrict__ on,
+            double* __restrict__ ow, int32_t* __restrict__ oc) {
     const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const d3 x = v < n ? ld3(p, v) : d3{0.0, 0.0, 0.0};
+    if (v >= n) return;
+    const d3 x = ld3(p, v);
     double d2[4];
     int32_t id[4];
-    nearest4<kBindThreads>(x, nodes, nn, d2, id, tile);
-    if (v >= n) return;
+    nearest4(x, grid, support, d2, id);
     double w[4];
     distance_weights(d2, support, w);
 #pragma unroll
@@ -436,13 +545,13 @@ bind_kernel(const double* __restrict__ p, int64_t n, const double* __restrict__ 
 
 // k nearest other nodes by (Euclidean distance, index); emits the (min, max)
 // pair of each as a sortable key. KMAX >= k entries are kept (the first k are
-// the answer); node positions are staged through LDS a tile at a time.
-constexpr int kKnnThreads = 64;
+// the answer).
+constexpr int kKnnThreads = 128;
 template <int KMAX>
 __global__ void __launch_bounds__(kKnnThreads)
-knn_kernel(const double* __restrict__ nodes, int64_t nn, int k, uint64_t* __restrict__ keys) {
-    __shared__ double tile[3 * kKnnThreads];
+knn_kernel(const double* __restrict__ nodes, int64_t nn, GridView grid, int k, uint64_t* __restrict__ keys) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
     double dist[KMAX];
     int32_t id[KMAX];
 #pragma unroll
@@ -450,27 +559,21 @@ knn_kernel(const double* __restrict__ nodes, int64_t nn, int k, uint64_t* __rest
         dist[m] = __builtin_inf();
         id[m] = 0x7fffffff;
     }
-    const d3 x = i < nn ? ld3(nodes, i) : d3{0.0, 0.0, 0.0};
-    for (int64_t base = 0; base < nn; base += kKnnThreads) {
-        __syncthreads();
-        const int64_t j = base + threadIdx.x;
-        if (j < nn) {
-            tile[3 * threadIdx.x] = nodes[3 * j];
-            tile[3 * threadIdx.x + 1] = nodes[3 * j + 1];
-            tile[3 * threadIdx.x + 2] = nodes[3 * j + 2];
-        }
-        __syncthreads();
-        const int count = static_cast<int>(nn - base < kKnnThreads ? nn - base : kKnnThreads);
-        for (int t = 0; t < count; ++t) {
-            if (base + t == i) continue;
-            const d3 g{tile[3 * t], tile[3 * t + 1], tile[3 * t + 2]};
-            const double s = sqrt(norm2(x - g));
-            if (s < dist[KMAX - 1] || (s == dist[KMAX - 1] && base + t < id[KMAX - 1]))
-                insert_sorted<KMAX>(s, static_cast<int32_t>(base + t), dist, id);
-        }
-    }
-    if (i >= nn) return;
+    const d3 x = ld3(nodes, i);
     const int held = static_cast<int>(nn - 1 < k ? nn - 1 : k);
+    grid_search(
+        grid, x,
+        [&](int32_t j, d3 g) {
+            if (j == i) return;
+            const double s = sqrt(norm2(x - g));
+            if (s < dist[KMAX - 1] || (s == dist[KMAX - 1] && j < id[KMAX - 1])) insert_sorted<KMAX>(s, j, dist, id);
+        },
+        [&](double bound) {
+            double kth = dist[0];
+#pragma unroll
+            for (int m = 1; m < KMAX; ++m) kth = m == held - 1 ? dist[m] : kth;
+            return kth < bound;
+        });
 #pragma unroll
     for (int m = 0; m < KMAX; ++m) {
         if (m >= k) break;
@@ -497,16 +600,15 @@ __global__ void unique_edges(const uint64_t* __restrict__ keys, const int32_t* _
 // the heaviest contributor, heaviest-neighbour fallback when the blend nearly
 // cancels; translation maps the new rest position onto the field's value.
 __global__ void __launch_bounds__(kBindThreads)
-transfer_kernel(const double* __restrict__ from, int64_t nf, const double* __restrict__ R,
-                                const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
-                                double* __restrict__ Ro, double* __restrict__ To) {
-    __shared__ double tile[3 * kBindThreads];
+transfer_kernel(const double* __restrict__ from, GridView grid, const double* __restrict__ R,
+                const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
+                double* __restrict__ Ro, double* __restrict__ To) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const d3 x = i < nt ? ld3(to, i) : d3{0.0, 0.0, 0.0};
+    if (i >= nt) return;
+    const d3 x = ld3(to, i);
     double d2[4];
     int32_t id[4];
-    nearest4<kBindThreads>(x, from, nf, d2, id, tile);
-    if (i >= nt) return;
+    nearest4(x, grid, support, d2, id);
     double w[4];
     distance_weights(d2, support, w);
     int32_t heavy = id[0];
@@ -621,15 +723,49 @@ void farthest_point_seeds(const double* points, int64_t n, int32_t seeds, int32_
     farthest_point_seeds_range(points, n, first, 0, seeds, out, w, stream);
 }
 
-void bind_vertices(const double* points, int64_t n, const double* nodes, int64_t nn, int32_t k, int32_t* on,
-                   double* ow, int32_t* oc, hipStream_t stream) {
+namespace {
+GridView view_of(const NodeGrid& g) { return GridView{g.frame.get(), g.cells, g.runs.get(), g.sorted.get()}; }
+}  // namespace
+
+void build_node_grid(const double* nodes, int64_t nn, NodeGrid& g, SortScratch& scratch, hipStream_t stream) {
+    g.count = nn;
+    if (nn <= 0) return;
+    if (nn >= (1ll << 31)) throw InvalidArgument("too many nodes");
+    // About one cell per node: a surface's nodes fill ~ cells^(2/3) of them,
+    // a few per occupied cell.
+    const int cells = std::clamp(static_cast<int>(std::ceil(std::cbrt(static_cast<double>(nn)))), 1, 128);
+    const int64_t total = static_cast<int64_t>(cells) * cells * cells;
+    g.cells = cells;
+    g.frame.resize(4);
+    grid_frame_kernel<<<1, 256, 0, stream>>>(nodes, nn, cells, g.frame.get());
+    NRICP_LAUNCH_CHECK("grid_frame_kernel");
+    g.keys.resize(nn);
+    g.vals.resize(nn);
+    grid_keys_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(nodes, nn, g.frame.get(), cells, g.keys.get(),
+                                                               g.vals.get());
+    NRICP_LAUNCH_CHECK("grid_keys_kernel");
+    radix_sort_pairs(g.keys.get(), g.vals.get(), nn, std::max(1, bits_for(static_cast<uint64_t>(total - 1))), scratch,
+                     stream);
+    g.runs.resize(2 * total);
+    g.runs.zero(stream);
+    g.sorted.resize(nn);
+    grid_fill_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(nodes, nn, g.keys.get(), g.vals.get(), g.runs.get(),
+                                                               g.sorted.get());
+    NRICP_LAUNCH_CHECK("grid_fill_kernel");
+}
+
+void bind_vertices(const double* points, int64_t n, const double* nodes, int64_t nn, const NodeGrid& grid, int32_t k,
+                   int32_t* on, double* ow, int32_t* oc, hipStream_t stream) {
     if (n <= 0 || nn <= 0) return;
+    if (grid.count != nn) throw InvalidArgument("bind_vertices: the grid is not over these nodes");
+    (void)nodes;
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nn, 4)), 4}));
-    bind_kernel<<<blocks_for(n, kBindThreads), kBindThreads, 0, stream>>>(points, n, nodes, nn, support, on, ow, oc);
+    bind_kernel<<<blocks_for(n, kBindThreads), kBindThreads, 0, stream>>>(points, n, view_of(grid), support, on, ow,
+                                                                           oc);
     NRICP_LAUNCH_CHECK("bind_kernel");
 }
 
-int64_t build_node_edges(const double* nodes, int64_t nn, int32_t k, DeviceBuffer<int32_t>& edges,
+int64_t build_node_edges(const double* nodes, int64_t nn, const NodeGrid& grid, int32_t k, DeviceBuffer<int32_t>& edges,
                          DeviceBuffer<uint64_t>& keys, DeviceBuffer<int32_t>& flags, DeviceBuffer<int32_t>& pos,
                          SortScratch& scratch, hipStream_t stream) {
     if (nn <= 1 || k <= 0) {
@@ -640,13 +776,15 @@ int64_t build_node_edges(const double* nodes, int64_t nn, int32_t k, DeviceBuffe
     const int kk = static_cast<int>(std::min<int64_t>(k, nn - 1));
     const int64_t m = nn * kk;
     keys.resize(m);
-    const dim3 grid(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
+    if (grid.count != nn) throw InvalidArgument("build_node_edges: the grid is not over these nodes");
+    const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
+    const GridView view = view_of(grid);
     if (kk <= 4)
-        knn_kernel<4><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        knn_kernel<4><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
     else if (kk <= 8)
-        knn_kernel<8><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        knn_kernel<8><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
     else
-        knn_kernel<16><<<grid, kKnnThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        knn_kernel<16><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
     NRICP_LAUNCH_CHECK("knn_kernel");
     radix_sort_pairs(keys.get(), nullptr, m, bits_for(static_cast<uint64_t>(nn) * static_cast<uint64_t>(nn)), scratch,
                      stream);
@@ -665,11 +803,14 @@ int64_t build_node_edges(const double* nodes, int64_t nn, int32_t k, DeviceBuffe
     return count;
 }
 
-void transfer_level(const double* from, int64_t nf, const double* R, const double* T, const double* to, int64_t nt,
-                    int32_t k, double* Ro, double* To, hipStream_t stream) {
+void transfer_level(const double* from, int64_t nf, const NodeGrid& from_grid, const double* R, const double* T,
+                    const double* to, int64_t nt, int32_t k, double* Ro, double* To, hipStream_t stream) {
     if (nt <= 0) return;
+    if (nf <= 0) throw InvalidArgument("transfer_level: the coarser level has no nodes");
+    if (from_grid.count != nf) throw InvalidArgument("transfer_level: the grid is not over the coarser nodes");
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nf, 4)), 4}));
-    transfer_kernel<<<blocks_for(nt, kBindThreads), kBindThreads, 0, stream>>>(from, nf, R, T, to, nt, support, Ro, To);
+    transfer_kernel<<<blocks_for(nt, kBindThreads), kBindThreads, 0, stream>>>(from, view_of(from_grid), R, T, to, nt,
+                                                                                support, Ro, To);
     NRICP_LAUNCH_CHECK("transfer_kernel");
 }
 

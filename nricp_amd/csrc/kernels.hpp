@@ -163,18 +163,33 @@ void farthest_point_seeds(const double* points, int64_t count, int32_t seeds, in
                           FpsWork& work, hipStream_t stream);
 void farthest_point_seeds_range(const double* points, int64_t count, int32_t first, int32_t begin, int32_t end,
                                 int32_t* out, FpsWork& work, hipStream_t stream);
+// Uniform grid over a node set (cells sorted by key, per-cell runs) for exact
+// nearest-node searches: bind_vertices, build_node_edges and transfer_level
+// take the grid of the nodes they search.
+struct NodeGrid {
+    int64_t count = 0;
+    int cells = 0;                  // per axis
+    DeviceBuffer<double> frame;     // origin xyz, cell size
+    DeviceBuffer<uint64_t> keys;    // cell of each node, sorted
+    DeviceBuffer<uint32_t> vals;    // node index, in cell order
+    DeviceBuffer<int32_t> runs;     // per cell [start, end)
+    DeviceBuffer<double4> sorted;   // (x, y, z, node index) in cell order
+};
+void build_node_grid(const double* nodes, int64_t node_count, NodeGrid& grid, SortScratch& scratch,
+                     hipStream_t stream);
 // Nearest `k` nodes by (d^2, index), weights (1 - d/dmax)^2 normalised.
-void bind_vertices(const double* points, int64_t count, const double* nodes, int64_t node_count, int32_t k,
-                   int32_t* out_nodes, double* out_weights, int32_t* out_counts, hipStream_t stream);
+void bind_vertices(const double* points, int64_t count, const double* nodes, int64_t node_count,
+                   const NodeGrid& grid, int32_t k, int32_t* out_nodes, double* out_weights, int32_t* out_counts,
+                   hipStream_t stream);
 // k nearest by (distance, index) per node, union as sorted unique (min, max)
 // pairs. Returns the edge count (synchronises).
-int64_t build_node_edges(const double* nodes, int64_t node_count, int32_t k, DeviceBuffer<int32_t>& edges,
-                         DeviceBuffer<uint64_t>& keys, DeviceBuffer<int32_t>& flags, DeviceBuffer<int32_t>& pos,
-                         SortScratch& scratch, hipStream_t stream);
+int64_t build_node_edges(const double* nodes, int64_t node_count, const NodeGrid& grid, int32_t k,
+                         DeviceBuffer<int32_t>& edges, DeviceBuffer<uint64_t>& keys, DeviceBuffer<int32_t>& flags,
+                         DeviceBuffer<int32_t>& pos, SortScratch& scratch, hipStream_t stream);
 // Warm start of a finer level from a coarser one.
-void transfer_level(const double* from_nodes, int64_t from_count, const double* from_rotations,
-                    const double* from_translations, const double* to_nodes, int64_t to_count, int32_t k,
-                    double* out_rotations, double* out_translations, hipStream_t stream);
+void transfer_level(const double* from_nodes, int64_t from_count, const NodeGrid& from_grid,
+                    const double* from_rotations, const double* from_translations, const double* to_nodes,
+                    int64_t to_count, int32_t k, double* out_rotations, double* out_translations, hipStream_t stream);
 
 // ------------------------------------------------------------ layout -------
 

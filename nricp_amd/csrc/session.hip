@@ -103,9 +103,10 @@ void build_graph_level(const double* points, int64_t nv, const int32_t* seeds, i
     L.bind_nodes.resize(4 * nv);
     L.bind_weights.resize(4 * nv);
     L.bind_counts.resize(nv);
-    bind_vertices(points, nv, L.node_positions.get(), node_count, k, L.bind_nodes.get(), L.bind_weights.get(),
-                  L.bind_counts.get(), stream);
-    L.edge_count = build_node_edges(L.node_positions.get(), node_count, neighbours, L.edges, L.edge_keys,
+    build_node_grid(L.node_positions.get(), node_count, L.grid, scratch, stream);
+    bind_vertices(points, nv, L.node_positions.get(), node_count, L.grid, k, L.bind_nodes.get(),
+                  L.bind_weights.get(), L.bind_counts.get(), stream);
+    L.edge_count = build_node_edges(L.node_positions.get(), node_count, L.grid, neighbours, L.edges, L.edge_keys,
                                     L.edge_flags, L.edge_pos, scratch, stream);
 }
 
@@ -460,7 +461,7 @@ void Session::begin_level() {
         NRICP_LAUNCH_CHECK("identity_state");
     } else {
         const GraphLevel& P = graph_->levels[level_ - 1];
-        transfer_level(P.node_positions.get(), P.node_count, rot_.get(), trans_.get(), L.node_positions.get(),
+        transfer_level(P.node_positions.get(), P.node_count, P.grid, rot_.get(), trans_.get(), L.node_positions.get(),
                        L.node_count, 4, rot_cand_.get(), trans_cand_.get(), st);
         std::swap(rot_, rot_cand_);
         std::swap(trans_, trans_cand_);

@@ -25,6 +25,7 @@ struct GraphLevel {
     DeviceBuffer<int32_t> edges;
     DeviceBuffer<uint64_t> edge_keys;  // scratch of the edge build
     DeviceBuffer<int32_t> edge_flags, edge_pos;
+    NodeGrid grid;                      // over node_positions
 };
 
 struct DeviceGraph {

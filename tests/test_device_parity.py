@@ -101,6 +101,66 @@ def test_bindings_and_edges_are_identical(device, oracle):
         assert np.array_equal(device.build_node_edges(nodes, k), oracle.build_node_edges(nodes, k))
 
 
+def _awkward_node_sets():
+    rng = np.random.default_rng(11)
+    lattice = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij"), -1).reshape(-1, 3)
+    yield "lattice (exact ties)", lattice.astype(np.float64), rng.uniform(-1, 6, size=(800, 3))
+    line = np.zeros((40, 3))
+    line[:, 0] = np.linspace(0.0, 1.0, 40)
+    yield "collinear", line, rng.normal(size=(500, 3))
+    yield "coincident", np.ones((7, 3)), rng.normal(size=(300, 3))
+    yield "far queries", rng.normal(size=(200, 3)), rng.normal(size=(400, 3)) * 50.0
+    yield "three nodes", rng.normal(size=(3, 3)), rng.normal(size=(100, 3))
+    sheet = rng.uniform(size=(5000, 3)) * np.array([10.0, 10.0, 0.01])
+    yield "thin sheet", sheet[oracle_free_fps(sheet, 700)], sheet
+
+
+def oracle_free_fps(points, count):
+    # Greedy farthest points (numpy), only to spread the nodes of a fixture.
+    chosen = [0]
+    d = np.sum((points - points[0]) ** 2, axis=1)
+    for _ in range(count - 1):
+        j = int(np.argmax(d))
+        chosen.append(j)
+        d = np.minimum(d, np.sum((points - points[j]) ** 2, axis=1))
+    return np.asarray(chosen)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_grid_searches_match_brute_force_on_awkward_node_sets(device, oracle, case):
+    # The device searches nodes through a uniform grid; the oracle scans them
+    # all. Lattices give exact distance ties, coincident nodes a degenerate
+    # box, far queries long searches, a thin sheet flat cells.
+    name, nodes, queries = list(_awkward_node_sets())[case]
+    for k in (1, 4):
+        d = device.bind_vertices(queries, nodes, k)
+        o = oracle.bind_vertices(queries, nodes, k)
+        for key in ("nodes", "weights", "counts"):
+            assert np.array_equal(d[key], o[key]), (name, k, key)
+    for k in (4, 8, 16):
+        assert np.array_equal(device.build_node_edges(nodes, k), oracle.build_node_edges(nodes, k)), (name, k)
+    rng = np.random.default_rng(case)
+    angles = rng.normal(size=(len(nodes), 3)) * 0.3
+    rotations = np.stack([_rotation(a) for a in angles])
+    translations = rng.normal(size=(len(nodes), 3)) * 0.1
+    td = device.transfer_level(nodes, rotations, translations, queries)
+    to = oracle.transfer_level(nodes, rotations, translations, queries)
+    # The blend's arithmetic is not bit for bit the oracle's (as in the level
+    # transfer test above); a different set of nearest nodes would show as an
+    # O(1) difference.
+    for key in ("rotations", "translations"):
+        assert np.allclose(td[key], to[key], rtol=0, atol=1e-12), (name, key)
+
+
+def _rotation(w):
+    t = np.linalg.norm(w)
+    if t == 0.0:
+        return np.eye(3)
+    k = w / t
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
+
+
 @pytest.mark.parametrize("neighbours", [4, 8])
 def test_template_graph_is_identical(device, oracle, neighbours):
     points, faces = uv_sphere(rings=24, segments=20)
