@@ -430,30 +430,62 @@ struct Nearest {
     uint32_t opens, tests;  // traversal work, for profiling
 };
 
-// Lexicographic (d2, triangle) minimum over the group's eight lanes.
-__device__ inline void group_min(double& d2, int32_t& tri, int32_t& leaf) {
-#pragma unroll
-    for (int m = 1; m < kLanes; m <<= 1) {
-        const double od = __shfl_xor(d2, m, 64);
-        const int32_t ot = __shfl_xor(tri, m, 64);
-        const int32_t ol = __shfl_xor(leaf, m, 64);
-        if (od < d2 || (od == d2 && ot < tri)) {
-            d2 = od;
-            tri = ot;
-            leaf = ol;
-        }
+// Moves within a group of eight lanes by DPP (register to register): lane j
+// receives lane j ^ k. Quad permutations give k = 1, 2, 3; the half-row
+// mirror gives k = 7; a quad permutation of the mirrored value gives
+// k = 4, 5, 6. Groups are converged (all eight lanes active together), so
+// every source lane is live.
+constexpr int kDppQuadSwap3 = 0x1B;  // quad_perm [3, 2, 1, 0]
+
+template <int K, typename T>
+__device__ inline T octet_xor(T v) {
+    static_assert(K >= 1 && K <= 7, "partner within the octet");
+    if constexpr (K == 1) return dpp_move<kDppQuadSwap1>(v);
+    if constexpr (K == 2) return dpp_move<kDppQuadSwap2>(v);
+    if constexpr (K == 3) return dpp_move<kDppQuadSwap3>(v);
+    if constexpr (K == 7) return dpp_move<kDppRowHalfMirror>(v);
+    if constexpr (K == 4) return dpp_move<kDppQuadSwap3>(dpp_move<kDppRowHalfMirror>(v));
+    if constexpr (K == 5) return dpp_move<kDppQuadSwap2>(dpp_move<kDppRowHalfMirror>(v));
+    if constexpr (K == 6) return dpp_move<kDppQuadSwap1>(dpp_move<kDppRowHalfMirror>(v));
+    return v;
+}
+
+template <int K>
+__device__ inline void group_min_step(double& d2, int32_t& tri, int32_t& leaf) {
+    const double od = octet_xor<K>(d2);
+    const int32_t ot = octet_xor<K>(tri);
+    const int32_t ol = octet_xor<K>(leaf);
+    if (od < d2 || (od == d2 && ot < tri)) {
+        d2 = od;
+        tri = ot;
+        leaf = ol;
     }
 }
 
-__device__ inline int group_sum(int v) {
-#pragma unroll
-    for (int m = 1; m < kLanes; m <<= 1) v += __shfl_xor(v, m, 64);
-    return v;
+// Lexicographic (d2, triangle) minimum over the group's eight lanes: pairs,
+// then quads, then the two quads (the order is total, so every lane keeps
+// the same winner).
+__device__ inline void group_min(double& d2, int32_t& tri, int32_t& leaf) {
+    group_min_step<1>(d2, tri, leaf);
+    group_min_step<2>(d2, tri, leaf);
+    group_min_step<7>(d2, tri, leaf);
+}
+
+__device__ inline int group_sum(int v) { return octet_sum(v); }
+
+// How many pushing lanes of the group come before lane j in the (distance,
+// lane) order.
+template <int K>
+__device__ inline void rank_step(double d, int push, int j, int& rank) {
+    const double od = octet_xor<K>(d);
+    const int op = octet_xor<K>(push);
+    const int other = j ^ K;
+    if (op && (od < d || (od == d && other < j))) ++rank;
 }
 
 __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
                                     const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
-                                    int32_t* stack_node, float* stack_dist, int j, int lane_base, Nearest& best) {
+                                    int32_t* stack_node, float* stack_dist, int j, Nearest& best) {
     int sp = 0;
     int32_t node = 0;
     int64_t budget = 2 * wide_count + 64;
@@ -498,13 +530,14 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
         // Internal children still within reach, pushed nearest on top.
         const bool push = ref != kWideEmpty && ref >= 0 && !(d > best.d2 * kPruneSlack);
         int rank = 0;
-#pragma unroll
-        for (int k = 1; k < kLanes; ++k) {
-            const int other = (j + k) & (kLanes - 1);
-            const double od = __shfl(d, lane_base + other, 64);
-            const int op = __shfl(push ? 1 : 0, lane_base + other, 64);
-            if (op && (od < d || (od == d && other < j))) ++rank;
-        }
+        const int pushing = push ? 1 : 0;
+        rank_step<1>(d, pushing, j, rank);
+        rank_step<2>(d, pushing, j, rank);
+        rank_step<3>(d, pushing, j, rank);
+        rank_step<4>(d, pushing, j, rank);
+        rank_step<5>(d, pushing, j, rank);
+        rank_step<6>(d, pushing, j, rank);
+        rank_step<7>(d, pushing, j, rank);
         const int npush = group_sum(push ? 1 : 0);
         if (sp + npush > kStackDepth) {
             best.triangle = -1;
@@ -565,7 +598,6 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
     __shared__ float stack_dist[kGroupsPerBlock][kStackDepth];
     const int j = threadIdx.x % kLanes;
     const int group = threadIdx.x / kLanes;
-    const int lane_base = (threadIdx.x % 64) & ~(kLanes - 1);
     const int64_t slot = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
     if (slot >= count) return;  // the whole group leaves together
     const int64_t i = order ? static_cast<int64_t>(order[slot]) : slot;
@@ -601,7 +633,7 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
         best.triangle = seed;
     }
     if (wide_count > 0) {
-        nearest_wide(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group], j, lane_base,
+        nearest_wide(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group], j,
                      best);
     } else if (nt == 1) {
         const double* c = corners;

@@ -131,26 +131,54 @@ __device__ inline bool beats(double v, int32_t i, double bv, int32_t bi) {
     return v > bv || (v == bv && i < bi);
 }
 
-// Butterfly arg-max over the wave: every lane ends with the winner.
-__device__ inline void wave_argmax(double& v, int32_t& i) {
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) {
-        const double ov = __shfl_xor(v, offset, 64);
-        const int32_t oi = __shfl_xor(i, offset, 64);
-        if (beats(ov, oi, v, i)) {
-            v = ov;
-            i = oi;
-        }
+// One arg-max step against the lane a DPP move pairs with (pairs are
+// symmetric, and the order is total, so both lanes keep the same winner).
+template <int Ctrl>
+__device__ inline void argmax_step(double& v, int32_t& i) {
+    const double ov = dpp_move<Ctrl>(v);
+    const int32_t oi = dpp_move<Ctrl>(i);
+    if (beats(ov, oi, v, i)) {
+        v = ov;
+        i = oi;
     }
 }
 
-// The lane holding index `i` (indices are unique within a reduction).
+// Arg-max over each row of sixteen lanes; every lane of the row gets it.
+__device__ inline void row_argmax(double& v, int32_t& i) {
+    argmax_step<kDppQuadSwap1>(v, i);
+    argmax_step<kDppQuadSwap2>(v, i);
+    argmax_step<kDppRowHalfMirror>(v, i);
+    argmax_step<kDppRowMirror>(v, i);
+}
+
+// Arg-max over the wave: rows by DPP moves, then the four row winners read
+// as scalars. Every lane ends with the winner.
+__device__ inline void wave_argmax(double& v, int32_t& i) {
+    row_argmax(v, i);
+    double bv = read_lane(v, 0);
+    int32_t bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        const double ov = read_lane(v, 16 * r);
+        const int32_t oi = __builtin_amdgcn_readlane(i, 16 * r);
+        if (beats(ov, oi, bv, bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    v = bv;
+    i = bi;
+}
+
+// The lane holding index `i` (indices are unique within a reduction); the
+// same for every lane.
 __device__ inline int owner_lane(int32_t mine, int32_t i) {
     return __ffsll(static_cast<unsigned long long>(__ballot(mine == i))) - 1;
 }
 
-__device__ inline d3 shfl_d3(d3 p, int lane) {
-    return d3{__shfl(p.x, lane, 64), __shfl(p.y, lane, 64), __shfl(p.z, lane, 64)};
+// p of lane `lane` (the same lane for every lane of the wave).
+__device__ inline d3 lane_d3(d3 p, int lane) {
+    return d3{read_lane(p.x, lane), read_lane(p.y, lane), read_lane(p.z, lane)};
 }
 
 // Builds the blocked layout: records (x, y, z, running distance to the first
@@ -190,7 +218,7 @@ sample_init_kernel(const double* __restrict__ p, int64_t n, const int32_t* __res
     }
     const int32_t own = vi;
     wave_argmax(v, vi);
-    vp = shfl_d3(vp, owner_lane(own, vi));
+    vp = lane_d3(vp, owner_lane(own, vi));
 #pragma unroll
     for (int offset = 32; offset > 0; offset >>= 1) {
         lo = d3{fmin(lo.x, __shfl_xor(lo.x, offset, 64)), fmin(lo.y, __shfl_xor(lo.y, offset, 64)),
@@ -212,42 +240,54 @@ sample_init_kernel(const double* __restrict__ p, int64_t n, const int32_t* __res
     }
 }
 
-// Sweeps one block for seed c: U records per lane per round, `rounds`
-// rounds. Returns the block's new maximum (every lane holds it).
-template <int U>
-__device__ inline void sample_sweep(double4* __restrict__ rec, const int32_t* __restrict__ ids, int64_t base,
-                                    int rounds, d3 c, int lane, double& v, int32_t& vi, d3& vp) {
-    v = -1.0;
-    vi = 0x7fffffff;
-    vp = d3{0.0, 0.0, 0.0};
+// Sweeps NB blocks (bases base[0..NB)) for seed c together, so their loads
+// are in flight at once: U records per lane per block per round, `rounds`
+// rounds. Returns each block's new maximum (every lane holds them).
+template <int U, int NB>
+__device__ inline void sample_sweep(double4* __restrict__ rec, const int32_t* __restrict__ ids,
+                                    const int64_t (&base)[NB], int rounds, d3 c, int lane, double (&v)[NB],
+                                    int32_t (&vi)[NB], d3 (&vp)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        v[b] = -1.0;
+        vi[b] = 0x7fffffff;
+        vp[b] = d3{0.0, 0.0, 0.0};
+    }
     for (int r = 0; r < rounds; ++r) {
-        double4 q[U];
-        int32_t id[U];
+        double4 q[NB][U];
+        int32_t id[NB][U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t j = base + static_cast<int64_t>(r * U + u) * 64 + lane;
-            q[u] = rec[j];
-            id[u] = ids[j];
-        }
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (q[u].w >= 0.0) {
-                const double d = norm2(d3{q[u].x, q[u].y, q[u].z} - c);
-                if (d < q[u].w) {
-                    q[u].w = d;
-                    reinterpret_cast<double*>(rec + base + static_cast<int64_t>(r * U + u) * 64 + lane)[3] = d;
-                }
-                if (beats(q[u].w, id[u], v, vi)) {
-                    v = q[u].w;
-                    vi = id[u];
-                    vp = d3{q[u].x, q[u].y, q[u].z};
+            for (int u = 0; u < U; ++u) {
+                const int64_t j = base[b] + static_cast<int64_t>(r * U + u) * 64 + lane;
+                q[b][u] = rec[j];
+                id[b][u] = ids[j];
+            }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (q[b][u].w >= 0.0) {
+                    const double d = norm2(d3{q[b][u].x, q[b][u].y, q[b][u].z} - c);
+                    if (d < q[b][u].w) {
+                        q[b][u].w = d;
+                        reinterpret_cast<double*>(rec + base[b] + static_cast<int64_t>(r * U + u) * 64 + lane)[3] = d;
+                    }
+                    if (beats(q[b][u].w, id[b][u], v[b], vi[b])) {
+                        v[b] = q[b][u].w;
+                        vi[b] = id[b][u];
+                        vp[b] = d3{q[b][u].x, q[b][u].y, q[b][u].z};
+                    }
                 }
             }
-        }
     }
-    const int32_t own = vi;
-    wave_argmax(v, vi);
-    vp = shfl_d3(vp, owner_lane(own, vi));
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int32_t own = vi[b];
+        wave_argmax(v[b], vi[b]);
+        vp[b] = lane_d3(vp[b], owner_lane(own, vi[b]));
+    }
 }
 
 // Seeds [begin, end) in one workgroup; the running state (records and block
@@ -285,7 +325,7 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
         double v = bv;
         int32_t vi = bi;
         wave_argmax(v, vi);
-        const d3 vp = shfl_d3(bp, owner_lane(bi, vi));
+        const d3 vp = lane_d3(bp, owner_lane(bi, vi));
         if (lane == 0) {
             slot_v[buf][wave] = v;
             slot_i[buf][wave] = vi;
@@ -294,18 +334,11 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
             slot_p[buf][wave][2] = vp.z;
         }
         __syncthreads();
+        // Each row of sixteen lanes holds the sixteen wave winners.
         double sv = slot_v[buf][lane % kSampleWaves];
         int32_t si = slot_i[buf][lane % kSampleWaves];
         const int32_t mine = si;
-#pragma unroll
-        for (int offset = kSampleWaves / 2; offset > 0; offset >>= 1) {
-            const double ov = __shfl_xor(sv, offset, 64);
-            const int32_t oi = __shfl_xor(si, offset, 64);
-            if (beats(ov, oi, sv, si)) {
-                sv = ov;
-                si = oi;
-            }
-        }
+        row_argmax(sv, si);
         const int from = owner_lane(mine, si) % kSampleWaves;
         const d3 c{slot_p[buf][from][0], slot_p[buf][from][1], slot_p[buf][from][2]};
         if (threadIdx.x == 0) out[s] = si;
@@ -316,18 +349,41 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
         const double dz = fmax(fmax(lo.z - c.z, c.z - hi.z), 0.0);
         const double reach = dx * dx + dy * dy + dz * dz;
         unsigned long long todo = __ballot(owned && reach * kSampleMargin < bv);
+        // Two blocks at a time while at least two remain (where the registers
+        // allow: up to two records per lane per block).
         while (todo) {
-            const int l = __ffsll(todo) - 1;
+            const int l0 = __ffsll(todo) - 1;
             todo &= todo - 1;
-            const int64_t base = static_cast<int64_t>(l * kSampleWaves + wave) * points_per_block;
-            double nv;
-            int32_t ni;
-            d3 np;
-            sample_sweep<U>(rec, ids, base, rounds, c, lane, nv, ni, np);
-            if (lane == l) {
-                bv = nv;
-                bi = ni;
-                bp = np;
+            if (U <= 2 && todo) {
+                const int l1 = __ffsll(todo) - 1;
+                todo &= todo - 1;
+                const int64_t base[2] = {static_cast<int64_t>(l0 * kSampleWaves + wave) * points_per_block,
+                                         static_cast<int64_t>(l1 * kSampleWaves + wave) * points_per_block};
+                double nv[2];
+                int32_t ni[2];
+                d3 np[2];
+                sample_sweep<U, 2>(rec, ids, base, rounds, c, lane, nv, ni, np);
+                if (lane == l0) {
+                    bv = nv[0];
+                    bi = ni[0];
+                    bp = np[0];
+                }
+                if (lane == l1) {
+                    bv = nv[1];
+                    bi = ni[1];
+                    bp = np[1];
+                }
+            } else {
+                const int64_t base[1] = {static_cast<int64_t>(l0 * kSampleWaves + wave) * points_per_block};
+                double nv[1];
+                int32_t ni[1];
+                d3 np[1];
+                sample_sweep<U, 1>(rec, ids, base, rounds, c, lane, nv, ni, np);
+                if (lane == l0) {
+                    bv = nv[0];
+                    bi = ni[0];
+                    bp = np[0];
+                }
             }
         }
     }
