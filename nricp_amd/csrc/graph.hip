@@ -716,36 +716,51 @@ void retract(const double* R, const double* T, const double* delta, int64_t nn, 
     NRICP_LAUNCH_CHECK("retract_kernel");
 }
 
+namespace {
+// Points per lane per block, so that at most kSampleBlocks blocks cover the
+// points; beyond 4 per lane, whole rounds of 4.
+int sample_ppl(int64_t n) {
+    int ppl = static_cast<int>((n + 64 * kSampleBlocks - 1) / (64 * kSampleBlocks));
+    if (ppl > 4) ppl = (ppl + 3) / 4 * 4;
+    return ppl;
+}
+}  // namespace
+
+void farthest_point_prepare(const double* points, int64_t n, int32_t first, int32_t* out, FpsWork& w,
+                            hipStream_t stream) {
+    if (n <= 0) return;
+    if (first < 0 || first >= n) throw InvalidArgument("farthest_point_seeds: the first seed is out of range");
+    if (n >= (1ll << 31) - 1) throw InvalidArgument("farthest_point_seeds: too many points");
+    const int ppl = sample_ppl(n);
+    const int64_t per_block = static_cast<int64_t>(64) * ppl;
+    const int blocks = static_cast<int>((n + per_block - 1) / per_block);
+    w.order.resize(n);
+    morton_order(points, n, w.order.get(), w.sort, w.keys, w.bounds, stream);
+    w.records.resize(static_cast<int64_t>(blocks) * per_block);
+    w.ids.resize(static_cast<int64_t>(blocks) * per_block);
+    w.box.resize(6 * static_cast<int64_t>(blocks));
+    w.best.resize(blocks);
+    w.best_index.resize(blocks);
+    w.ppl = ppl;
+    sample_init_kernel<<<blocks, 64, 0, stream>>>(points, n, w.order.get(), first, ppl, w.records.get(), w.ids.get(),
+                                                  w.box.get(), w.best.get(), w.best_index.get(), out);
+    NRICP_LAUNCH_CHECK("sample_init_kernel");
+}
+
 void farthest_point_seeds_range(const double* points, int64_t n, int32_t first, int32_t begin, int32_t end,
                                 int32_t* out, FpsWork& w, hipStream_t stream) {
     if (n <= 0 || end <= begin) return;
-    if (first < 0 || first >= n) throw InvalidArgument("farthest_point_seeds: the first seed is out of range");
-    if (n >= (1ll << 31) - 1) throw InvalidArgument("farthest_point_seeds: too many points");
     if (end > n) throw InvalidArgument("farthest_point_seeds: more seeds than points");
-    // ppl points per lane per block, so that at most kSampleBlocks blocks
-    // cover the points; beyond 4 per lane, whole rounds of 4.
-    int ppl = static_cast<int>((n + 64 * kSampleBlocks - 1) / (64 * kSampleBlocks));
-    if (ppl > 4) ppl = (ppl + 3) / 4 * 4;
-    const int64_t per_block = static_cast<int64_t>(64) * ppl;
-    const int blocks = static_cast<int>((n + per_block - 1) / per_block);
     if (begin == 0) {
-        w.order.resize(n);
-        morton_order(points, n, w.order.get(), w.sort, w.keys, w.bounds, stream);
-        w.records.resize(static_cast<int64_t>(blocks) * per_block);
-        w.ids.resize(static_cast<int64_t>(blocks) * per_block);
-        w.box.resize(6 * static_cast<int64_t>(blocks));
-        w.best.resize(blocks);
-        w.best_index.resize(blocks);
-        w.ppl = ppl;
-        sample_init_kernel<<<blocks, 64, 0, stream>>>(points, n, w.order.get(), first, ppl, w.records.get(),
-                                                      w.ids.get(), w.box.get(), w.best.get(), w.best_index.get(),
-                                                      out);
-        NRICP_LAUNCH_CHECK("sample_init_kernel");
+        farthest_point_prepare(points, n, first, out, w, stream);
         begin = 1;
         if (end <= begin) return;
-    } else if (w.ppl != ppl || static_cast<int64_t>(w.best.size()) != blocks) {
-        throw InvalidArgument("farthest_point_seeds: a range resumed with different points");
     }
+    const int ppl = sample_ppl(n);
+    const int64_t per_block = static_cast<int64_t>(64) * ppl;
+    const int blocks = static_cast<int>((n + per_block - 1) / per_block);
+    if (w.ppl != ppl || static_cast<int64_t>(w.best.size()) != blocks)
+        throw InvalidArgument("farthest_point_seeds: a range resumed with different points");
     const int unroll = ppl < 4 ? ppl : 4;
     const int rounds = ppl / unroll;
     switch (unroll) {

@@ -163,6 +163,11 @@ void farthest_point_seeds(const double* points, int64_t count, int32_t seeds, in
                           FpsWork& work, hipStream_t stream);
 void farthest_point_seeds_range(const double* points, int64_t count, int32_t first, int32_t begin, int32_t end,
                                 int32_t* out, FpsWork& work, hipStream_t stream);
+// The part of the range [0, ...) before the first step: the Morton order
+// (work.order), the blocked records and seed 0. A range starting at 1 then
+// continues from it.
+void farthest_point_prepare(const double* points, int64_t count, int32_t first, int32_t* out, FpsWork& work,
+                            hipStream_t stream);
 // Uniform grid over a node set (cells sorted by key, per-cell runs) for exact
 // nearest-node searches: bind_vertices, build_node_edges and transfer_level
 // take the grid of the nodes they search.

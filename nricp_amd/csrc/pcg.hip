@@ -91,8 +91,12 @@ __device__ inline bool spd_inverse6(const double (&a)[36], double jitter, double
 // the system keeps its damping.
 template <typename S>
 __global__ void precondition_kernel(const S* __restrict__ bsr, const int32_t* __restrict__ diag_pos, int64_t nn,
-                                    double lambda, int scaling, S* __restrict__ damp, S* __restrict__ minv) {
+                                    double lambda, int scaling, S* __restrict__ damp, S* __restrict__ minv,
+                                    unsigned* __restrict__ barrier) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // The solve's grid-barrier counter starts at zero (the solve runs next on
+    // the same stream).
+    if (i == 0) barrier[0] = barrier[1] = 0u;
     if (i >= nn) return;
     const S* blk = bsr + 36 * static_cast<int64_t>(diag_pos[i]);
     double a[36];
@@ -530,8 +534,9 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
                 double tol, double inner_tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp, S* minv,
                 DeviceBuffer<S>& vbuf, hipStream_t stream) {
     if (nn <= 0) return;
+    ws.barrier.resize(2);
     precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, L.diag_position.get(), nn, lambda,
-                                                                     scaling ? 1 : 0, damp, minv);
+                                                                     scaling ? 1 : 0, damp, minv, ws.barrier.get());
     NRICP_LAUNCH_CHECK("precondition_kernel");
     constexpr int kmax = 2;
     const int64_t passes = (nn + kNodesPerPass - 1) / kNodesPerPass;
@@ -563,8 +568,6 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
                         : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
     }
     ws.reduce.resize(6 * static_cast<int64_t>(groups));
-    ws.barrier.resize(2);
-    ws.barrier.zero(stream);
     ws.outcome.resize(4);
     ws.xr.resize(6 * nn);
     vbuf.resize(12 * nn);

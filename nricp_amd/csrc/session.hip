@@ -31,6 +31,19 @@ __global__ void gather_rows3(const double* __restrict__ p, const int32_t* __rest
     out[3 * i + 2] = p[3 * j + 2];
 }
 
+// Up to three small device arrays, back to back, into pinned host memory
+// (written by the kernel over the bus; one launch instead of a copy each).
+__global__ void gather_to_host(const double* __restrict__ a, int na, const double* __restrict__ b, int nb,
+                               const double* __restrict__ c, int nc, double* __restrict__ out) {
+    const int i = threadIdx.x;
+    if (i < na)
+        out[i] = a[i];
+    else if (i < na + nb)
+        out[i] = b[i - na];
+    else if (i < na + nb + nc)
+        out[i] = c[i - na - nb];
+}
+
 __global__ void identity_state(double* __restrict__ R, double* __restrict__ T, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -245,6 +258,7 @@ KernelClock::~KernelClock() {
 
 Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const DeviceGraph* graph) : settings_(s) {
     const auto t0 = std::chrono::steady_clock::now();
+    NRICP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&staging_device_), staging_.get(), 0));
     if (s.level_settings_count > 0 && s.levels == nullptr) throw InvalidArgument("level settings pointer is null");
     level_settings_.assign(s.levels, s.levels + std::max(0, s.level_settings_count));
     settings_.levels = nullptr;
@@ -304,6 +318,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         seeds_.resize(nmax);
         fps_ready_.resize(lv);
         for (auto& e : fps_ready_) NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        NRICP_CHECK(hipEventCreateWithFlags(&order_ready_, hipEventDisableTiming));
     }
     if (s.max_nodes_per_vertex < 1) throw InvalidArgument("max_nodes_per_vertex must be at least 1");
     if (s.graph_neighbours < 0 || s.graph_neighbours > 16) throw InvalidArgument("graph_neighbours must be in [0, 16]");
@@ -331,6 +346,7 @@ Session::~Session() {
     (void)hipStreamSynchronize(fps_stream_.get());
     (void)hipStreamSynchronize(stream_.get());
     for (hipEvent_t e : fps_ready_) (void)hipEventDestroy(e);
+    if (order_ready_) (void)hipEventDestroy(order_ready_);
 }
 
 KernelClock& Session::clock() {
@@ -374,6 +390,13 @@ AssemblyInputs Session::inputs(const double* R, const double* T) const {
     return in;
 }
 
+void Session::to_host(hipStream_t st, int at, const double* a, int na, const double* b, int nb, const double* c,
+                      int nc) {
+    if (na + nb + nc > 64 || at + na + nb + nc > kStaging) throw InvalidArgument("staging overflow");
+    gather_to_host<<<1, 64, 0, st>>>(a, na, b, nb, c, nc, staging_device_ + at);
+    NRICP_LAUNCH_CHECK("gather_to_host");
+}
+
 void Session::sync() {
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     clock_.collect(false);
@@ -397,7 +420,11 @@ void Session::prepare() {
         // levels' solves.
         const int lv = static_cast<int>(own_graph_.levels.size());
         clock_.begin("fps", fps_stream_.get());
-        int32_t done = 0;
+        // First the template's Morton order (which the queries share) and
+        // seed 0, marked by an event of its own.
+        farthest_point_prepare(points_.get(), nv_, 0, seeds_.get(), fps_work_, fps_stream_.get());
+        NRICP_CHECK(hipEventRecord(order_ready_, fps_stream_.get()));
+        int32_t done = 1;
         for (int l = 0; l < lv; ++l) {
             const int32_t want = static_cast<int32_t>(level_count_for(lv, settings_.nodes_initial, settings_.growth,
                                                                       nv_, l));
@@ -420,6 +447,19 @@ void Session::prepare() {
     if (shared_graph_)
         morton_order(points_.get(), nv_, query_order_.get(), sort_, order_keys_, order_bounds_, st);
     clock_.end(st);
+    // While the first level's seeds are still being drawn, a cold pass of
+    // the rest template against the target: the first level's template is
+    // the rest template up to rounding, so the triangles found here start
+    // its first pass warm. Only the search is shortened; the warm start
+    // never changes a result.
+    if (!shared_graph_) NRICP_CHECK(hipStreamWaitEvent(st, order_ready_, 0));
+    clock_.begin("closest_points", st);
+    const QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, 0, 0.0};
+    closest_points(bvh_, points_.get(), nullptr, has_mask_ ? mask_.get() : nullptr, nv_, q, corr_points_.get(),
+                   corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(), nullptr, st, nullptr,
+                   query_order());
+    clock_.end(st);
+    have_warm_ = true;
     level_ = 0;
 }
 
@@ -587,9 +627,7 @@ bool Session::inner_step() {
         clock_.begin("energy", st);
         energies(inputs(rot_cand_.get(), trans_cand_.get()), assembly_, energy_cand_.get(), st);
         clock_.end(st);
-        NRICP_CHECK(hipMemcpyAsync(staging_.get(), energy_cur_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
-        NRICP_CHECK(hipMemcpyAsync(staging_.get() + 5, energy_cand_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
-        NRICP_CHECK(hipMemcpyAsync(staging_.get() + 10, pcg_.outcome.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        to_host(st, 0, energy_cur_.get(), 5, energy_cand_.get(), 5, pcg_.outcome.get(), 4);
         sync();
         const double* h = staging_.get();
         cur = (((h[0] + h[1]) + h[2]) + h[3]) + h[4];
@@ -666,7 +704,7 @@ bool Session::end_level() {
     hipStream_t st = stream_.get();
     const nricp_hip_level_settings& ls = level_settings();
     summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
-    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    to_host(st, 16, summary_.get(), 6);
     sync();
     const double accepted = staging_[16];
     corr_mean_ = accepted > 0.0 ? staging_[21] / accepted : 0.0;
@@ -700,11 +738,11 @@ void Session::final_statistics() {
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),
                    nullptr, st, have_warm_ ? corr_tri_.get() : nullptr, query_order());
     summarize_correspondences(rejection_.get(), corr_dist_.get(), nv_, summary_partials_, summary_.get(), st);
-    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 16, summary_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    to_host(st, 16, summary_.get(), 6);
     sync();
     const int64_t accepted = static_cast<int64_t>(staging_[16]);
     distance_quantiles(rejection_.get(), corr_dist_.get(), nv_, quantile_keys_, sort_, quantiles_.get(), accepted, st);
-    NRICP_CHECK(hipMemcpyAsync(staging_.get() + 24, quantiles_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    to_host(st, 24, quantiles_.get(), 2);
     sync();
     report_.counts = {nv_, accepted, static_cast<int64_t>(staging_[17]), static_cast<int64_t>(staging_[18]),
                       static_cast<int64_t>(staging_[19]), static_cast<int64_t>(staging_[20])};

@@ -148,6 +148,7 @@ private:
     FpsWork fps_work_;
     DeviceBuffer<int32_t> seeds_;          // finest level's seeds; coarser levels use prefixes
     std::vector<hipEvent_t> fps_ready_;    // per level: its seeds are written
+    hipEvent_t order_ready_ = nullptr;     // the sampler's Morton order is written
     SortScratch sort_;
     int64_t nv_ = 0, nf_ = 0;
     DeviceBuffer<double> points_;
@@ -184,7 +185,13 @@ private:
     DeviceBuffer<double> rhs_;
     DeviceBuffer<double> energy_cur_, energy_cand_, energy_fit_, summary_, summary_partials_, quantiles_;
     DeviceBuffer<uint64_t> quantile_keys_;
-    PinnedBuffer<double> staging_{32};
+    static constexpr int kStaging = 32;
+    PinnedBuffer<double> staging_{kStaging};
+    double* staging_device_ = nullptr;  // staging_ as the device addresses it
+    // Writes a (na values), then b, then c into staging_[at ...) on the
+    // stream; read them after sync().
+    void to_host(hipStream_t st, int at, const double* a, int na, const double* b = nullptr, int nb = 0,
+                 const double* c = nullptr, int nc = 0);
     // schedule
     Phase phase_ = Phase::Prepare;
     size_t level_ = 0;
