@@ -18,12 +18,15 @@ FP64 right-hand side, energies and refinement; the headline `value`) and
 residual, so both stay on the FP64 oracle's trajectory.
 
 CPU baseline: the FP64 oracle (a single-threaded restatement of the reference's
-route A) on this host. For configs 0 and 1 it runs the whole wrap, and the
-device forms' vertex error against it is reported. For config 2 a whole wrap
-takes minutes, so it runs a bounded sample -- the same job with one outer
-iteration per level -- and the wrap time is extrapolated as setup + 10 x the
-sample's solve time (the per-iteration cost is flat within a level); the
-device forms are then compared with each other.
+route A) on this host, run before the GPU is touched. For configs 0 and 1 it
+runs the whole wrap, and the device forms' vertex error against it is
+reported. For config 2 a whole wrap takes minutes, so it runs a bounded sample
+-- the same job with one outer iteration per level -- and the wrap time is
+extrapolated as setup + 10 x the sample's solve time (the per-iteration cost is
+flat within a level); the device forms are then compared with each other. The
+sample runs once on one core (latency) and then as independent copies on all
+usable cores at once, up to 16 (throughput: `cpu_baseline.value`, with the
+core count in `cores`).
 
 The timed wraps run with profiling off. Two more wraps per form then run with
 HIP-event stage timing and traversal counters on; the stage breakdown, the
@@ -166,7 +169,8 @@ def query_stats(form: dict) -> dict:
     }
 
 
-def cpu_baseline(config: int, job, settings, kwargs) -> dict:
+def _oracle_sample(config: int, job, settings, kwargs) -> dict:
+    """One oracle run of the CPU baseline's sample (in this process)."""
     import nricp_oracle
 
     full = config in (0, 1)
@@ -177,26 +181,61 @@ def cpu_baseline(config: int, job, settings, kwargs) -> dict:
                                         **kwargs)
     wall = time.perf_counter() - t0
     report = json.loads(result["report"])
-    if full:
-        seconds = wall
-        sample = (f"one complete wrap of {job.name} ({job.graph_levels} levels x {OUTER} outer iterations), FP64 "
-                  f"route-A restatement, single thread: {wall:.1f} s")
-    else:
-        seconds = report["setup_seconds"] + OUTER * report["solve_seconds"]
-        sample = (f"{job.name} with 1 outer iteration per level (of {OUTER}), FP64 route-A restatement, single "
-                  f"thread: {wall:.1f} s measured (setup {report['setup_seconds']:.1f} s + solve "
-                  f"{report['solve_seconds']:.1f} s); wrap time extrapolated as setup + {OUTER} x solve = "
-                  f"{seconds:.1f} s")
-    log(f"cpu oracle: {sample}")
+    seconds = wall if full else report["setup_seconds"] + OUTER * report["solve_seconds"]
+    return {"wall": wall, "seconds": seconds, "setup": report["setup_seconds"], "solve": report["solve_seconds"],
+            "vertices": np.asarray(result["vertices"]) if full else None}
+
+
+def _host_description() -> str:
+    model = "unknown CPU"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        smt = "SMT on" if Path("/sys/devices/system/cpu/smt/active").read_text().strip() == "1" else "SMT off"
+    except OSError:
+        smt = "SMT unknown"
+    return f"{model}, {smt}"
+
+
+def cpu_baseline(config: int, job, settings, kwargs, procs: int) -> dict:
+    """The FP64 oracle on this host: one job on one core (the accuracy
+    reference and the latency), then `procs` independent copies at once, one
+    per core, for the all-core throughput. Runs before the GPU is touched, so
+    the worker processes start from a process with no device state."""
+    full = config in (0, 1)
+    one = _oracle_sample(config, job, settings, kwargs)
+    what = (f"one complete wrap of {job.name} ({job.graph_levels} levels x {OUTER} outer iterations)" if full else
+            f"{job.name} with 1 outer iteration per level (of {OUTER}), wrap time extrapolated as setup + {OUTER} x "
+            f"solve")
+    log(f"cpu oracle, 1 core: {one['wall']:.1f} s measured, {one['seconds']:.1f} s per wrap")
+    per_core = [one["seconds"]]
+    if procs > 1:
+        import multiprocessing as mp
+
+        ctx = mp.get_context("spawn")
+        t0 = time.perf_counter()
+        with ctx.Pool(procs) as pool:
+            runs = pool.starmap(_oracle_sample, [(config, job, settings, kwargs)] * procs)
+        log(f"cpu oracle, {procs} processes at once: {time.perf_counter() - t0:.1f} s wall")
+        per_core = [r["seconds"] for r in runs]
+    throughput = float(sum(1.0 / t for t in per_core))
+    sample = (f"FP64 route-A restatement (single-threaded per job) on {_host_description()}: {what}; 1 core "
+              f"{one['seconds']:.1f} s per wrap (measured {one['wall']:.1f} s); {len(per_core)} independent jobs on "
+              f"{len(per_core)} cores at once: {throughput:.4f} wraps/s")
     return {
-        "value": 1.0 / seconds,
+        "value": throughput,
         "unit": "solves/s",
-        "cores": 1,
+        "cores": len(per_core),
         "kind": "port",
         "sample": sample,
-        "seconds_per_wrap": seconds,
-        "measured_seconds": wall,
-        "vertices": np.asarray(result["vertices"]) if full else None,
+        "one_core_solves_per_s": 1.0 / one["seconds"],
+        "seconds_per_wrap_one_core": one["seconds"],
+        "vertices": one["vertices"],
     }
 
 
@@ -235,6 +274,9 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--forms", default="mixed,double")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="oracle jobs run at once for the all-core CPU throughput (default: the usable cores, at "
+                         "most 16)")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device three times: a plain copy of known size, against "
@@ -247,6 +289,18 @@ def main(argv=None) -> int:
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
+    t0 = time.perf_counter()
+    job, settings, kwargs = build_job(args.config, args.seed)
+    log(f"job {job.name}: template {len(job.template_points)} v / {len(job.template_faces)} f, target "
+        f"{len(job.target_points)} v / {len(job.target_faces)} f, generated in {time.perf_counter() - t0:.1f} s")
+
+    # The CPU baseline first: its worker processes start before this process
+    # touches the GPU.
+    cpu = None
+    if not args.no_cpu and rank == 0:
+        procs = args.cpu_procs if args.cpu_procs > 0 else min(len(os.sched_getaffinity(0)), 16)
+        cpu = cpu_baseline(args.config, job, settings, kwargs, procs)
+
     import nricp_amd
 
     dp = nricp_amd.load()
@@ -257,16 +311,10 @@ def main(argv=None) -> int:
     log(f"device: {info['device']}")
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
-    t0 = time.perf_counter()
-    job, settings, kwargs = build_job(args.config, args.seed)
-    log(f"job {job.name}: template {len(job.template_points)} v / {len(job.template_faces)} f, target "
-        f"{len(job.target_points)} v / {len(job.target_faces)} f, generated in {time.perf_counter() - t0:.1f} s")
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     results = {f: run_form(dp, job, settings, kwargs, f, args.steps, args.warmup) for f in forms}
     head = results[forms[0]]
-
-    cpu = None if args.no_cpu or rank != 0 else cpu_baseline(args.config, job, settings, kwargs)
     diag = float(np.linalg.norm(job.template_points.max(0) - job.template_points.min(0)))
 
     def error(a, b):
@@ -326,7 +374,8 @@ def main(argv=None) -> int:
     if "double" in results and "mixed" in results:
         line["mixed_over_double"] = round(results["mixed"]["solves_per_s"] / results["double"]["solves_per_s"], 3)
     if cpu is not None:
-        line["gpu_over_cpu_1core"] = round(head["solves_per_s"] / cpu["value"], 1)
+        line["gpu_over_cpu_all_cores"] = round(head["solves_per_s"] / cpu["value"], 1)
+        line["gpu_over_cpu_1core"] = round(head["solves_per_s"] / cpu["one_core_solves_per_s"], 1)
     if rank == 0:
         text = json.dumps(line)
         print(text, flush=True)
