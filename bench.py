@@ -57,6 +57,12 @@ HBM_PEAK_GBS = 8000.0
 TOLERANCE = 1e-12
 MAX_PCG = 5000
 OUTER = 10
+# HBM bytes per pcg_kernel launch (FETCH_SIZE + WRITE_SIZE per dispatch, each
+# scaled by what the counter reads for a plain device copy of known size),
+# averaged over a wrap's launches: measured on the 200k-vertex job by the
+# counter passes recorded in profiles/pcg_kernel_traffic_geodesic-200k.md.
+# Other jobs report null.
+PCG_TRAFFIC = {(2, "mixed"): 4.322e7, (2, "double"): 6.790e7}
 # Device-timed stages of a wrap (HIP events around each stage's launches).
 STAGES = ("fps", "graph", "target_bvh", "layout", "deform", "closest_points", "assemble", "pcg", "energy",
           "local_step")
@@ -133,7 +139,7 @@ def run_form(dp, job, settings, kwargs, form: str, steps: int, warmup: int, prof
     }
 
 
-def hbm_roofline(form: dict, stage: str, bytes_key: str, kernel: str) -> dict:
+def hbm_roofline(form: dict, stage: str, bytes_key: str, kernel: str, traffic=None) -> dict:
     """Algorithmic bytes of one launch over its average device time, both
     from the profiled wraps (per wrap)."""
     st = form["stages"][stage]
@@ -148,7 +154,7 @@ def hbm_roofline(form: dict, stage: str, bytes_key: str, kernel: str) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic,
         "avg_launch_us": round(1e6 * seconds / launches, 2),
         "algorithmic_bytes_per_launch": round(bytes_per_launch),
     }
@@ -332,7 +338,7 @@ def main(argv=None) -> int:
             "pcg_iterations_per_solve": round(r["counters"]["pcg_iterations"] / max(r["counters"]["pcg_launches"], 1),
                                               2),
             "stage_ms_per_profiled_wrap": {k: round(v["ms"], 3) for k, v in r["stages"].items()},
-            "roofline_pcg": hbm_roofline(r, "pcg", "pcg_bytes", "pcg_kernel"),
+            "roofline_pcg": hbm_roofline(r, "pcg", "pcg_bytes", "pcg_kernel", PCG_TRAFFIC.get((args.config, name))),
             "roofline_assembly": hbm_roofline(r, "assemble", "assemble_bytes", "assembly (rows, node and pair blocks)"),
             "query": query_stats(r),
         }
@@ -366,7 +372,7 @@ def main(argv=None) -> int:
             "form": forms[0],
             "pcg_tolerance": TOLERANCE,
         },
-        "roofline": hbm_roofline(head, "pcg", "pcg_bytes", "pcg_kernel"),
+        "roofline": hbm_roofline(head, "pcg", "pcg_bytes", "pcg_kernel", PCG_TRAFFIC.get((args.config, forms[0]))),
         "cpu_baseline": None if cpu is None else {k: (round(v, 6) if isinstance(v, float) else v)
                                                  for k, v in cpu.items() if k != "vertices"},
         "forms": forms_out,
