@@ -302,6 +302,50 @@ int nricp_hip_session_kernel_time(const nricp_hip_session* session, const char* 
                                   int64_t* launches);
 void nricp_hip_session_destroy(nricp_hip_session* session);
 
+/* -------------------------------------------------------------- server ---- */
+
+/* A request loop over wrap jobs (no reference counterpart: the reference
+ * serves one job per RouteBSession, src/python/bindings.cpp:1510-1612). Jobs
+ * are admitted to the device in arrival order while one of `max_active`
+ * slots is free and their estimated device memory fits the budget; each
+ * admitted job runs as its own resident session on its own streams, so the
+ * jobs' work overlaps on the device, and a job's result does not depend on
+ * what else runs. `device_budget_bytes` <= 0 takes 90 % of the device memory
+ * free at creation. Submission copies the inputs and returns at once. */
+typedef struct nricp_hip_server nricp_hip_server;
+typedef struct nricp_hip_server_config {
+    int32_t max_active;
+    double device_budget_bytes;
+} nricp_hip_server_config;
+typedef struct nricp_hip_server_stats {
+    int64_t queued, active, finished;
+    double budget_bytes, reserved_bytes; /* reserved: estimates of the active jobs */
+} nricp_hip_server_stats;
+typedef struct nricp_hip_job_info {
+    double queued_seconds;  /* submission to admission */
+    double run_seconds;     /* admission to completion (session set-up, wrap, readback) */
+    double device_bytes;    /* device memory the job's session held */
+    double estimated_bytes; /* what admission reserved */
+} nricp_hip_job_info;
+int nricp_hip_server_create(const nricp_hip_server_config* config, nricp_hip_server** out_server);
+/* Waits for the active jobs; queued jobs are dropped. */
+void nricp_hip_server_destroy(nricp_hip_server* server);
+/* Device bytes admission reserves for a job. */
+int nricp_hip_server_estimate(const nricp_hip_problem* problem, const nricp_hip_settings* settings,
+                              double* out_bytes);
+int nricp_hip_server_submit(nricp_hip_server* server, const nricp_hip_problem* problem,
+                            const nricp_hip_settings* settings, int64_t* out_ticket);
+/* Waits up to `timeout_seconds` (negative: without limit); *out_done is 1 when the job has finished. */
+int nricp_hip_server_wait(nricp_hip_server* server, int64_t ticket, double timeout_seconds, int32_t* out_done);
+/* A finished job's fitted template (V x 3, normalized frame) and run report (as
+ * nricp_hip_session_report). Returns the job's own status (its error through
+ * nricp_hip_last_error) when the job failed. */
+int nricp_hip_server_result(nricp_hip_server* server, int64_t ticket, double* out_vertices, char* report,
+                            size_t report_capacity, size_t* out_report_length, nricp_hip_job_info* out_info);
+/* Forgets a finished job. */
+int nricp_hip_server_release(nricp_hip_server* server, int64_t ticket);
+int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats* out_stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,16 @@
 
 namespace nricp_hip {
 
+// Device memory goes through a process-wide cache: a released block is kept
+// for the next request of its size class instead of being returned with
+// hipFree, which waits for the whole device and would stall every other job
+// on the GPU. Blocks are never returned before the process ends; the bytes
+// held by the calling thread's live buffers are counted per thread (a
+// worker drives one job at a time, so that is the job's footprint).
+void* device_allocate(int64_t bytes, int64_t* granted, int* device);
+void device_release(void* p, int64_t bytes, int device);
+int64_t thread_device_bytes();
+
 template <typename T>
 class DeviceBuffer {
 public:
@@ -19,9 +29,10 @@ public:
     ~DeviceBuffer() { release(); }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), size_(o.size_), cap_(o.cap_) {
+    DeviceBuffer(DeviceBuffer&& o) noexcept
+        : ptr_(o.ptr_), size_(o.size_), cap_(o.cap_), bytes_(o.bytes_), device_(o.device_) {
         o.ptr_ = nullptr;
-        o.size_ = o.cap_ = 0;
+        o.size_ = o.cap_ = o.bytes_ = 0;
     }
     DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
         if (this != &o) {
@@ -29,8 +40,10 @@ public:
             ptr_ = o.ptr_;
             size_ = o.size_;
             cap_ = o.cap_;
+            bytes_ = o.bytes_;
+            device_ = o.device_;
             o.ptr_ = nullptr;
-            o.size_ = o.cap_ = 0;
+            o.size_ = o.cap_ = o.bytes_ = 0;
         }
         return *this;
     }
@@ -40,15 +53,16 @@ public:
         if (n > cap_) {
             release();
             const int64_t bytes = (n > 0 ? n : 1) * static_cast<int64_t>(sizeof(T));
-            NRICP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr_), bytes));
+            ptr_ = static_cast<T*>(device_allocate(bytes, &bytes_, &device_));
             cap_ = n;
         }
         size_ = n;
     }
     void release() {
-        if (ptr_) (void)hipFree(ptr_);
+        if (ptr_) device_release(ptr_, bytes_, device_);
         ptr_ = nullptr;
         size_ = cap_ = 0;
+        bytes_ = 0;
     }
     void upload(const T* host, int64_t n, hipStream_t s) {
         resize(n);
@@ -75,6 +89,8 @@ private:
     T* ptr_ = nullptr;
     int64_t size_ = 0;
     int64_t cap_ = 0;
+    int64_t bytes_ = 0;  // granted by the cache
+    int device_ = 0;
 };
 
 // A stream owned by one object (default priority).

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "server.hpp"
 #include "session.hpp"
 
 using namespace nricp_hip;
@@ -740,3 +741,84 @@ int nricp_hip_session_kernel_time(const nricp_hip_session* s, const char* name, 
 void nricp_hip_session_destroy(nricp_hip_session* s) { delete s; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- server --
+
+struct nricp_hip_server {
+    std::unique_ptr<nricp_hip::Server> server;
+};
+
+int nricp_hip_server_create(const nricp_hip_server_config* config, nricp_hip_server** out) {
+    return guarded([&] {
+        require(config && out, "server_create: null argument");
+        ensure_device();
+        auto h = std::make_unique<nricp_hip_server>();
+        h->server = std::make_unique<Server>(config->max_active, config->device_budget_bytes);
+        *out = h.release();
+    });
+}
+
+void nricp_hip_server_destroy(nricp_hip_server* server) { delete server; }
+
+int nricp_hip_server_estimate(const nricp_hip_problem* problem, const nricp_hip_settings* settings, double* out) {
+    return guarded([&] {
+        require(problem && settings && out, "server_estimate: null argument");
+        *out = static_cast<double>(estimate_job_bytes(*problem, *settings));
+    });
+}
+
+int nricp_hip_server_submit(nricp_hip_server* server, const nricp_hip_problem* problem,
+                            const nricp_hip_settings* settings, int64_t* out) {
+    return guarded([&] {
+        require(server && problem && settings && out, "server_submit: null argument");
+        *out = server->server->submit(*problem, *settings);
+    });
+}
+
+int nricp_hip_server_wait(nricp_hip_server* server, int64_t ticket, double timeout, int32_t* done) {
+    return guarded([&] {
+        require(server && done, "server_wait: null argument");
+        *done = server->server->wait(ticket, timeout) ? 1 : 0;
+    });
+}
+
+int nricp_hip_server_result(nricp_hip_server* server, int64_t ticket, double* vertices, char* report,
+                            size_t capacity, size_t* length, nricp_hip_job_info* info) {
+    std::shared_ptr<const ServerJob> job;
+    const int st = guarded([&] {
+        require(server && length, "server_result: null argument");
+        job = server->server->job(ticket);
+        if (!job->done) throw InvalidArgument("server_result: the job has not finished");
+    });
+    if (st != 0) return st;
+    if (info) {
+        info->queued_seconds = job->started > 0.0 ? job->started - job->submitted : 0.0;
+        info->run_seconds = job->finished > 0.0 && job->started > 0.0 ? job->finished - job->started : 0.0;
+        info->device_bytes = static_cast<double>(job->device_bytes);
+        info->estimated_bytes = static_cast<double>(job->estimate);
+    }
+    if (job->status != 0) {
+        return guarded([&] {
+            if (job->status == 1) throw InvalidArgument(job->error);
+            throw DeviceError(job->error);
+        });
+    }
+    *length = job->report.size();
+    if (report && capacity > job->report.size()) std::memcpy(report, job->report.c_str(), job->report.size() + 1);
+    if (vertices) std::copy(job->vertices.begin(), job->vertices.end(), vertices);
+    return 0;
+}
+
+int nricp_hip_server_release(nricp_hip_server* server, int64_t ticket) {
+    return guarded([&] {
+        require(server, "server_release: null handle");
+        server->server->release(ticket);
+    });
+}
+
+int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats* out) {
+    return guarded([&] {
+        require(server && out, "server_stats: null argument");
+        *out = server->server->stats();
+    });
+}

@@ -149,18 +149,20 @@ __device__ inline void block_sum(const double (&v)[Count], double* lds /* Count 
 
 // ------------------------------------------------------ persistent grids ----
 //
-// The sampling (FPS) and PCG kernels are persistent grids whose groups meet at
-// grid barriers, so every group of a launch must be resident at once. They are
-// ordinary launches -- a cooperative launch would serialise them with the work
-// of other streams, and the sampling runs beside the solve on its own stream --
-// and residency is guaranteed by construction instead:
+// The PCG runs as a persistent grid whose groups meet at grid barriers, so
+// every group of a launch must be resident at once. It is an ordinary launch
+// -- a cooperative launch would serialise it with the work of other streams,
+// where the sampling and other jobs run -- and residency is guaranteed by
+// construction instead:
 //   * a launch never has more groups than the device holds at once for that
-//     kernel (occupancy query), and
-//   * each of the two persistent kernels uses at most half of the compute
-//     units, so both fit together, and
-//   * every other kernel runs to completion without waiting on anything, so
-//     the groups of a persistent launch that are not yet resident become
-//     resident once the ordinary kernels ahead of them retire.
+//     kernel (occupancy query), nor more than half of the compute units;
+//   * launch_persistent admits a launch only while the persistent launches in
+//     flight on the device (other jobs' included; released by a stream
+//     callback when each completes) leave a compute unit for every group;
+//   * every other kernel runs to completion without waiting on anything (the
+//     sampler is one workgroup), so the groups of an admitted launch that are
+//     not yet resident become resident once the ordinary kernels ahead of them
+//     retire.
 int persistent_group_budget(const void* kernel, int threads);  // groups one persistent launch may use
 void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream);
 

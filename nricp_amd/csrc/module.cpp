@@ -895,6 +895,98 @@ private:
     std::vector<int64_t> level_nodes_;
 };
 
+// The request loop (nricp_hip_server_*): jobs are submitted with the route
+// arguments, normalised as solve_route_* normalises them, and come back as
+// their fitted template in caller units with the run report and timings.
+class Server {
+public:
+    Server(int max_active, double budget) {
+        nricp_hip_server_config c{};
+        c.max_active = max_active;
+        c.device_budget_bytes = budget;
+        nricp_hip_server* h = nullptr;
+        check(nricp_hip_server_create(&c, &h));
+        handle_ = h;
+    }
+    ~Server() { close(); }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    void close() {
+        if (!handle_) return;
+        py::gil_scoped_release release;
+        nricp_hip_server_destroy(handle_);
+        handle_ = nullptr;
+    }
+    int64_t submit(std::unique_ptr<Prepared> p) {
+        if (!handle_) throw py::value_error("the server is closed");
+        const nricp_hip_problem pb = problem_of(*p);
+        int64_t ticket = 0;
+        check(nricp_hip_server_submit(handle_, &pb, &p->s, &ticket));
+        pending_[ticket] = Pending{p->norm, static_cast<int64_t>(p->tp.size() / 3)};
+        return ticket;
+    }
+    // Waits up to `timeout` seconds (None: without limit); None if the job is
+    // still running then.
+    py::object result(int64_t ticket, const py::object& timeout) {
+        if (!handle_) throw py::value_error("the server is closed");
+        auto it = pending_.find(ticket);
+        if (it == pending_.end()) throw py::value_error("unknown or already collected ticket " + std::to_string(ticket));
+        const double limit = timeout.is_none() ? -1.0 : timeout.cast<double>();
+        int32_t done = 0;
+        int st;
+        {
+            py::gil_scoped_release release;
+            st = nricp_hip_server_wait(handle_, ticket, limit, &done);
+        }
+        check(st);
+        if (!done) return py::none();
+        std::vector<double> v(3 * it->second.vertices);
+        size_t len = 0;
+        nricp_hip_job_info info{};
+        const int rs = nricp_hip_server_result(handle_, ticket, v.data(), nullptr, 0, &len, &info);
+        if (rs != 0) {
+            pending_.erase(it);
+            (void)nricp_hip_server_release(handle_, ticket);
+            check(rs);
+        }
+        std::string text(len + 1, '\0');
+        check(nricp_hip_server_result(handle_, ticket, nullptr, text.data(), text.size(), &len, nullptr));
+        text.resize(len);
+        denormalize(v, it->second.norm);
+        py::dict d;
+        d["vertices"] = arr2(v.data(), static_cast<py::ssize_t>(v.size() / 3), 3);
+        d["report"] = text;
+        d["queued_seconds"] = info.queued_seconds;
+        d["run_seconds"] = info.run_seconds;
+        d["device_bytes"] = info.device_bytes;
+        d["estimated_bytes"] = info.estimated_bytes;
+        pending_.erase(it);
+        check(nricp_hip_server_release(handle_, ticket));
+        return d;
+    }
+    py::dict stats() {
+        if (!handle_) throw py::value_error("the server is closed");
+        nricp_hip_server_stats st{};
+        check(nricp_hip_server_stats_get(handle_, &st));
+        py::dict d;
+        d["queued"] = st.queued;
+        d["active"] = st.active;
+        d["finished"] = st.finished;
+        d["budget_bytes"] = st.budget_bytes;
+        d["reserved_bytes"] = st.reserved_bytes;
+        return d;
+    }
+
+private:
+    struct Pending {
+        Normalization norm;
+        int64_t vertices;
+    };
+    nricp_hip_server* handle_ = nullptr;
+    std::map<int64_t, Pending> pending_;
+};
+
 int route_of(const std::string& name) {
     if (name == "route_a") return NRICP_HIP_ROUTE_A;
     if (name == "route_b") return NRICP_HIP_ROUTE_B;
@@ -1586,4 +1678,29 @@ PYBIND11_MODULE(nricp_dp, m) {
         .def("counters", &Session::counters)
         .def("profile", &Session::profile, py::arg("enable") = true)
         .def("kernel_time", &Session::kernel_time, py::arg("stage"));
+
+    py::class_<Server>(m, "Server",
+                       "A request loop over wrap jobs: submit() copies a job and returns a ticket at once; jobs run "
+                       "on the device in arrival order, up to max_active at a time and within the device memory "
+                       "budget (bytes; 0 takes 90 % of the free memory), each as its own resident session, so their "
+                       "work overlaps; result(ticket) waits for a job and returns its fitted template, report and "
+                       "timings. A job's result does not depend on what else runs.")
+        .def(py::init<int, double>(), py::arg("max_active") = 4, py::arg("device_budget_bytes") = 0.0)
+        .def("submit",
+             [](Server& self, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
+                const py::object& mk, const py::object& li, const py::object& lt, double dist, double mincos,
+                double slope, double d0, double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double pt,
+                int pm, int nb, const std::string& route, const std::string& precision) {
+                 const int r = route_of(route);
+                 const bool exact = r == NRICP_HIP_ROUTE_A;
+                 return self.submit(prepare(tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att,
+                                            diag, gl, n0, g, exact ? kRouteATolerance : pt,
+                                            exact ? kRouteAMaxIterations : pm, nb, r, exact ? "double" : precision));
+             },
+             ROUTE_ARGS, py::arg("route") = "route_b", py::arg("precision") = "mixed")
+        .def("result", &Server::result, py::arg("ticket"), py::arg("timeout") = py::none())
+        .def("stats", &Server::stats)
+        .def("close", &Server::close)
+        .def("__enter__", [](Server& self) -> Server& { return self; }, py::return_value_policy::reference)
+        .def("__exit__", [](Server& self, const py::object&, const py::object&, const py::object&) { self.close(); });
 }

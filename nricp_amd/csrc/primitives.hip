@@ -8,6 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "primitives.hpp"
@@ -231,11 +234,53 @@ __global__ void run_start_flags(const uint64_t* __restrict__ keys, int64_t n, in
 
 }  // namespace
 
-int persistent_group_budget(const void* kernel, int threads) {
+namespace {
+
+// Grid-barrier launches in flight per device, counted in compute units (a
+// persistent group holds one). Several jobs can run on one device at once
+// (each on its own streams); admitting a persistent launch only while the
+// launches in flight leave room for all of its groups keeps every admitted
+// launch fully resident once the ordinary kernels ahead of it retire, so no
+// two grid barriers can wait on each other's compute units.
+struct Residency {
+    std::mutex mutex;
+    std::condition_variable freed;
+    std::map<int, int> in_flight;
+};
+
+Residency& residency() {
+    static Residency* r = new Residency;  // outlives every stream callback
+    return *r;
+}
+
+struct Admitted {
+    int device, groups;
+};
+
+void release_groups(void* arg) {
+    const Admitted* a = static_cast<const Admitted*>(arg);
+    {
+        std::lock_guard<std::mutex> lock(residency().mutex);
+        residency().in_flight[a->device] -= a->groups;
+    }
+    residency().freed.notify_all();
+    delete a;
+}
+
+int device_cus(int* device_out) {
     int device = 0;
     NRICP_CHECK(hipGetDevice(&device));
     int cus = 0;
     NRICP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    *device_out = device;
+    return cus;
+}
+
+}  // namespace
+
+int persistent_group_budget(const void* kernel, int threads) {
+    int device = 0;
+    const int cus = device_cus(&device);
     int per_cu = 0;
     NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
     if (per_cu < 1) throw DeviceError("a persistent kernel does not fit on a compute unit");
@@ -245,7 +290,22 @@ int persistent_group_budget(const void* kernel, int threads) {
 void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream) {
     if (groups < 1 || groups > persistent_group_budget(kernel, threads))
         throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
-    NRICP_CHECK(hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream));
+    int device = 0;
+    const int cus = device_cus(&device);
+    {
+        std::unique_lock<std::mutex> lock(residency().mutex);
+        residency().freed.wait(lock, [&] { return residency().in_flight[device] + groups <= cus; });
+        residency().in_flight[device] += groups;
+    }
+    Admitted* admitted = new Admitted{device, groups};
+    hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);
+    if (e == hipSuccess) e = hipLaunchHostFunc(stream, release_groups, admitted);
+    if (e != hipSuccess) {
+        // Nothing will run the release: wait for the stream, then release here.
+        (void)hipStreamSynchronize(stream);
+        release_groups(admitted);
+        check(e, "persistent launch");
+    }
 }
 
 int bits_for(uint64_t v) {
