@@ -366,15 +366,29 @@ def config_job(index: int, seed: int = 0) -> WrapJob:
     raise ValueError(f"no single-job configuration {index}")
 
 
-def batch_jobs(count: int = 64, seed: int = 1000, lo: int = 2000, hi: int = 50_000) -> list[WrapJob]:
+def batch_jobs(count: int = 64, seed: int = 1000, lo: int = 2000, hi: int = 50_000,
+               kind: str = "humanoid") -> list[WrapJob]:
     """Uneven batch: template sizes log-uniform in [lo, hi], target 2-3x the
-    template's triangles, nodes ~ vertices / 25, one seed per job."""
+    template's triangles, nodes ~ vertices / 25, one seed per job. "humanoid"
+    generates each job as the 50k humanoid configuration does (4 levels);
+    "sphere" uses geodesic spheres (3 levels), which are much faster to
+    generate."""
     rng = np.random.default_rng(seed)
     jobs = []
     for k in range(count):
         v = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
-        f = frequency_for_vertices(int(v))
-        g = max(f + 1, int(round(f * math.sqrt(rng.uniform(2.0, 3.0)))))
-        nodes = max(8, int(round((10 * f * f + 2) / 25)))
-        jobs.append(sphere_job(f, g, seed + k, nodes_final=nodes, graph_levels=3, name=f"batch{k}"))
+        ratio = rng.uniform(2.0, 3.0)
+        if kind == "humanoid":
+            nodes = max(8, int(round(v / 25)))
+            job = humanoid_job(seed + k, template_vertices=int(v), target_triangles=int(2 * v * ratio),
+                               nodes_final=nodes, graph_levels=4)
+            job.name = f"batch{k}"
+            jobs.append(job)
+        elif kind == "sphere":
+            f = frequency_for_vertices(int(v))
+            g = max(f + 1, int(round(f * math.sqrt(ratio))))
+            nodes = max(8, int(round((10 * f * f + 2) / 25)))
+            jobs.append(sphere_job(f, g, seed + k, nodes_final=nodes, graph_levels=3, name=f"batch{k}"))
+        else:
+            raise ValueError(f"unknown batch kind {kind!r}")
     return jobs
