@@ -89,23 +89,42 @@ def vertex_normals(points: np.ndarray, faces: np.ndarray) -> np.ndarray:
     return n / np.maximum(length, 1e-300)
 
 
-def rbf_bumps(points: np.ndarray, normals: np.ndarray, rng: np.random.Generator, count: int = 8,
-              amplitude: tuple[float, float] = (0.05, 0.15), width: tuple[float, float] = (0.2, 0.5),
-              center: np.ndarray | None = None) -> np.ndarray:
-    """Displace along the normals by ``count`` Gaussian bumps on the sphere of
-    directions around ``center``: amplitude U(a) x bbox diagonal, angular width
-    U(w) radians."""
+@dataclass
+class BumpField:
+    """``count`` Gaussian bumps on the sphere of directions around ``origin``:
+    displacement(x) = sum_k amp_k exp(-(angle(x, centre_k) / width_k)^2)."""
+    origin: np.ndarray
+    centres: np.ndarray
+    amps: np.ndarray
+    widths: np.ndarray
+
+    def displacement(self, points: np.ndarray) -> np.ndarray:
+        dirs = points - self.origin
+        dirs = dirs / np.maximum(np.linalg.norm(dirs, axis=1, keepdims=True), 1e-300)
+        angle = np.arccos(np.clip(dirs @ self.centres.T, -1.0, 1.0))
+        return (self.amps[None, :] * np.exp(-(angle / self.widths[None, :]) ** 2)).sum(axis=1)
+
+
+def bump_field(points: np.ndarray, rng: np.random.Generator, count: int = 8,
+               amplitude: tuple[float, float] = (0.05, 0.15), width: tuple[float, float] = (0.2, 0.5),
+               center: np.ndarray | None = None) -> BumpField:
+    """Amplitude U(a) x the points' bbox diagonal, angular width U(w) radians,
+    centres uniform on the sphere."""
     diag = bbox_diagonal(points)
     origin = points.mean(axis=0) if center is None else center
-    dirs = points - origin
-    dirs /= np.maximum(np.linalg.norm(dirs, axis=1, keepdims=True), 1e-300)
     centres = rng.normal(size=(count, 3))
     centres /= np.linalg.norm(centres, axis=1, keepdims=True)
     amps = rng.uniform(*amplitude, size=count) * diag
     widths = rng.uniform(*width, size=count)
-    angle = np.arccos(np.clip(dirs @ centres.T, -1.0, 1.0))
-    displacement = (amps[None, :] * np.exp(-(angle / widths[None, :]) ** 2)).sum(axis=1)
-    return points + normals * displacement[:, None]
+    return BumpField(np.asarray(origin, dtype=np.float64), centres, amps, widths)
+
+
+def rbf_bumps(points: np.ndarray, normals: np.ndarray, rng: np.random.Generator, count: int = 8,
+              amplitude: tuple[float, float] = (0.05, 0.15), width: tuple[float, float] = (0.2, 0.5),
+              center: np.ndarray | None = None) -> np.ndarray:
+    """Displace along the normals by a bump field (bump_field)."""
+    field = bump_field(points, rng, count, amplitude, width, center)
+    return points + normals * field.displacement(points)[:, None]
 
 
 def shuffle_triangles(faces: np.ndarray, rng: np.random.Generator) -> np.ndarray:
@@ -122,12 +141,14 @@ class CapsuleBody:
     ends: np.ndarray
     radii: np.ndarray
     parents: list[int] = field(default_factory=list)
+    motion: tuple | None = None  # set by posed(): (rotations, rest starts, posed starts)
 
     def posed(self, rng: np.random.Generator, max_degrees: float = 30.0) -> "CapsuleBody":
         """Each segment rotated rigidly about its own joint (its start point) by
         up to ``max_degrees``; children follow their parent's end point."""
         starts = self.starts.copy()
         ends = self.ends.copy()
+        rotations = []
         for i in range(len(starts)):
             p = self.parents[i]
             if p >= 0:
@@ -140,7 +161,27 @@ class CapsuleBody:
             k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
             r = np.eye(3) + np.sin(angle) * k + (1 - np.cos(angle)) * (k @ k)
             ends[i] = starts[i] + r @ (ends[i] - starts[i])
-        return CapsuleBody(starts, ends, self.radii.copy(), list(self.parents))
+            rotations.append(r)
+        posed = CapsuleBody(starts, ends, self.radii.copy(), list(self.parents))
+        # Segment i maps x to R_i (x - start_i) + start'_i.
+        posed.motion = (np.array(rotations), self.starts.copy(), starts.copy())
+        return posed
+
+    def nearest_segment(self, x: np.ndarray) -> np.ndarray:
+        """Index of the capsule whose surface is nearest to each point."""
+        best = None
+        index = np.zeros(len(x), dtype=np.int64)
+        for i, (s, e, r) in enumerate(zip(self.starts, self.ends, self.radii)):
+            d = e - s
+            t = np.clip(((x - s) @ d) / max(float(d @ d), 1e-12), 0.0, 1.0)
+            dist = np.abs(np.linalg.norm(x - (s + t[:, None] * d), axis=1) - r)
+            if best is None:
+                best = dist
+            else:
+                closer = dist < best
+                best = np.where(closer, dist, best)
+                index = np.where(closer, i, index)
+        return index
 
     def sdf(self, x: np.ndarray) -> np.ndarray:
         """Smooth-union signed distance (polynomial smooth min, k = 0.02)."""
@@ -288,6 +329,9 @@ class WrapJob:
     nodes_final: int
     graph_levels: int
     seed: int
+    # Where each template vertex belongs on the target, when the generator
+    # knows it (the quality harness's correspondence error); None otherwise.
+    ground_truth: np.ndarray | None = None
 
     @property
     def nodes_initial(self) -> int:
@@ -304,14 +348,19 @@ def sphere_job(template_frequency: int, target_frequency: int, seed: int, nodes_
     tp, tf = geodesic_sphere(template_frequency)
     gp, gf = geodesic_sphere(target_frequency)
     gn = vertex_normals(gp, gf)
-    gp = rbf_bumps(gp, gn, rng)
+    field = bump_field(gp, rng)
+    gp = gp + gn * field.displacement(gp)[:, None]
+    # The clean target surface in each template vertex's direction (the unit
+    # sphere displaced radially by the bump field).
+    truth = tp / np.linalg.norm(tp, axis=1, keepdims=True)
+    truth = truth * (1.0 + field.displacement(tp))[:, None]
     diag = bbox_diagonal(gp)
     if jitter > 0.0:
         gp = gp + rng.normal(scale=jitter * diag, size=gp.shape)
     if normal_noise > 0.0:
         gp = gp + vertex_normals(gp, gf) * rng.normal(scale=normal_noise * diag, size=(len(gp), 1))
     gf = shuffle_triangles(gf, rng)
-    return WrapJob(name, tp, tf, gp, gf, nodes_final, graph_levels, seed)
+    return WrapJob(name, tp, tf, gp, gf, nodes_final, graph_levels, seed, truth)
 
 
 def humanoid_job(seed: int, template_vertices: int = 50_000, target_triangles: int = 120_000,
@@ -322,6 +371,11 @@ def humanoid_job(seed: int, template_vertices: int = 50_000, target_triangles: i
     body = humanoid(rng)
     tp, tf = mesh_body(body, template_vertices)
     posed = body.posed(rng)
+    # Each template vertex carried by its nearest capsule's rigid motion (the
+    # target's small bumps and holes aside).
+    rotations, rest_starts, posed_starts = posed.motion
+    seg = body.nearest_segment(tp)
+    truth = np.einsum("nij,nj->ni", rotations[seg], tp - rest_starts[seg]) + posed_starts[seg]
     gp, gf = mesh_body(posed, target_triangles // 2)
     gp = rbf_bumps(gp, vertex_normals(gp, gf), rng, amplitude=(0.005, 0.015))
     keep = rng.random(len(gf)) >= hole_fraction
@@ -329,7 +383,7 @@ def humanoid_job(seed: int, template_vertices: int = 50_000, target_triangles: i
     used = np.unique(gf)
     remap = -np.ones(len(gp), dtype=np.int64)
     remap[used] = np.arange(len(used))
-    return WrapJob("humanoid", tp, tf, gp[used], remap[gf], nodes_final, graph_levels, seed)
+    return WrapJob("humanoid", tp, tf, gp[used], remap[gf], nodes_final, graph_levels, seed, truth)
 
 
 def level_settings(levels: int, outer: int = 10, inner: int = 1, **weights) -> list[dict]:
