@@ -8,7 +8,8 @@ vertices / 25, 4 levels x 10 Gauss-Newton steps, FP32 iterations with FP64
 refinement. The batch runs through nricp_dp.Server, so each job is its own
 resident session on its own streams and the jobs' kernels share the device:
 while one job waits on a grid barrier or a host decision, the others' work
-fills the compute units.
+fills the compute units. The process asks HIP for 16 hardware queues (4 by
+default), so the jobs' streams do not serialise on shared queues.
 
 Tuning: the batch's first `--tune-jobs` jobs are run at each candidate slot
 count; the count with the highest jobs/s runs the whole batch. The whole batch
@@ -89,7 +90,12 @@ def main(argv=None) -> int:
     ap.add_argument("--tune-jobs", type=int, default=24)
     ap.add_argument("--check", type=int, default=4, help="jobs compared alone vs in the batch")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="hardware queues HIP gives the process (GPU_MAX_HW_QUEUES; unless already set): with the "
+                         "default 4, the jobs' streams share queues and their kernels serialise")
     args = ap.parse_args(argv)
+    # Read once, when HIP initialises: set before anything touches the device.
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.hw_queues))
 
     t0 = time.perf_counter()
     jobs = make_batch(args.jobs, args.seed)

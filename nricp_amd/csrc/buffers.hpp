@@ -11,15 +11,40 @@
 
 namespace nricp_hip {
 
-// Device memory goes through a process-wide cache: a released block is kept
-// for the next request of its size class instead of being returned with
-// hipFree, which waits for the whole device and would stall every other job
-// on the GPU. Blocks are never returned before the process ends; the bytes
-// held by the calling thread's live buffers are counted per thread (a
+// Device memory goes through a process-wide, stream-ordered cache. hipFree
+// waits for the whole device and would stall every other job on the GPU, so
+// released blocks are kept for the next request of their size class. Each
+// block belongs to the stream in scope when it was allocated (StreamScope):
+// released, it goes back to that stream's free list and only allocations
+// ordered on the same stream take it again, so work still queued there that
+// uses the old contents runs before any new writer. When a Stream is
+// destroyed it is synchronised first and its free list becomes idle, for any
+// stream to take. A block allocated with no stream in scope waits for the
+// device when released. A buffer that work on another stream also uses must
+// outlive that work (the owner synchronises that stream first). Blocks are
+// returned to the runtime only when an allocation would otherwise fail. The
+// bytes held by the calling thread's live buffers are counted per thread (a
 // worker drives one job at a time, so that is the job's footprint).
-void* device_allocate(int64_t bytes, int64_t* granted, int* device);
-void device_release(void* p, int64_t bytes, int device);
+void* device_allocate(int64_t bytes, int64_t* granted, int* device, hipStream_t* stream);
+void device_release(void* p, int64_t bytes, int device, hipStream_t stream);
 int64_t thread_device_bytes();
+// The calling thread's allocation stream; returns the previous one.
+hipStream_t swap_allocation_stream(hipStream_t s);
+// Stream lifetime, for the cache: created, and retired once synchronised.
+void stream_created(hipStream_t s);
+void stream_retired(hipStream_t s);
+
+// Orders the buffers the calling thread allocates in this scope on `s`.
+class StreamScope {
+public:
+    explicit StreamScope(hipStream_t s) : saved_(swap_allocation_stream(s)) {}
+    ~StreamScope() { swap_allocation_stream(saved_); }
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+
+private:
+    hipStream_t saved_;
+};
 
 template <typename T>
 class DeviceBuffer {
@@ -30,7 +55,7 @@ public:
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     DeviceBuffer(DeviceBuffer&& o) noexcept
-        : ptr_(o.ptr_), size_(o.size_), cap_(o.cap_), bytes_(o.bytes_), device_(o.device_) {
+        : ptr_(o.ptr_), size_(o.size_), cap_(o.cap_), bytes_(o.bytes_), device_(o.device_), stream_(o.stream_) {
         o.ptr_ = nullptr;
         o.size_ = o.cap_ = o.bytes_ = 0;
     }
@@ -42,6 +67,7 @@ public:
             cap_ = o.cap_;
             bytes_ = o.bytes_;
             device_ = o.device_;
+            stream_ = o.stream_;
             o.ptr_ = nullptr;
             o.size_ = o.cap_ = o.bytes_ = 0;
         }
@@ -53,13 +79,13 @@ public:
         if (n > cap_) {
             release();
             const int64_t bytes = (n > 0 ? n : 1) * static_cast<int64_t>(sizeof(T));
-            ptr_ = static_cast<T*>(device_allocate(bytes, &bytes_, &device_));
+            ptr_ = static_cast<T*>(device_allocate(bytes, &bytes_, &device_, &stream_));
             cap_ = n;
         }
         size_ = n;
     }
     void release() {
-        if (ptr_) device_release(ptr_, bytes_, device_);
+        if (ptr_) device_release(ptr_, bytes_, device_, stream_);
         ptr_ = nullptr;
         size_ = cap_ = 0;
         bytes_ = 0;
@@ -91,14 +117,22 @@ private:
     int64_t cap_ = 0;
     int64_t bytes_ = 0;  // granted by the cache
     int device_ = 0;
+    hipStream_t stream_ = nullptr;  // the stream the block is ordered on
 };
 
 // A stream owned by one object (default priority).
 class Stream {
 public:
-    Stream() { NRICP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+    Stream() {
+        NRICP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        stream_created(s_);
+    }
     ~Stream() {
-        if (s_) (void)hipStreamDestroy(s_);
+        if (s_) {
+            (void)hipStreamSynchronize(s_);
+            stream_retired(s_);
+            (void)hipStreamDestroy(s_);
+        }
     }
     Stream(const Stream&) = delete;
     Stream& operator=(const Stream&) = delete;

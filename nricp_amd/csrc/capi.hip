@@ -115,6 +115,7 @@ int nricp_hip_vertex_normals(const double* points, int64_t nv, const int32_t* fa
         require(nv >= 0 && nf >= 0 && (nv == 0 || points) && (nf == 0 || faces) && out, "vertex_normals: bad arrays");
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         SortScratch sort;
         DeviceBuffer<double> p, n, scratch;
         DeviceBuffer<int32_t> f;
@@ -139,6 +140,7 @@ void query_target(const double* tp, int64_t tv, const int32_t* tf, int64_t nt, c
     require(nq >= 0 && (nq == 0 || q), "queries: bad array");
     ensure_device();
     hipStream_t st = op_stream();
+    StreamScope scope(st);
     SortScratch sort;
     Lbvh bvh;
     VertexFaces vf;
@@ -240,6 +242,7 @@ int nricp_hip_farthest_point_seeds(const double* points, int64_t n, int32_t seed
         const int32_t count = static_cast<int32_t>(std::min<int64_t>(seeds, n));
         const int32_t f = static_cast<int32_t>(std::clamp<int64_t>(first, 0, n - 1));
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         DeviceBuffer<double> p;
         FpsWork work;
         DeviceBuffer<int32_t> o(count);
@@ -267,6 +270,7 @@ int nricp_hip_bind_vertices(const double* points, int64_t n, const double* nodes
         }
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         DeviceBuffer<double> p, g, w;
         DeviceBuffer<int32_t> b, c;
         p.upload(points, 3 * n, st);
@@ -293,6 +297,7 @@ int nricp_hip_build_node_edges(const double* nodes, int64_t nn, int32_t k, int32
         if (nn <= 1) return;
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         SortScratch sort;
         DeviceBuffer<double> g;
         DeviceBuffer<int32_t> e, flags, pos;
@@ -315,6 +320,7 @@ int nricp_hip_evaluate_deformation(const double* points, int64_t n, const int32_
         if (n == 0) return;
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         DeviceBuffer<double> p, w, r, t, g, o;
         DeviceBuffer<int32_t> b, c;
         p.upload(points, 3 * n, st);
@@ -337,6 +343,7 @@ int nricp_hip_retract(const double* R, const double* T, const double* delta, int
         if (nn == 0) return;
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         DeviceBuffer<double> r, t, d, ro(9 * nn), to(3 * nn);
         r.upload(R, 9 * nn, st);
         t.upload(T, 3 * nn, st);
@@ -362,6 +369,7 @@ int nricp_hip_transfer_level(const double* from, int64_t nf, const double* R, co
         }
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         DeviceBuffer<double> f, r, t, g, ro(9 * nt), tout(3 * nt);
         f.upload(from, 3 * nf, st);
         r.upload(R, 9 * nf, st);
@@ -447,6 +455,7 @@ int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precis
         require(inputs != nullptr, "fused_system: null inputs");
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         UploadedEnergy u;
         upload_energy(*inputs, u, st);
         const int64_t nn = inputs->node_count;
@@ -512,6 +521,7 @@ int nricp_hip_solve_system(const double* diag, int64_t nn, const int32_t* pairs,
                     "solve_system: pair indices out of range");
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         // Block-sparse rows of the full symmetric matrix, built on the host
         // (this entry point takes an already-assembled system).
         std::vector<std::vector<std::pair<int32_t, int64_t>>> rows(nn);  // (column, source)
@@ -581,11 +591,14 @@ int nricp_hip_graph_build(const double* points, int64_t nv, int32_t levels, int3
         require(nv > 0 && points, "graph_build: the template has no vertices");
         ensure_device();
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         auto g = std::make_unique<nricp_hip_graph>();
         DeviceBuffer<double> p;
         p.upload(points, 3 * nv, st);
         SortScratch sort;
         build_device_graph(p.get(), nv, levels, n0, growth, k, neighbours, g->graph, sort, st);
+        // Sessions read the graph from their own streams.
+        NRICP_CHECK(hipStreamSynchronize(st));
         *out = g.release();
     });
 }
@@ -611,6 +624,7 @@ int nricp_hip_graph_level_copy(const nricp_hip_graph* g, int32_t level, int32_t*
     return guarded([&] {
         require(g && level >= 0 && level < static_cast<int32_t>(g->graph.levels.size()), "graph level out of range");
         hipStream_t st = op_stream();
+        StreamScope scope(st);
         const GraphLevel& L = g->graph.levels[level];
         const int64_t nv = g->graph.vertex_count;
         if (idx) L.node_indices.download(idx, L.node_count, st);

@@ -41,11 +41,14 @@ using Idx = py::array_t<int64_t, py::array::c_style | py::array::forcecast>;
 constexpr double kRouteATolerance = 1e-12;  // route A solves the system exactly
 constexpr int kRouteAMaxIterations = 5000;
 
-void check(int status) {
-    if (status == NRICP_HIP_OK) return;
-    const std::string msg = nricp_hip_last_error();
+void raise_status(int status, const std::string& msg) {
     if (status == NRICP_HIP_INVALID_ARGUMENT) throw py::value_error(msg);
     throw std::runtime_error(msg);
+}
+
+void check(int status) {
+    if (status == NRICP_HIP_OK) return;
+    raise_status(status, nricp_hip_last_error());
 }
 
 struct V3 {
@@ -946,9 +949,12 @@ public:
         nricp_hip_job_info info{};
         const int rs = nricp_hip_server_result(handle_, ticket, v.data(), nullptr, 0, &len, &info);
         if (rs != 0) {
+            // The job failed: raise its error (read before the release
+            // below replaces the thread's last error).
             pending_.erase(it);
+            const std::string message = nricp_hip_last_error();
             (void)nricp_hip_server_release(handle_, ticket);
-            check(rs);
+            raise_status(rs, message);
         }
         std::string text(len + 1, '\0');
         check(nricp_hip_server_result(handle_, ticket, nullptr, text.data(), text.size(), &len, nullptr));

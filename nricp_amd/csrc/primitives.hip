@@ -241,7 +241,11 @@ namespace {
 // (each on its own streams); admitting a persistent launch only while the
 // launches in flight leave room for all of its groups keeps every admitted
 // launch fully resident once the ordinary kernels ahead of it retire, so no
-// two grid barriers can wait on each other's compute units.
+// two grid barriers can wait on each other's compute units. The dispatcher
+// deals a launch's groups to the XCDs in turn, so a launch of g groups needs
+// up to ceil(g / xcds) compute units on every XCD: a launch is charged
+// ceil(g / xcds) * xcds units, which keeps each XCD's share within its own
+// compute units whatever XCD the launch starts on.
 struct Residency {
     std::mutex mutex;
     std::condition_variable freed;
@@ -276,6 +280,15 @@ int device_cus(int* device_out) {
     return cus;
 }
 
+int device_xcds(int device) {
+    int xcds = 1;
+    if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || xcds < 1) {
+        (void)hipGetLastError();
+        xcds = 1;
+    }
+    return xcds;
+}
+
 }  // namespace
 
 int persistent_group_budget(const void* kernel, int threads) {
@@ -292,12 +305,14 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
         throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
     int device = 0;
     const int cus = device_cus(&device);
+    const int xcds = device_xcds(device);
+    const int charge = std::min(cus, (groups + xcds - 1) / xcds * xcds);
     {
         std::unique_lock<std::mutex> lock(residency().mutex);
-        residency().freed.wait(lock, [&] { return residency().in_flight[device] + groups <= cus; });
-        residency().in_flight[device] += groups;
+        residency().freed.wait(lock, [&] { return residency().in_flight[device] + charge <= cus; });
+        residency().in_flight[device] += charge;
     }
-    Admitted* admitted = new Admitted{device, groups};
+    Admitted* admitted = new Admitted{device, charge};
     hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);
     if (e == hipSuccess) e = hipLaunchHostFunc(stream, release_groups, admitted);
     if (e != hipSuccess) {

@@ -273,6 +273,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
     report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
     hipStream_t st = stream_.get();
+    StreamScope scope(st);  // the session's buffers are ordered on its stream
     nv_ = pb.template_vertex_count;
     nf_ = pb.template_face_count;
     points_.upload(pb.template_points, 3 * nv_, st);
@@ -419,6 +420,10 @@ void Session::prepare() {
         // seeds it uses, so the finer levels' sampling overlaps the coarser
         // levels' solves.
         const int lv = static_cast<int>(own_graph_.levels.size());
+        // The sampler's buffers are ordered on its own stream: whatever it
+        // allocates or releases, the main stream never reuses while the
+        // sampling still runs.
+        StreamScope fps_scope(fps_stream_.get());
         clock_.begin("fps", fps_stream_.get());
         // First the template's Morton order (which the queries share) and
         // seed 0, marked by an event of its own.
@@ -464,6 +469,9 @@ void Session::prepare() {
 }
 
 void Session::reset() {
+    // The rebuild rewrites buffers that queued work of the last run may still read.
+    NRICP_CHECK(hipStreamSynchronize(fps_stream_.get()));
+    NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     phase_ = Phase::Prepare;
     level_ = 0;
     outer_ = inner_ = 0;
@@ -752,6 +760,7 @@ void Session::final_statistics() {
 
 nricp_hip_progress Session::step() {
     const auto t0 = std::chrono::steady_clock::now();
+    StreamScope scope(stream_.get());
     switch (phase_) {
         case Phase::Prepare:
             prepare();
@@ -797,6 +806,7 @@ void Session::run() {
 
 void Session::vertices(double* out) {
     hipStream_t st = stream_.get();
+    StreamScope scope(st);
     if (phase_ == Phase::Prepare || (phase_ == Phase::BeginLevel && level_ == 0)) {
         NRICP_CHECK(hipMemcpyAsync(out, points_.get(), 3 * nv_ * sizeof(double), hipMemcpyDeviceToHost, st));
         sync();
@@ -832,6 +842,7 @@ void Session::level_nodes(int64_t* out) const {
 
 void Session::state(double* R, double* T, double* G) {
     hipStream_t st = stream_.get();
+    StreamScope scope(st);
     if (graph_->levels.empty()) return;  // nothing built yet: no nodes
     const size_t l = (phase_ == Phase::BeginLevel && level_ > 0) ? level_ - 1 : level_;
     const GraphLevel& L = graph_->levels[l];

@@ -10,6 +10,8 @@ gaps; a submitter thread submits each job at its arrival time. Latency is a
 job's submission-to-result time as the server measured it (queueing
 included). Throughput is the jobs completed over the span from the first
 arrival to the last completion.
+The process asks HIP for 16 hardware queues (4 by default), so concurrent
+jobs' streams do not serialise on shared queues.
 
 Run from the repository root:  python -m nricp_amd.serve_bench
 Prints one JSON line per (form, offered load), then a summary line.
@@ -117,7 +119,12 @@ def main(argv=None) -> int:
     ap.add_argument("--forms", default="mixed,double")
     ap.add_argument("--seed", type=int, default=3000)
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="hardware queues HIP gives the process (GPU_MAX_HW_QUEUES; unless already set): with the "
+                         "default 4, the jobs' streams share queues and their kernels serialise")
     args = ap.parse_args(argv)
+    # Read once, when HIP initialises: set before anything touches the device.
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.hw_queues))
 
     t0 = time.perf_counter()
     procs = min(len(os.sched_getaffinity(0)), 16)
