@@ -8,10 +8,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <condition_variable>
+#include <chrono>
+#include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
 
 #include "primitives.hpp"
 
@@ -246,57 +251,107 @@ namespace {
 // up to ceil(g / xcds) compute units on every XCD: a launch is charged
 // ceil(g / xcds) * xcds units, which keeps each XCD's share within its own
 // compute units whatever XCD the launch starts on.
+//
+// The ledger: launches admitted and not yet known to be finished, each
+// with the event recorded after it. Charges return when the events complete,
+// found by querying them at the next admission (an event costs the stream
+// nothing, where a host callback would hold its later work back until the
+// host had run it).
+struct Admitted {
+    int device, charge;
+    hipEvent_t done;
+};
+
 struct Residency {
     std::mutex mutex;
-    std::condition_variable freed;
-    std::map<int, int> in_flight;
+    std::map<int, int> used;                      // device -> compute units charged
+    std::deque<Admitted> in_flight;               // admission order
+    std::vector<std::pair<int, hipEvent_t>> spare;  // finished events, for reuse
 };
 
 Residency& residency() {
-    static Residency* r = new Residency;  // outlives every stream callback
+    static Residency* r = new Residency;  // never destroyed: used until exit
     return *r;
 }
 
-struct Admitted {
-    int device, groups;
+bool finished(hipEvent_t e) {
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: leave no trace for the next launch check
+        return false;
+    }
+    return true;  // complete (or failed: the failure surfaces on the job's own stream)
+}
+
+// Returns the charges of every finished launch. Caller holds the mutex.
+void reclaim(Residency& r) {
+    for (auto it = r.in_flight.begin(); it != r.in_flight.end();) {
+        if (finished(it->done)) {
+            r.used[it->device] -= it->charge;
+            r.spare.emplace_back(it->device, it->done);
+            it = r.in_flight.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+hipEvent_t take_event(Residency& r, int device) {
+    for (auto it = r.spare.begin(); it != r.spare.end(); ++it) {
+        if (it->first == device) {
+            const hipEvent_t e = it->second;
+            r.spare.erase(it);
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+struct DeviceShape {
+    int cus = 0, xcds = 1;
 };
 
-void release_groups(void* arg) {
-    const Admitted* a = static_cast<const Admitted*>(arg);
-    {
-        std::lock_guard<std::mutex> lock(residency().mutex);
-        residency().in_flight[a->device] -= a->groups;
-    }
-    residency().freed.notify_all();
-    delete a;
-}
-
-int device_cus(int* device_out) {
+// Compute units and XCDs of the current device, queried once per device
+// (the queries cost tens of microseconds, paid before every solve's launch).
+DeviceShape device_shape(int* device_out) {
     int device = 0;
     NRICP_CHECK(hipGetDevice(&device));
-    int cus = 0;
-    NRICP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     *device_out = device;
-    return cus;
-}
-
-int device_xcds(int device) {
-    int xcds = 1;
-    if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || xcds < 1) {
+    static std::mutex mutex;
+    static std::map<int, DeviceShape>* known = new std::map<int, DeviceShape>;
+    std::lock_guard<std::mutex> lock(mutex);
+    auto it = known->find(device);
+    if (it != known->end()) return it->second;
+    DeviceShape d;
+    NRICP_CHECK(hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (hipDeviceGetAttribute(&d.xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || d.xcds < 1) {
         (void)hipGetLastError();
-        xcds = 1;
+        d.xcds = 1;
     }
-    return xcds;
+    (*known)[device] = d;
+    return d;
 }
 
 }  // namespace
 
 int persistent_group_budget(const void* kernel, int threads) {
     int device = 0;
-    const int cus = device_cus(&device);
-    int per_cu = 0;
-    NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
-    if (per_cu < 1) throw DeviceError("a persistent kernel does not fit on a compute unit");
+    const int cus = device_shape(&device).cus;
+    // Whether one group fits a compute unit, checked once per (device,
+    // kernel, block size).
+    static std::mutex mutex;
+    static std::set<std::tuple<int, const void*, int>>* fits = new std::set<std::tuple<int, const void*, int>>;
+    {
+        std::lock_guard<std::mutex> lock(mutex);
+        if (!fits->count({device, kernel, threads})) {
+            int per_cu = 0;
+            NRICP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+            if (per_cu < 1) throw DeviceError("a persistent kernel does not fit on a compute unit");
+            fits->insert({device, kernel, threads});
+        }
+    }
     return std::max(1, cus / 2);
 }
 
@@ -304,23 +359,40 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
     if (groups < 1 || groups > persistent_group_budget(kernel, threads))
         throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
     int device = 0;
-    const int cus = device_cus(&device);
-    const int xcds = device_xcds(device);
+    const DeviceShape shape = device_shape(&device);
+    const int cus = shape.cus, xcds = shape.xcds;
     const int charge = std::min(cus, (groups + xcds - 1) / xcds * xcds);
-    {
-        std::unique_lock<std::mutex> lock(residency().mutex);
-        residency().freed.wait(lock, [&] { return residency().in_flight[device] + charge <= cus; });
-        residency().in_flight[device] += charge;
+    Residency& r = residency();
+    std::unique_lock<std::mutex> lock(r.mutex);
+    // Until the charge fits: poll every admitted launch's event (launches
+    // of different jobs finish in no particular order), yielding the core
+    // between polls at first and then sleeping.
+    for (int polls = 0;; ++polls) {
+        reclaim(r);
+        if (r.used[device] + charge <= cus) break;
+        bool any = false;
+        for (const Admitted& a : r.in_flight) any = any || a.device == device;
+        if (!any) throw DeviceError("persistent launch ledger out of step");
+        lock.unlock();
+        if (polls < 512)
+            std::this_thread::yield();
+        else
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        lock.lock();
     }
-    Admitted* admitted = new Admitted{device, charge};
+    const hipEvent_t done = take_event(r, device);
+    // Launch and record under the lock, so the ledger never holds a charge
+    // without its event.
     hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);
-    if (e == hipSuccess) e = hipLaunchHostFunc(stream, release_groups, admitted);
+    if (e == hipSuccess) e = hipEventRecord(done, stream);
     if (e != hipSuccess) {
-        // Nothing will run the release: wait for the stream, then release here.
+        r.spare.emplace_back(device, done);
+        lock.unlock();
         (void)hipStreamSynchronize(stream);
-        release_groups(admitted);
         check(e, "persistent launch");
     }
+    r.used[device] += charge;
+    r.in_flight.push_back(Admitted{device, charge, done});
 }
 
 int bits_for(uint64_t v) {
