@@ -172,16 +172,20 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
 // above) on one monotonic arrival counter, zeroed before the launch: a group
 // arriving at barrier k draws a ticket in [kP, (k+1)P) and waits until the
 // counter reaches (k+1)P -- one atomic and a poll, no reset and no separate
-// generation word. Agent-scope acq_rel on the ticket and acquire on the poll
-// order every global write before the barrier against every read after it.
-// grid_arrive_wait is the part thread 0 runs between the two workgroup
-// barriers.
+// generation word. The ticket is an agent-scope release (this XCD's L2 is
+// written back first); the poll is relaxed, and one agent-scope acquire fence
+// after it (it invalidates this XCD's L2 and L1) orders every global write
+// before the barrier against every read after it. An acquire on each poll
+// would invalidate the L2 once per poll, evicting the lines that the groups
+// still computing on the same XCD are reading. grid_arrive_wait is the part
+// thread 0 runs between the two workgroup barriers.
 __device__ inline void grid_arrive_wait(unsigned* state, unsigned participants) {
-    const unsigned ticket = __hip_atomic_fetch_add(state, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned ticket = __hip_atomic_fetch_add(state, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned target = (ticket / participants + 1u) * participants;
-    while (static_cast<int>(__hip_atomic_load(state, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+    while (static_cast<int>(__hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
         __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 __device__ inline void grid_barrier(unsigned* state, unsigned participants) {
