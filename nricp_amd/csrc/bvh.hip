@@ -413,15 +413,27 @@ __device__ inline d3 closest_on_triangle(d3 p, d3 a, d3 b, d3 c, double* bary) {
 // rounding of the FP64 distances -- so every triangle that can tie or beat the
 // best is tested, and the result is the lexicographic minimum over all
 // triangles. Every decision is taken on values the eight lanes share (after
-// shuffles), so the group stays converged. The stack holds at most 7 entries
-// per wide level (21 levels for keys of at most 61 bits); the overflow check
-// and the step budget only guard against a malformed tree (the query then
-// reports triangle -1).
+// shuffles), so the group stays converged.
+//
+// Stack size: an open pushes at most 8 entries and then pops one, so at
+// most 7 entries per wide level of the current path stay below the newest
+// push: a tree of W wide levels needs at most 7 W - 6 (stack_entries_needed).
+// The query is instantiated for a few stack sizes and launched with the
+// smallest that the built tree's depth allows. The stack is the query's LDS,
+// and LDS is what limits the waves resident per compute unit for this
+// latency-bound traversal: the smallest stack keeps the most waves in flight.
+// The overflow check and the step budget only guard against a malformed tree
+// (the query then reports triangle -1).
 constexpr int kQueryThreads = 64;
 constexpr int kLanes = kWideArity;
 constexpr int kGroupsPerBlock = kQueryThreads / kLanes;
-constexpr int kStackDepth = 192;
 constexpr double kPruneSlack = 1.0 + 1e-12;
+
+// Binary depths 0, 3, 6, ... root the wide nodes.
+int64_t stack_entries_needed(int32_t depth_max) {
+    const int64_t levels = depth_max / 3 + 1;
+    return std::max<int64_t>(8, 7 * levels - 6);
+}
 
 struct Nearest {
     double d2;
@@ -483,6 +495,7 @@ __device__ inline void rank_step(double d, int push, int j, int& rank) {
     if (op && (od < d || (od == d && other < j))) ++rank;
 }
 
+template <int StackDepth>
 __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
                                     const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
                                     int32_t* stack_node, float* stack_dist, int j, Nearest& best) {
@@ -539,7 +552,7 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
         rank_step<6>(d, pushing, j, rank);
         rank_step<7>(d, pushing, j, rank);
         const int npush = group_sum(push ? 1 : 0);
-        if (sp + npush > kStackDepth) {
+        if (sp + npush > StackDepth) {
             best.triangle = -1;
             return;
         }
@@ -585,6 +598,7 @@ __device__ inline double robust_weight(int kind, double s, double t) {
 // (src/solver/correspondence.cpp:33-59), then the IRLS weight of
 // src/solver/route_a.cpp:19-38 for accepted rows (written by the group's
 // first lane).
+template <int StackDepth>
 __global__ void __launch_bounds__(kQueryThreads)
 query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, const double* __restrict__ corners,
              const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
@@ -594,8 +608,8 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
              double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
              int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
              const int32_t* __restrict__ order, uint32_t* __restrict__ stats) {
-    __shared__ int32_t stack_node[kGroupsPerBlock][kStackDepth];
-    __shared__ float stack_dist[kGroupsPerBlock][kStackDepth];
+    __shared__ int32_t stack_node[kGroupsPerBlock][StackDepth];
+    __shared__ float stack_dist[kGroupsPerBlock][StackDepth];
     const int j = threadIdx.x % kLanes;
     const int group = threadIdx.x / kLanes;
     const int64_t slot = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
@@ -633,8 +647,8 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
         best.triangle = seed;
     }
     if (wide_count > 0) {
-        nearest_wide(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group], j,
-                     best);
+        nearest_wide<StackDepth>(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group],
+                                 j, best);
     } else if (nt == 1) {
         const double* c = corners;
         double bary[3];
@@ -883,12 +897,20 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
         }
         const int32_t wide_count = runs[2 * kMaxDepth];
         bvh.wide_count = wide_count;
+        bvh.depth_max = 0;
+        for (int d = kMaxDepth - 1; d >= 0; --d) {
+            if (runs[2 * d + 1] > runs[2 * d]) {
+                bvh.depth_max = d;
+                break;
+            }
+        }
         bvh.wide.resize(std::max<int64_t>(wide_count, 1));
         collapse_kernel<<<blocks_for(ni, 128), 128, 0, stream>>>(bvh.nodes.get(), ni, bvh.depth.get(),
                                                                  bvh.wide_index.get(), bvh.wide.get());
         NRICP_LAUNCH_CHECK("collapse_kernel");
     } else {
         bvh.wide_count = 0;
+        bvh.depth_max = 0;
     }
 }
 
@@ -916,13 +938,24 @@ void closest_points(const Lbvh& bvh, const double* queries, const double* query_
                     double* out_distance, int32_t* out_triangle, uint8_t* out_rejection, double* out_robust,
                     hipStream_t stream, const int32_t* warm, const int32_t* order, QueryStats* stats) {
     if (count <= 0) return;
-    query_kernel<<<blocks_for(count * kLanes, kQueryThreads), kQueryThreads, 0, stream>>>(
-        bvh.wide.get(), bvh.wide_count, bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(),
-        bvh.faces.get(),
-        bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm, out_points,
-        out_normals, out_distance, out_triangle, out_rejection, out_robust, order,
-        stats ? stats->rows.get() : nullptr);
-    NRICP_LAUNCH_CHECK("query_kernel");
+    const int64_t need = stack_entries_needed(bvh.depth_max);
+    auto launch = [&](auto kernel) {
+        kernel<<<blocks_for(count * kLanes, kQueryThreads), kQueryThreads, 0, stream>>>(
+            bvh.wide.get(), bvh.wide_count, bvh.triangle_count, bvh.leaf_corners.get(), bvh.leaf_triangle.get(),
+            bvh.faces.get(), bvh.vertex_normals.get(), queries, query_normals, mask, count, s, bvh.points.get(), warm,
+            out_points, out_normals, out_distance, out_triangle, out_rejection, out_robust, order,
+            stats ? stats->rows.get() : nullptr);
+        NRICP_LAUNCH_CHECK("query_kernel");
+    };
+    // The depth cap of the tree build (kMaxDepth - 1) needs 7 * 43 - 6 = 295.
+    if (need <= 64)
+        launch(query_kernel<64>);
+    else if (need <= 96)
+        launch(query_kernel<96>);
+    else if (need <= 128)
+        launch(query_kernel<128>);
+    else
+        launch(query_kernel<296>);
 }
 
 void fold_query_stats(QueryStats& stats, int64_t count, hipStream_t stream) {

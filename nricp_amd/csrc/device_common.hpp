@@ -157,8 +157,10 @@ __device__ inline void block_sum(const double (&v)[Count], double* lds /* Count 
 //   * a launch never has more groups than the device holds at once for that
 //     kernel (occupancy query), nor more than half of the compute units;
 //   * launch_persistent admits a launch only while the persistent launches in
-//     flight on the device (other jobs' included; released by a stream
-//     callback when each completes) leave a compute unit for every group;
+//     flight on the device (other jobs' included; each released once the
+//     event recorded after it completes) leave a compute unit for every
+//     group on every XCD (a launch of g groups is charged ceil(g / xcds) *
+//     xcds, the most any XCD receives when groups are dealt to XCDs in turn);
 //   * every other kernel runs to completion without waiting on anything (the
 //     sampler is one workgroup), so the groups of an admitted launch that are
 //     not yet resident become resident once the ordinary kernels ahead of them

@@ -58,6 +58,7 @@ static_assert(sizeof(WideNode) == 256, "wide node layout");
 struct Lbvh {
     int64_t triangle_count = 0;
     int64_t wide_count = 0;
+    int32_t depth_max = 0;              // deepest internal node's binary depth (root 0)
     DeviceBuffer<WideNode> wide;        // wide nodes, root = 0
     DeviceBuffer<int32_t> depth;        // binary depth of each internal node
     DeviceBuffer<int32_t> wide_index;   // wide index of each internal node whose depth % 3 == 0
