@@ -11,10 +11,12 @@ while one job waits on a grid barrier or a host decision, the others' work
 fills the compute units. The process asks HIP for 16 hardware queues (4 by
 default), so the jobs' streams do not serialise on shared queues.
 
-Tuning: the batch's first `--tune-jobs` jobs are run at each candidate slot
-count; the count with the highest jobs/s runs the whole batch. The whole batch
-is also run one job at a time (the serial baseline), and a sample of jobs is
-checked to give the same fitted vertices alone and in the batch.
+Tuning: the whole batch is run `--repeats` times at each candidate slot
+count (a run takes about a second; a subset of the jobs tunes poorly, since
+the mix of sizes decides how many jobs overlap well). The count with the best
+median jobs/s then runs the batch once more for the reported figure. The
+whole batch is also run one job at a time (the serial baseline), and a sample
+of jobs is checked to give the same fitted vertices alone and in the batch.
 
 Run from the repository root:  python -m nricp_amd.batch
 Prints one JSON line.
@@ -87,7 +89,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--form", default="mixed")
     ap.add_argument("--candidates", default="1,2,4,6,8,12,16", help="slot counts the tuner measures")
-    ap.add_argument("--tune-jobs", type=int, default=24)
+    ap.add_argument("--repeats", type=int, default=2, help="runs of the whole batch per candidate")
     ap.add_argument("--check", type=int, default=4, help="jobs compared alone vs in the batch")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--hw-queues", type=int, default=16,
@@ -114,11 +116,10 @@ def main(argv=None) -> int:
     run_batch(dp, jobs[:8], 4, args.form)
 
     tuning = {}
-    subset = jobs[: min(args.tune_jobs, len(jobs))]
     for slots in [int(c) for c in args.candidates.split(",")]:
-        wall, _ = run_batch(dp, subset, slots, args.form)
-        tuning[slots] = round(len(subset) / wall, 3)
-        log(f"tuning: {slots} slots -> {tuning[slots]} jobs/s")
+        rates = [len(jobs) / run_batch(dp, jobs, slots, args.form)[0] for _ in range(max(1, args.repeats))]
+        tuning[slots] = round(float(np.median(rates)), 3)
+        log(f"tuning: {slots} slots -> {tuning[slots]} jobs/s (median of {len(rates)})")
     best = max(tuning, key=lambda k: tuning[k])
 
     wall, results = run_batch(dp, jobs, best, args.form)

@@ -33,6 +33,9 @@ hipStream_t swap_allocation_stream(hipStream_t s);
 // Stream lifetime, for the cache: created, and retired once synchronised.
 void stream_created(hipStream_t s);
 void stream_retired(hipStream_t s);
+// A synchronised stream about to be destroyed: its grid-barrier launches
+// leave the admission ledger (their events go with them).
+void launches_retired(hipStream_t s);
 
 // Orders the buffers the calling thread allocates in this scope on `s`.
 class StreamScope {
@@ -131,6 +134,7 @@ public:
         if (s_) {
             (void)hipStreamSynchronize(s_);
             stream_retired(s_);
+            launches_retired(s_);
             (void)hipStreamDestroy(s_);
         }
     }

@@ -260,13 +260,23 @@ namespace {
 struct Admitted {
     int device, charge;
     hipEvent_t done;
+    hipStream_t stream;  // the launch's stream, where `done` is recorded
 };
 
+struct Spare {
+    int device;
+    hipEvent_t event;
+    hipStream_t stream;  // where the event was last recorded
+};
+
+// An event is queried only while the stream it was last recorded on is
+// alive: before a stream is destroyed (synchronised first), its launches
+// leave the ledger and their events are destroyed (launches_retired).
 struct Residency {
     std::mutex mutex;
-    std::map<int, int> used;                      // device -> compute units charged
-    std::deque<Admitted> in_flight;               // admission order
-    std::vector<std::pair<int, hipEvent_t>> spare;  // finished events, for reuse
+    std::map<int, int> used;         // device -> compute units charged
+    std::deque<Admitted> in_flight;  // admission order
+    std::vector<Spare> spare;        // events of finished launches, for reuse
 };
 
 Residency& residency() {
@@ -276,11 +286,11 @@ Residency& residency() {
 
 bool finished(hipEvent_t e) {
     const hipError_t q = hipEventQuery(e);
-    if (q == hipErrorNotReady) {
-        (void)hipGetLastError();  // not an error: leave no trace for the next launch check
-        return false;
-    }
-    return true;  // complete (or failed: the failure surfaces on the job's own stream)
+    if (q == hipSuccess) return true;
+    // Not ready is no error; any other status surfaces on the job's own
+    // stream. Either way leave no trace for the thread's next launch check.
+    (void)hipGetLastError();
+    return q != hipErrorNotReady;
 }
 
 // Returns the charges of every finished launch. Caller holds the mutex.
@@ -288,7 +298,7 @@ void reclaim(Residency& r) {
     for (auto it = r.in_flight.begin(); it != r.in_flight.end();) {
         if (finished(it->done)) {
             r.used[it->device] -= it->charge;
-            r.spare.emplace_back(it->device, it->done);
+            r.spare.push_back(Spare{it->device, it->done, it->stream});
             it = r.in_flight.erase(it);
         } else {
             ++it;
@@ -298,8 +308,8 @@ void reclaim(Residency& r) {
 
 hipEvent_t take_event(Residency& r, int device) {
     for (auto it = r.spare.begin(); it != r.spare.end(); ++it) {
-        if (it->first == device) {
-            const hipEvent_t e = it->second;
+        if (it->device == device) {
+            const hipEvent_t e = it->event;
             r.spare.erase(it);
             return e;
         }
@@ -386,13 +396,36 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
     hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);
     if (e == hipSuccess) e = hipEventRecord(done, stream);
     if (e != hipSuccess) {
-        r.spare.emplace_back(device, done);
+        r.spare.push_back(Spare{device, done, stream});
         lock.unlock();
         (void)hipStreamSynchronize(stream);
         check(e, "persistent launch");
     }
     r.used[device] += charge;
-    r.in_flight.push_back(Admitted{device, charge, done});
+    r.in_flight.push_back(Admitted{device, charge, done, stream});
+}
+
+void launches_retired(hipStream_t stream) {
+    Residency& r = residency();
+    std::lock_guard<std::mutex> lock(r.mutex);
+    // The stream has drained: its launches are complete.
+    for (auto it = r.in_flight.begin(); it != r.in_flight.end();) {
+        if (it->stream == stream) {
+            r.used[it->device] -= it->charge;
+            (void)hipEventDestroy(it->done);
+            it = r.in_flight.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    for (auto it = r.spare.begin(); it != r.spare.end();) {
+        if (it->stream == stream) {
+            (void)hipEventDestroy(it->event);
+            it = r.spare.erase(it);
+        } else {
+            ++it;
+        }
+    }
 }
 
 int bits_for(uint64_t v) {

@@ -60,6 +60,78 @@ def test_closest_points_match_exactly_except_equidistant_ties(device, oracle):
     assert np.array_equal(d["vertex_normals"], o["vertex_normals"])
 
 
+def _morton10(q):
+    """Interleaved bits of three 10-bit cell coordinates (x highest), as the
+    device's hierarchy keys use them."""
+    code = 0
+    for bit in range(9, -1, -1):
+        for axis in range(3):
+            code = (code << 1) | ((int(q[axis]) >> bit) & 1)
+    return code
+
+
+def _deepest_internal_node(keys):
+    """Deepest internal node of the binary radix tree over sorted unique keys
+    (each node splits its range at the highest bit in which it differs)."""
+    keys = [int(k) for k in keys]
+
+    def walk(lo, hi, depth):
+        if hi == lo:
+            return -1
+        bit = (keys[lo] ^ keys[hi]).bit_length() - 1
+        split = next(i for i in range(lo, hi + 1) if (keys[i] >> bit) & 1)
+        return max(depth, walk(lo, split - 1, depth + 1), walk(split, hi, depth + 1))
+
+    return walk(0, len(keys) - 1, 0)
+
+
+def _deep_hierarchy_mesh(seed=11):
+    """Separate small triangles whose centroids put the hierarchy's keys on a
+    chain that splits one Morton bit per level (30 levels), above 4,097 keys
+    sharing one cell (about 12 more levels of index bits): a tree deeper than
+    a query stack of 96 entries covers by the 7 W - 6 bound."""
+    rng = np.random.default_rng(seed)
+    centroids = [rng.uniform(0.05, 0.95, size=3) for _ in range(4096)]  # cell 0
+    centroids.append(np.zeros(3))  # sets the low corner of the centroid box
+    for level in range(10):
+        for axis in range(3):
+            c = np.full(3, 0.5)
+            c[axis] = 2.0**level + 0.5
+            centroids.append(c)
+    centroids.append(np.full(3, 1024.0))  # sets the high corner: cells are unit cubes
+    centroids = np.array(centroids)
+    # A small triangle around each centroid, in a random orientation.
+    points, faces = [], []
+    for t, c in enumerate(centroids):
+        u, v = rng.normal(size=3), rng.normal(size=3)
+        a, b = 0.01 * u, 0.01 * v
+        points += [c + a, c + b, c - a - b]
+        faces.append([3 * t, 3 * t + 1, 3 * t + 2])
+    points, faces = np.array(points), np.array(faces)
+    # The keys as the device builds them: cells of the centroids' box, then
+    # the triangle index in the low bits.
+    cent = points[faces].mean(axis=1)
+    lo, hi = cent.min(0), cent.max(0)
+    cells = np.clip(np.floor((cent - lo) / (hi - lo) * 1024.0), 0, 1023).astype(np.int64)
+    index_bits = max(1, int(len(faces) - 1).bit_length())
+    keys = sorted((_morton10(q) << index_bits) | t for t, q in enumerate(cells))
+    return points, faces, _deepest_internal_node(keys)
+
+
+def test_closest_points_on_a_deep_hierarchy(device, oracle):
+    points, faces, depth = _deep_hierarchy_mesh()
+    wide_levels = depth // 3 + 1
+    assert 7 * wide_levels - 6 > 96  # the query runs with its 128-entry stack
+    rng = np.random.default_rng(12)
+    queries = np.vstack([rng.uniform(-0.5, 1.5, size=(3000, 3)),        # at the deep cluster
+                         rng.uniform(-1.0, 520.0, size=(1000, 3))])     # along the chain
+    d = device.closest_points(points, faces, queries)
+    o = oracle.closest_points(points, faces, queries)
+    assert np.array_equal(d["triangles"], o["triangles"])
+    assert np.array_equal(d["distances"], o["distances"])
+    assert np.array_equal(d["points"], o["points"])
+
+
 def test_farthest_point_seeds_are_identical(device, oracle):
     rng = np.random.default_rng(4)
     points = rng.normal(size=(5000, 3))
