@@ -128,12 +128,15 @@ def run_form(dp, job, settings, kwargs, form: str, steps: int, warmup: int, prof
         "solves_per_s": steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
         "p50_ms": 1e3 * statistics.median(latencies),
+        "p95_ms": 1e3 * float(np.percentile(latencies, 95)),
         "p99_ms": 1e3 * float(np.percentile(latencies, 99)),
         "gn_solves_per_s": delta["gauss_newton_solves"] / elapsed,
         "counters": delta,
         "profiled_counters": p_delta,
         "stages": stages,
         "vertices": np.asarray(result["vertices"]),
+        "node_positions": np.asarray(result["node_positions"]),
+        "level_nodes": [int(n) for n in result["level_nodes"]],
         "report": report,
         "max_query_steps": p_after.get("query_max_steps", 0.0),
     }
@@ -188,8 +191,14 @@ def _oracle_sample(config: int, job, settings, kwargs) -> dict:
     wall = time.perf_counter() - t0
     report = json.loads(result["report"])
     seconds = wall if full else report["setup_seconds"] + OUTER * report["solve_seconds"]
+    # The template graph is built in full either way, so the finest level's
+    # nodes compare at every size; the final correspondences only after a
+    # complete wrap.
     return {"wall": wall, "seconds": seconds, "setup": report["setup_seconds"], "solve": report["solve_seconds"],
-            "vertices": np.asarray(result["vertices"]) if full else None}
+            "vertices": np.asarray(result["vertices"]) if full else None,
+            "node_positions": np.asarray(result["node_positions"]),
+            "level_nodes": [int(n) for n in result["level_nodes"]],
+            "correspondence": report.get("correspondence") if full else None}
 
 
 def _host_description() -> str:
@@ -242,7 +251,13 @@ def cpu_baseline(config: int, job, settings, kwargs, procs: int) -> dict:
         "one_core_solves_per_s": 1.0 / one["seconds"],
         "seconds_per_wrap_one_core": one["seconds"],
         "vertices": one["vertices"],
+        "node_positions": one["node_positions"],
+        "level_nodes": one["level_nodes"],
+        "correspondence": one["correspondence"],
     }
+
+
+CPU_ARRAYS = ("vertices", "node_positions", "level_nodes", "correspondence")
 
 
 def copy_calibration(mib: int) -> None:
@@ -333,6 +348,7 @@ def main(argv=None) -> int:
             "solves_per_s": round(r["solves_per_s"], 4),
             "ms_per_step": round(r["ms_per_step"], 3),
             "p50_ms": round(r["p50_ms"], 3),
+            "p95_ms": round(r["p95_ms"], 3),
             "p99_ms": round(r["p99_ms"], 3),
             "gn_solves_per_s": round(r["gn_solves_per_s"], 2),
             "pcg_iterations_per_solve": round(r["counters"]["pcg_iterations"] / max(r["counters"]["pcg_launches"], 1),
@@ -344,6 +360,20 @@ def main(argv=None) -> int:
         }
         if cpu is not None and cpu["vertices"] is not None:
             entry["vertex_rms_vs_oracle"], entry["vertex_max_vs_oracle"] = error(r["vertices"], cpu["vertices"])
+        if cpu is not None:
+            # Index parity: the finest level's nodes (FPS seeds) and the level
+            # sizes, and after a complete wrap the final correspondence
+            # counts (accepted and each rejection reason).
+            same_shape = r["node_positions"].shape == cpu["node_positions"].shape
+            entry["nodes_identical_to_oracle"] = bool(same_shape and np.array_equal(r["node_positions"],
+                                                                                    cpu["node_positions"]))
+            entry["level_nodes_equal_oracle"] = r["level_nodes"] == cpu["level_nodes"]
+            if cpu["correspondence"] is not None:
+                counts = ("total", "accepted", "rejected_distance", "rejected_normal", "rejected_slope",
+                          "rejected_masked")
+                dev = r["report"].get("correspondence", {})
+                entry["correspondence_counts_equal_oracle"] = all(dev.get(k) == cpu["correspondence"].get(k)
+                                                                  for k in counts)
         forms_out[name] = entry
     if "mixed" in results and "double" in results:
         rms, worst = error(results["mixed"]["vertices"], results["double"]["vertices"])
@@ -359,6 +389,7 @@ def main(argv=None) -> int:
         "warmup": args.warmup,
         "ms_per_step": round(head["ms_per_step"], 3),
         "p50_ms": round(head["p50_ms"], 3),
+        "p95_ms": round(head["p95_ms"], 3),
         "p99_ms": round(head["p99_ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -374,7 +405,7 @@ def main(argv=None) -> int:
         },
         "roofline": hbm_roofline(head, "pcg", "pcg_bytes", "pcg_kernel", PCG_TRAFFIC.get((args.config, forms[0]))),
         "cpu_baseline": None if cpu is None else {k: (round(v, 6) if isinstance(v, float) else v)
-                                                 for k, v in cpu.items() if k != "vertices"},
+                                                 for k, v in cpu.items() if k not in CPU_ARRAYS},
         "forms": forms_out,
     }
     if "double" in results and "mixed" in results:
