@@ -14,10 +14,17 @@ skips for a missing device or a missing extension: it fails.
 
 from __future__ import annotations
 
+import os
 import sys
 from pathlib import Path
 
 import pytest
+
+# Read once, when HIP initialises (before any test touches the device). With
+# the runtime's default of 4 hardware queues, concurrent jobs' streams share
+# queues and their kernels serialise, which hides ordering mistakes between
+# jobs; 16 lets the serving tests' jobs really overlap.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 ROOT = Path(__file__).resolve().parents[1]
 for sub in ("", "nricp_amd", "oracle", "tests"):
