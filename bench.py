@@ -10,7 +10,10 @@ of frequency 141 (198,812 vertices / 397,620 triangles) onto a frequency-200
 sphere (800,000 triangles) with 8 seeded RBF bumps and normal noise of 0.05 %
 of the box diagonal; 8,000 graph nodes over 3 levels, 8 graph neighbours, 10
 Gauss-Newton steps per level. --config 1 is the 50k capsule humanoid (2,000
-nodes, 4 levels), --config 0 the 10k icosphere (500 nodes, 3 levels).
+nodes, 4 levels), --config 0 the 10k icosphere (500 nodes, 3 levels), and
+--config 3 the 200k-vertex job built with the humanoid generator instead of
+spheres (203k-vertex template, 794k-triangle posed target with holes and the
+same normal noise, 8,000 nodes over 3 levels).
 
 Two device forms are timed: "mixed" (FP32 assembly blocks and PCG iterations,
 FP64 right-hand side, energies and refinement; the headline `value`) and
@@ -20,7 +23,7 @@ residual, so both stay on the FP64 oracle's trajectory.
 CPU baseline: the FP64 oracle (a single-threaded restatement of the reference's
 route A) on this host, run before the GPU is touched. For configs 0 and 1 it
 runs the whole wrap, and the device forms' vertex error against it is
-reported. For config 2 a whole wrap takes minutes, so it runs a bounded sample
+reported. For configs 2 and 3 a whole wrap takes minutes, so it runs a bounded sample
 -- the same job with one outer iteration per level -- and the wrap time is
 extrapolated as setup + 10 x the sample's solve time (the per-iteration cost is
 flat within a level); the device forms are then compared with each other. The
@@ -291,7 +294,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, help="0: icosphere 10k, 1: humanoid 50k, 2: geodesic 200k")
+    ap.add_argument("--config", type=int, default=2,
+                    help="0: icosphere 10k, 1: humanoid 50k, 2: geodesic 200k, 3: humanoid 200k")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--forms", default="mixed,double")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")

@@ -114,7 +114,7 @@ def gate_failures(metrics: dict, baseline: dict) -> list[str]:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--config", type=int, default=1, help="0: icosphere 10k, 1: humanoid 50k, 2: geodesic 200k")
+    ap.add_argument("--config", type=int, default=1, help="0: icosphere 10k, 1: humanoid 50k, 2: geodesic 200k, 3: humanoid 200k")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--forms", default="mixed,double")
     ap.add_argument("--json-out", default=None)

@@ -364,9 +364,11 @@ def sphere_job(template_frequency: int, target_frequency: int, seed: int, nodes_
 
 
 def humanoid_job(seed: int, template_vertices: int = 50_000, target_triangles: int = 120_000,
-                 nodes_final: int = 2000, graph_levels: int = 4, hole_fraction: float = 0.02) -> WrapJob:
+                 nodes_final: int = 2000, graph_levels: int = 4, hole_fraction: float = 0.02,
+                 normal_noise: float = 0.0, name: str = "humanoid") -> WrapJob:
     """Capsule humanoid template; posed + bumped + independently meshed target
-    with a fraction of its triangles deleted."""
+    with a fraction of its triangles deleted, and optionally Gaussian noise
+    along the target's normals (sigma a fraction of its diagonal)."""
     rng = np.random.default_rng(seed)
     body = humanoid(rng)
     tp, tf = mesh_body(body, template_vertices)
@@ -378,12 +380,15 @@ def humanoid_job(seed: int, template_vertices: int = 50_000, target_triangles: i
     truth = np.einsum("nij,nj->ni", rotations[seg], tp - rest_starts[seg]) + posed_starts[seg]
     gp, gf = mesh_body(posed, target_triangles // 2)
     gp = rbf_bumps(gp, vertex_normals(gp, gf), rng, amplitude=(0.005, 0.015))
+    if normal_noise > 0.0:
+        diag = bbox_diagonal(gp)
+        gp = gp + vertex_normals(gp, gf) * rng.normal(scale=normal_noise * diag, size=(len(gp), 1))
     keep = rng.random(len(gf)) >= hole_fraction
     gf = shuffle_triangles(gf[keep], rng)
     used = np.unique(gf)
     remap = -np.ones(len(gp), dtype=np.int64)
     remap[used] = np.arange(len(used))
-    return WrapJob("humanoid", tp, tf, gp[used], remap[gf], nodes_final, graph_levels, seed, truth)
+    return WrapJob(name, tp, tf, gp[used], remap[gf], nodes_final, graph_levels, seed, truth)
 
 
 def level_settings(levels: int, outer: int = 10, inner: int = 1, **weights) -> list[dict]:
@@ -409,7 +414,8 @@ def level_settings(levels: int, outer: int = 10, inner: int = 1, **weights) -> l
 
 def config_job(index: int, seed: int = 0) -> WrapJob:
     """The benchmark configurations by index (0: CPU case, 1: humanoid 50k,
-    2: 200k single job)."""
+    2: 200k single job on geodesic spheres, 3: the 200k single job with the
+    humanoid generator of configuration 1)."""
     if index == 0:
         return sphere_job(32, 64, seed, nodes_final=500, graph_levels=3, name="icosphere-10k")
     if index == 1:
@@ -417,6 +423,9 @@ def config_job(index: int, seed: int = 0) -> WrapJob:
     if index == 2:
         return sphere_job(141, 200, seed, nodes_final=8000, graph_levels=3, jitter=0.0,
                           normal_noise=5e-4, name="geodesic-200k")
+    if index == 3:
+        return humanoid_job(seed, template_vertices=200_000, target_triangles=800_000, nodes_final=8000,
+                            graph_levels=3, normal_noise=5e-4, name="humanoid-200k")
     raise ValueError(f"no single-job configuration {index}")
 
 
