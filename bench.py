@@ -17,8 +17,13 @@ same normal noise, 8,000 nodes over 3 levels).
 
 Two device forms are timed: "mixed" (FP32 assembly blocks and PCG iterations,
 FP64 right-hand side, energies and refinement; the headline `value`) and
-"double" (plain FP64). Both solve each system to a 1e-12 relative true
-residual, so both stay on the FP64 oracle's trajectory.
+"double" (plain FP64), through the same device schedule. Both solve each
+system to a 1e-8 relative true residual (the benchmark schedule's PCG
+tolerance; no iteration cap stops a solve short of it). The oracle solves
+each system directly; the line reports each form's vertex error against it
+and whether the final correspondence counts agree, so the tolerance's effect
+is visible (--pcg-tolerance 1e-12 keeps both forms on the oracle's
+accept/reject trajectory exactly, as the parity tests do).
 
 CPU baseline: the FP64 oracle (a single-threaded restatement of the reference's
 route A) on this host, run before the GPU is touched. For configs 0 and 1 it
@@ -57,15 +62,16 @@ for sub in ("", "nricp_amd", "oracle"):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
-TOLERANCE = 1e-12
-MAX_PCG = 5000
+TOLERANCE = 1e-8  # the benchmark schedule's PCG tolerance (--pcg-tolerance)
+MAX_PCG = 5000     # far above any solve's need: no solve stops short of the tolerance
 OUTER = 10
 # HBM bytes per pcg_kernel launch (FETCH_SIZE + WRITE_SIZE per dispatch, each
 # scaled by what the counter reads for a plain device copy of known size),
 # averaged over a wrap's launches: measured on the 200k-vertex job by the
-# counter passes recorded in profiles/pcg_kernel_traffic_geodesic-200k.md.
-# Other jobs report null.
-PCG_TRAFFIC = {(2, "mixed"): 4.322e7, (2, "double"): 6.790e7}
+# counter passes recorded in profiles/pcg_kernel_traffic_geodesic-200k.md, per
+# (config, form, PCG tolerance). Other jobs and tolerances report null.
+PCG_TRAFFIC = {(2, "mixed", 1e-8): 3.181e7, (2, "double", 1e-8): 5.236e7,
+               (2, "mixed", 1e-12): 4.322e7, (2, "double", 1e-12): 6.790e7}
 # Device-timed stages of a wrap (HIP events around each stage's launches).
 STAGES = ("fps", "graph", "target_bvh", "layout", "deform", "closest_points", "assemble", "pcg", "energy",
           "local_step")
@@ -85,9 +91,10 @@ def build_job(config: int, seed: int):
 
 
 def run_form(dp, job, settings, kwargs, form: str, steps: int, warmup: int, profiled: int = 2) -> dict:
-    route = "route_a" if form == "double" else "route_b"
+    # Both forms run the same device schedule (route B's session, route A
+    # semantics) to the same PCG tolerance; only the arithmetic differs.
     session = dp.RouteBSession(job.template_points, job.template_faces, job.target_points, job.target_faces,
-                               settings, route=route, precision="mixed" if form == "mixed" else "double",
+                               settings, route="route_b", precision="mixed" if form == "mixed" else "double",
                                pcg_tolerance=TOLERANCE, pcg_max_iterations=MAX_PCG, **kwargs)
     for w in range(warmup):
         session.reset()
@@ -290,6 +297,7 @@ def copy_calibration(mib: int) -> None:
 
 
 def main(argv=None) -> int:
+    global TOLERANCE
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -302,11 +310,15 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-procs", type=int, default=0,
                     help="oracle jobs run at once for the all-core CPU throughput (default: the usable cores, at "
                          "most 16)")
+    ap.add_argument("--pcg-tolerance", type=float, default=TOLERANCE,
+                    help="relative true-residual tolerance of every device solve (default 1e-8, the benchmark "
+                         "schedule's; 1e-12 keeps both forms on the FP64 oracle's accept/reject trajectory exactly)")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device three times: a plain copy of known size, against "
                          "which rocprofv3 FETCH_SIZE / WRITE_SIZE counts are scaled")
     args = ap.parse_args(argv)
+    TOLERANCE = args.pcg_tolerance
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent wrap jobs shard across "
@@ -358,7 +370,8 @@ def main(argv=None) -> int:
             "pcg_iterations_per_solve": round(r["counters"]["pcg_iterations"] / max(r["counters"]["pcg_launches"], 1),
                                               2),
             "stage_ms_per_profiled_wrap": {k: round(v["ms"], 3) for k, v in r["stages"].items()},
-            "roofline_pcg": hbm_roofline(r, "pcg", "pcg_bytes", "pcg_kernel", PCG_TRAFFIC.get((args.config, name))),
+            "roofline_pcg": hbm_roofline(r, "pcg", "pcg_bytes", "pcg_kernel",
+                                         PCG_TRAFFIC.get((args.config, name, TOLERANCE))),
             "roofline_assembly": hbm_roofline(r, "assemble", "assemble_bytes", "assembly (rows, node and pair blocks)"),
             "query": query_stats(r),
         }
@@ -407,7 +420,8 @@ def main(argv=None) -> int:
             "form": forms[0],
             "pcg_tolerance": TOLERANCE,
         },
-        "roofline": hbm_roofline(head, "pcg", "pcg_bytes", "pcg_kernel", PCG_TRAFFIC.get((args.config, forms[0]))),
+        "roofline": hbm_roofline(head, "pcg", "pcg_bytes", "pcg_kernel",
+                                 PCG_TRAFFIC.get((args.config, forms[0], TOLERANCE))),
         "cpu_baseline": None if cpu is None else {k: (round(v, 6) if isinstance(v, float) else v)
                                                  for k, v in cpu.items() if k not in CPU_ARRAYS},
         "forms": forms_out,
