@@ -55,12 +55,14 @@ __device__ inline void left_jacobian_inverse(d3 w, double* J) {
 
 // ------------------------------------------------------------ pass 1 -------
 
+// Block `block` of the vertex rows: its vertices' residual rows (and, with
+// Records, the scales, residuals and arms the assembly reuses), and the
+// block's energy partials. lds: 3 x waves doubles.
 template <bool Records>
-__global__ void __launch_bounds__(kThreads) vertex_rows(AssemblyInputs in, double* __restrict__ scale,
-                                                        double* __restrict__ resid, double* __restrict__ arm,
-                                                        double* __restrict__ partials) {
-    __shared__ double lds[3 * (kThreads / kWave)];
-    const int64_t v = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+__device__ inline void vertex_rows_block(const AssemblyInputs& in, double* __restrict__ scale,
+                                         double* __restrict__ resid, double* __restrict__ arm,
+                                         double* __restrict__ partials, int64_t block, double* lds) {
+    const int64_t v = block * kThreads + threadIdx.x;
     double e[3] = {0.0, 0.0, 0.0};
     if (v < in.vertex_count) {
         const d3 x = ld3(in.points, v);
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(kThreads) vertex_rows(AssemblyInputs in, doubl
     double out[3];
     block_sum<3, kThreads>(e, lds, out);
     if (threadIdx.x == 0) {
-        double* p = partials + 5 * static_cast<int64_t>(blockIdx.x);
+        double* p = partials + 5 * block;
         p[0] = out[0];
         p[1] = out[1];
         p[2] = 0.0;
@@ -127,9 +129,11 @@ __global__ void __launch_bounds__(kThreads) vertex_rows(AssemblyInputs in, doubl
 
 // ------------------------------------------------------------ pass 2 -------
 
-__global__ void __launch_bounds__(kThreads) graph_energy(AssemblyInputs in, double* __restrict__ partials) {
-    __shared__ double lds[2 * (kThreads / kWave)];
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+// Block `block` of the graph terms (smoothness per edge, anchor per node) and
+// its energy partials. lds: 2 x waves doubles.
+__device__ inline void graph_energy_block(const AssemblyInputs& in, double* __restrict__ partials, int64_t block,
+                                          double* lds) {
+    const int64_t i = block * kThreads + threadIdx.x;
     double e[2] = {0.0, 0.0};
     if (i < in.edge_count && in.w.smooth > 0.0) {
         const int64_t a = in.edges[2 * i], b = in.edges[2 * i + 1];
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(kThreads) graph_energy(AssemblyInputs in, doub
     double out[2];
     block_sum<2, kThreads>(e, lds, out);
     if (threadIdx.x == 0) {
-        double* p = partials + 5 * static_cast<int64_t>(blockIdx.x);
+        double* p = partials + 5 * block;
         p[0] = 0.0;
         p[1] = 0.0;
         p[2] = out[0];
@@ -159,6 +163,8 @@ __global__ void __launch_bounds__(kThreads) graph_energy(AssemblyInputs in, doub
     }
 }
 
+// The five energies from every block's partials, in a fixed order (thread t
+// adds blocks t, t + 256, ...; then the block sum).
 __global__ void __launch_bounds__(kThreads) reduce_partials(const double* __restrict__ partials, int blocks,
                                                             double* __restrict__ out) {
     __shared__ double lds[5 * (kThreads / kWave)];
@@ -169,6 +175,22 @@ __global__ void __launch_bounds__(kThreads) reduce_partials(const double* __rest
     block_sum<5, kThreads>(e, lds, s);
     if (threadIdx.x == 0)
         for (int k = 0; k < 5; ++k) out[k] = s[k];
+}
+
+// The rows and graph parts of an energy pass in one launch: blocks
+// [0, vblocks) are the vertex rows, the rest the graph terms, each writing
+// its own partials (reduce_partials folds them). Folding in the last block to
+// finish instead was measured slower: each block's agent-scope arrival writes
+// back and invalidates its XCD's L2.
+template <bool Records>
+__global__ void __launch_bounds__(kThreads)
+energy_rows(AssemblyInputs in, double* __restrict__ scale, double* __restrict__ resid, double* __restrict__ arm,
+            double* __restrict__ partials, int vblocks) {
+    __shared__ double lds[3 * (kThreads / kWave)];
+    if (static_cast<int>(blockIdx.x) < vblocks)
+        vertex_rows_block<Records>(in, scale, resid, arm, partials, blockIdx.x, lds);
+    else
+        graph_energy_block(in, partials + 5 * static_cast<int64_t>(vblocks), blockIdx.x - vblocks, lds);
 }
 
 // ------------------------------------------------------------ pass 3 -------
@@ -417,19 +439,18 @@ void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records,
     const int vblocks = std::max(1, blocks_for(in.vertex_count, kThreads));
     const int gblocks = std::max(1, blocks_for(std::max(in.edge_count, in.node_count), kThreads));
     s.partials.resize(5 * static_cast<int64_t>(vblocks + gblocks));
+    const int blocks = vblocks + gblocks;
     if (records) {
         s.vertex_scale.resize(3 * std::max<int64_t>(in.vertex_count, 1));
         s.vertex_resid.resize(7 * std::max<int64_t>(in.vertex_count, 1));
         s.vertex_arm.resize(12 * std::max<int64_t>(in.vertex_count, 1));
-        vertex_rows<true><<<vblocks, kThreads, 0, stream>>>(in, s.vertex_scale.get(), s.vertex_resid.get(),
-                                                             s.vertex_arm.get(), s.partials.get());
+        energy_rows<true><<<blocks, kThreads, 0, stream>>>(in, s.vertex_scale.get(), s.vertex_resid.get(),
+                                                            s.vertex_arm.get(), s.partials.get(), vblocks);
     } else {
-        vertex_rows<false><<<vblocks, kThreads, 0, stream>>>(in, nullptr, nullptr, nullptr, s.partials.get());
+        energy_rows<false><<<blocks, kThreads, 0, stream>>>(in, nullptr, nullptr, nullptr, s.partials.get(), vblocks);
     }
-    NRICP_LAUNCH_CHECK("vertex_rows");
-    graph_energy<<<gblocks, kThreads, 0, stream>>>(in, s.partials.get() + 5 * static_cast<int64_t>(vblocks));
-    NRICP_LAUNCH_CHECK("graph_energy");
-    reduce_partials<<<1, kThreads, 0, stream>>>(s.partials.get(), vblocks + gblocks, out);
+    NRICP_LAUNCH_CHECK("energy_rows");
+    reduce_partials<<<1, kThreads, 0, stream>>>(s.partials.get(), blocks, out);
     NRICP_LAUNCH_CHECK("reduce_partials");
 }
 
