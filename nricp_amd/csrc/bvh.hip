@@ -18,22 +18,22 @@ namespace {
 
 // --------------------------------------------------------------- normals ---
 
-__global__ void face_cross_kernel(const double* __restrict__ p, const int32_t* __restrict__ f, int64_t nf,
-                                  double* __restrict__ w) {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nf) return;
-    const d3 a = ld3(p, f[3 * t]), b = ld3(p, f[3 * t + 1]), c = ld3(p, f[3 * t + 2]);
-    st3(w, t, cross(b - a, c - a));
-}
-
-// Sum in increasing face order starting from zero, then normalise: the same
-// sequence of additions the CPU route performs per vertex.
-__global__ void gather_normals_kernel(const double* __restrict__ w, const int32_t* __restrict__ off,
-                                      const int32_t* __restrict__ faces, int64_t nv, double* __restrict__ out) {
+// Each vertex sums the cross products of its faces in increasing face order
+// starting from zero, then normalises: the same sequence of additions the CPU
+// route performs per vertex. A face's cross product is recomputed by each of
+// its vertices (same operations, same bits) rather than stored by a pass of
+// its own, which saves a launch per correspondence pass.
+__global__ void vertex_normals_kernel(const double* __restrict__ p, const int32_t* __restrict__ f,
+                                      const int32_t* __restrict__ off, const int32_t* __restrict__ faces, int64_t nv,
+                                      double* __restrict__ out) {
     const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     d3 s{0.0, 0.0, 0.0};
-    for (int32_t k = off[v]; k < off[v + 1]; ++k) s = s + ld3(w, faces[k]);
+    for (int32_t k = off[v]; k < off[v + 1]; ++k) {
+        const int64_t t = faces[k];
+        const d3 a = ld3(p, f[3 * t]), b = ld3(p, f[3 * t + 1]), c = ld3(p, f[3 * t + 2]);
+        s = s + cross(b - a, c - a);
+    }
     st3(out, v, unit(s));
 }
 
@@ -810,17 +810,12 @@ void build_vertex_faces(const int32_t* faces, int64_t nf, int64_t nv, VertexFace
     }
 }
 
-void vertex_normals(const double* points, const int32_t* faces, int64_t nf, const VertexFaces& vf, int64_t nv,
-                    double* face_scratch, double* normals, hipStream_t stream) {
-    if (nf > 0) {
-        face_cross_kernel<<<blocks_for(nf, 256), 256, 0, stream>>>(points, faces, nf, face_scratch);
-        NRICP_LAUNCH_CHECK("face_cross_kernel");
-    }
-    if (nv > 0) {
-        gather_normals_kernel<<<blocks_for(nv, 256), 256, 0, stream>>>(face_scratch, vf.offsets.get(),
-                                                                        vf.faces.get(), nv, normals);
-        NRICP_LAUNCH_CHECK("gather_normals_kernel");
-    }
+void vertex_normals(const double* points, const int32_t* faces, const VertexFaces& vf, int64_t nv, double* normals,
+                    hipStream_t stream) {
+    if (nv <= 0) return;
+    vertex_normals_kernel<<<blocks_for(nv, 256), 256, 0, stream>>>(points, faces, vf.offsets.get(), vf.faces.get(), nv,
+                                                                    normals);
+    NRICP_LAUNCH_CHECK("vertex_normals_kernel");
 }
 
 void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hipStream_t stream) {
@@ -830,9 +825,7 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     // Target vertex normals (face order per vertex, as the CPU route).
     build_vertex_faces(bvh.faces.get(), nt, nv, vf, scratch, stream);
     bvh.vertex_normals.resize(3 * nv);
-    bvh.face_cross.resize(3 * nt);
-    vertex_normals(bvh.points.get(), bvh.faces.get(), nt, vf, nv, bvh.face_cross.get(), bvh.vertex_normals.get(),
-                   stream);
+    vertex_normals(bvh.points.get(), bvh.faces.get(), vf, nv, bvh.vertex_normals.get(), stream);
     // Morton keys of the centroids, triangle index in the low bits.
     bvh.centroids.resize(3 * nt);
     centroid_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.points.get(), bvh.faces.get(), nt, bvh.centroids.get());

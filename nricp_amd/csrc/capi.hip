@@ -117,15 +117,14 @@ int nricp_hip_vertex_normals(const double* points, int64_t nv, const int32_t* fa
         hipStream_t st = op_stream();
         StreamScope scope(st);
         SortScratch sort;
-        DeviceBuffer<double> p, n, scratch;
+        DeviceBuffer<double> p, n;
         DeviceBuffer<int32_t> f;
         p.upload(points, 3 * nv, st);
         f.upload(faces, 3 * nf, st);
         VertexFaces vf;
         build_vertex_faces(f.get(), nf, nv, vf, sort, st);
         n.resize(3 * nv);
-        scratch.resize(3 * nf);
-        vertex_normals(p.get(), f.get(), nf, vf, nv, scratch.get(), n.get(), st);
+        vertex_normals(p.get(), f.get(), vf, nv, n.get(), st);
         n.download(out, 3 * nv, st);
         NRICP_CHECK(hipStreamSynchronize(st));
     });

@@ -25,8 +25,8 @@ struct VertexFaces {
 void build_vertex_faces(const int32_t* faces, int64_t face_count, int64_t vertex_count, VertexFaces& out,
                         SortScratch& scratch, hipStream_t stream);
 // Area-weighted unit vertex normals, summed per vertex in face order.
-void vertex_normals(const double* points, const int32_t* faces, int64_t face_count, const VertexFaces& vf,
-                    int64_t vertex_count, double* face_scratch /* 3F */, double* normals, hipStream_t stream);
+void vertex_normals(const double* points, const int32_t* faces, const VertexFaces& vf, int64_t vertex_count,
+                    double* normals, hipStream_t stream);
 
 // --------------------------------------------------------------- LBVH ------
 
@@ -77,7 +77,6 @@ struct Lbvh {
     DeviceBuffer<int32_t> level_runs;    // [start, end) of each depth in level_order
     DeviceBuffer<double> centroids;
     DeviceBuffer<double> bounds;      // partial boxes
-    DeviceBuffer<double> face_cross;  // scratch for the target normals
 };
 // Builds the hierarchy over a target mesh already on the device (points,
 // faces copied into the Lbvh). Morton codes (10 bits per axis) of the triangle

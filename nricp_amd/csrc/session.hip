@@ -296,7 +296,6 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         NRICP_LAUNCH_CHECK("scatter_landmarks");
         NRICP_CHECK(hipStreamSynchronize(st));
     }
-    face_scratch_.resize(3 * nf_);
     // Target mesh; its hierarchy is built by the first step, on the device.
     bvh_.triangle_count = pb.target_face_count;
     target_vertex_count_ = pb.target_vertex_count;
@@ -539,7 +538,7 @@ void Session::refresh() {
     clock_.begin("deform", st);
     evaluate_deformation(points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_,
                          rot_.get(), trans_.get(), L.node_positions.get(), L.node_count, deformed_.get(), st);
-    vertex_normals(deformed_.get(), faces_.get(), nf_, template_vf_, nv_, face_scratch_.get(), qnormals_.get(), st);
+    vertex_normals(deformed_.get(), faces_.get(), template_vf_, nv_, qnormals_.get(), st);
     clock_.end(st);
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, ls.robust_kernel,
                     ls.robust_threshold};
@@ -740,7 +739,7 @@ void Session::final_statistics() {
     const GraphLevel& L = graph_->levels[level_];
     evaluate_deformation(points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_,
                          rot_.get(), trans_.get(), L.node_positions.get(), L.node_count, deformed_.get(), st);
-    vertex_normals(deformed_.get(), faces_.get(), nf_, template_vf_, nv_, face_scratch_.get(), qnormals_.get(), st);
+    vertex_normals(deformed_.get(), faces_.get(), template_vf_, nv_, qnormals_.get(), st);
     QuerySettings q{settings_.distance_threshold, settings_.min_normal_cosine, settings_.min_slope, 0, 0.0};
     closest_points(bvh_, deformed_.get(), qnormals_.get(), has_mask_ ? mask_.get() : nullptr, nv_, q,
                    corr_points_.get(), corr_normals_.get(), corr_dist_.get(), corr_tri_.get(), rejection_.get(),

@@ -158,7 +158,6 @@ private:
     DeviceBuffer<uint8_t> landmark_flag_;
     DeviceBuffer<double> landmark_target_;
     VertexFaces template_vf_, target_vf_;
-    DeviceBuffer<double> face_scratch_;
     Lbvh bvh_;
     int64_t target_vertex_count_ = 0;
     DeviceGraph own_graph_;
