@@ -304,7 +304,7 @@ struct PcgWorkspace {
     int last_groups = 0;             // workgroups of the last launch
 };
 // Solves (A + lambda D) x = b with block-Jacobi preconditioned conjugate
-// gradients (Chronopoulos-Gear form: two grid barriers per iteration) in one
+// gradients (pipelined form: one grid barrier per iteration) in one
 // persistent launch. FP64 matrix. x is written to `solution`; the outcome is
 // left on the device in ws.outcome.
 void pcg_solve(const LevelLayout& layout, const double* bsr, const double* rhs, int64_t node_count,
