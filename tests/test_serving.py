@@ -56,6 +56,26 @@ def test_results_are_the_same_alone_and_in_the_mix(device):
 
 
 @pytest.mark.gpu
+def test_every_route_and_form_gives_the_same_bits_in_the_mix(device):
+    # Route A (FP64 to 1e-12), route B in both forms and route C (local and
+    # global steps) running side by side.
+    jobs = _jobs()[:4]
+    specs = [("route_a", "double"), ("route_b", "double"), ("route_c", "mixed"), ("route_c", "double")]
+    alone = []
+    for job, (route, form) in zip(jobs, specs):
+        solve = getattr(device, f"solve_{route}")
+        extra = {} if route == "route_a" else {"precision": form}
+        alone.append(solve(*_args(job), nodes_initial=job.nodes_initial, **extra, **KWARGS))
+    with device.Server(max_active=4) as server:
+        tickets = [server.submit(*_args(j), nodes_initial=j.nodes_initial, route=route, precision=form, **KWARGS)
+                   for j, (route, form) in zip(jobs, specs)]
+        mixed = [server.result(t) for t in tickets]
+    for job, spec, a, m in zip(jobs, specs, alone, mixed):
+        assert np.array_equal(a["vertices"], m["vertices"]), (job.name, spec)
+        assert json.loads(a["report"])["route"] == json.loads(m["report"])["route"] == spec[0]
+
+
+@pytest.mark.gpu
 def test_admission_keeps_reserved_memory_within_the_budget(device):
     jobs = _jobs()
     estimates = []
