@@ -150,12 +150,14 @@ __global__ void leaf_kernel(const uint64_t* __restrict__ keys, int64_t nt, uint6
     if (i >= nt) return;
     const int32_t t = static_cast<int32_t>(keys[i] & index_mask);
     leaf_triangle[i] = t;
+    double* c = corners + kLeafStride * i;
     for (int k = 0; k < 3; ++k) {
         const d3 v = ld3(p, f[3 * t + k]);
-        corners[9 * i + 3 * k] = v.x;
-        corners[9 * i + 3 * k + 1] = v.y;
-        corners[9 * i + 3 * k + 2] = v.z;
+        c[3 * k] = v.x;
+        c[3 * k + 1] = v.y;
+        c[3 * k + 2] = v.z;
     }
+    c[9] = static_cast<double>(t);  // exact: triangle indices are below 2^31
 }
 
 __global__ void key_index_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t index_mask,
@@ -223,7 +225,7 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int64_t n, BvhN
 __device__ inline void child_box(const BvhNode* __restrict__ nodes, const double* __restrict__ corners, int32_t c,
                                  float* lo, float* hi) {
     if (c < 0) {
-        const double* t = corners + 9 * static_cast<int64_t>(~c);
+        const double* t = corners + kLeafStride * static_cast<int64_t>(~c);
         for (int k = 0; k < 3; ++k) {
             lo[k] = __double2float_rd(fmin(fmin(t[k], t[3 + k]), t[6 + k]));
             hi[k] = __double2float_ru(fmax(fmax(t[k], t[3 + k]), t[6 + k]));
@@ -497,7 +499,7 @@ __device__ inline void rank_step(double d, int push, int j, int& rank) {
 
 template <int StackDepth>
 __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
-                                    const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
+                                    const double* __restrict__ corners,
                                     int32_t* stack_node, float* stack_dist, int j, Nearest& best) {
     int sp = 0;
     int32_t node = 0;
@@ -525,12 +527,16 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
         int32_t ctri = best.triangle, cleaf = best.leaf;
         if (ref != kWideEmpty && ref < 0 && !(d > best.d2 * kPruneSlack)) {
             const int32_t leaf = ~ref;
-            const double* c = corners + 9 * static_cast<int64_t>(leaf);
+            // The record (corners, then the triangle id) as five 16-byte
+            // loads: the id shares one with the last coordinate, so it
+            // arrives with the corners instead of a round trip later.
+            const double2* r = reinterpret_cast<const double2*>(corners + kLeafStride * static_cast<int64_t>(leaf));
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
             double bary[3];
-            const d3 q = closest_on_triangle(p, d3{c[0], c[1], c[2]}, d3{c[3], c[4], c[5]}, d3{c[6], c[7], c[8]},
+            const d3 q = closest_on_triangle(p, d3{r0.x, r0.y, r1.x}, d3{r1.y, r2.x, r2.y}, d3{r3.x, r3.y, r4.x},
                                              bary);
             const double d2 = norm2(q - p);
-            const int32_t tri = leaf_triangle[leaf];
+            const int32_t tri = static_cast<int32_t>(r4.y);
             if (d2 < cd2 || (d2 == cd2 && tri < ctri)) {
                 cd2 = d2;
                 ctri = tri;
@@ -649,8 +655,7 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
         best.triangle = seed;
     }
     if (wide_count > 0) {
-        nearest_wide<StackDepth>(q, wide, wide_count, corners, leaf_triangle, stack_node[group], stack_dist[group],
-                                 j, best);
+        nearest_wide<StackDepth>(q, wide, wide_count, corners, stack_node[group], stack_dist[group], j, best);
     } else if (nt == 1) {
         const double* c = corners;
         double bary[3];
@@ -846,7 +851,7 @@ void build_lbvh(Lbvh& bvh, int64_t nv, SortScratch& scratch, VertexFaces& vf, hi
     NRICP_LAUNCH_CHECK("morton_kernel");
     radix_sort_pairs(bvh.keys.get(), nullptr, nt, 30 + index_bits, scratch, stream);
     bvh.leaf_triangle.resize(nt);
-    bvh.leaf_corners.resize(9 * nt);
+    bvh.leaf_corners.resize(kLeafStride * nt);
     leaf_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(bvh.keys.get(), nt, (1ull << index_bits) - 1,
                                                           bvh.points.get(), bvh.faces.get(), bvh.leaf_triangle.get(),
                                                           bvh.leaf_corners.get());

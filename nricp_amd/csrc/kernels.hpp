@@ -55,6 +55,10 @@ struct WideNode {
 };
 static_assert(sizeof(WideNode) == 256, "wide node layout");
 
+// Per leaf: its triangle's corners a, b, c (9 doubles) and the triangle id
+// (a double, exact), 80 bytes, so the query loads them as one group.
+constexpr int64_t kLeafStride = 10;
+
 struct Lbvh {
     int64_t triangle_count = 0;
     int64_t wide_count = 0;
@@ -65,7 +69,7 @@ struct Lbvh {
     DeviceBuffer<BvhNode> nodes;        // T - 1 internal nodes, root = 0
     DeviceBuffer<int32_t> leaf_parent;  // T
     DeviceBuffer<int32_t> leaf_triangle;  // T: triangle id of each leaf (Morton order)
-    DeviceBuffer<double> leaf_corners;  // T x 9: a, b, c of each leaf
+    DeviceBuffer<double> leaf_corners;  // T x kLeafStride: a, b, c of each leaf, then its triangle id
     DeviceBuffer<int32_t> faces;        // T x 3 (original order)
     DeviceBuffer<double> vertex_normals;  // Vt x 3
     DeviceBuffer<double> points;          // Vt x 3
