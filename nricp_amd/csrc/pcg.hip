@@ -200,21 +200,23 @@ __device__ inline S precondition_row(const S (&mrow)[6], S v, int lane_base) {
 
 // This lane's share of one node's block row, fixed for the whole solve and
 // kept in registers: its first block (the row's block `lane`, or -1 past the
-// row's end) with that block's column, and the row's end (further blocks of
-// the lane are every eighth after the first).
+// row's end) with that block's column, the column of its second block (block
+// first + 8, when the row has one), and the row's end (further blocks of the
+// lane are every eighth after the second).
 struct RowShare {
-    int32_t first, column, end;
+    int32_t first, column, second_column, end;
 };
 
 __device__ inline RowShare row_share(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
                                      int64_t node, bool valid, int lane) {
-    RowShare r{-1, 0, 0};
+    RowShare r{-1, 0, 0, 0};
     if (valid) {
         r.end = row_off[node + 1];
         const int32_t b = row_off[node] + lane;
         if (b < r.end) {
             r.first = b;
             r.column = cols[b];
+            if (b + 8 < r.end) r.second_column = cols[b + 8];
         }
     }
     return r;
@@ -286,8 +288,9 @@ __device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* 
 // eight lanes together: lane j multiplies blocks j, j + 8, ... of the node's
 // block row (the full 6x6 block times the column node's 6-vector), and the
 // eight partial 6-vectors are summed across the lanes in a fixed order. The
-// lane's first block and its column come from registers, so a row of at most
-// eight blocks costs one round of loads. `self` is this lane's entry of u
+// column of the lane's first block (for an FP32 matrix, of its first two)
+// comes from registers, so a row of at most eight (sixteen) blocks costs no
+// dependent load. `self` is this lane's entry of u
 // (the row's own entry, which the lane holds) and `damp` its damping. Every
 // lane of the wave must call it (the sum crosses lanes); `valid` false adds
 // zeros. Returns row `lane` of the product (lanes 6 and 7 get row 0). Acc is
@@ -299,7 +302,14 @@ __device__ inline Acc node_product(const RowShare& share, const int32_t* __restr
     Acc acc[6] = {Acc(0), Acc(0), Acc(0), Acc(0), Acc(0), Acc(0)};
     if (share.first >= 0) {
         add_block_at<S, V, Acc>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
-        for (int32_t b = share.first + 8; b < share.end; b += 8)
+        int32_t b = share.first + 8;
+        // The FP64 kernels have no registers to spare for the second column.
+        if constexpr (sizeof(S) == 4) {
+            if (b < share.end)
+                add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(share.second_column), acc);
+            b += 8;
+        }
+        for (; b < share.end; b += 8)
             add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
     }
 #pragma unroll
