@@ -160,6 +160,13 @@ int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precis
                            double* out_pair_blocks, int64_t capacity, int64_t* out_pair_count,
                            double* out_energies);
 
+/* The local step of the local/global route on the given state: per node the
+ * weighted Horn fit of its bound vertices' correspondences, landmarks and the
+ * edges it starts, taken under-relaxed, R <- exp(relaxation log(R_fit R^T)) R
+ * (fit_rotations + fit_rotations_kernel, src/cuda/local_global.cu:188-302,536-564;
+ * the reference runs it with relaxation 0.2). Writes (N, 3, 3) rotations. */
+int nricp_hip_fit_rotations(const nricp_hip_energy_inputs* inputs, double relaxation, double* out_rotations);
+
 /* Damped solve of the block system (damped + solve_direct,
  * src/solver/normal_equations.cpp:81-96, src/solver/direct_solve.cpp:10-64;
  * binding solve_normal_equations, src/python/bindings.cpp:1355-1380), by

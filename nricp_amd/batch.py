@@ -49,19 +49,20 @@ def log(msg: str) -> None:
     print(f"[batch] {msg}", file=sys.stderr, flush=True)
 
 
-def _make_job(k: int, seed: int):
-    return workloads.batch_jobs(1, seed=seed + 7919 * k, kind="humanoid")[0]
+def _make_job(k: int, count: int, seed: int):
+    return workloads.batch_jobs(count, seed=seed, kind="humanoid", indices=[k])[0]
 
 
 def make_batch(count: int, seed: int) -> list:
-    """The batch, generated in worker processes before the GPU is touched."""
+    """The batch (workloads.batch_jobs: sizes log-uniform, job k seeded
+    seed + k), generated in worker processes before the GPU is touched."""
     procs = min(len(os.sched_getaffinity(0)), 16)
     if procs > 1:
         import multiprocessing as mp
 
         with mp.get_context("spawn").Pool(procs) as pool:
-            return pool.starmap(_make_job, [(k, seed) for k in range(count)])
-    return [_make_job(k, seed) for k in range(count)]
+            return pool.starmap(_make_job, [(k, count, seed) for k in range(count)])
+    return [_make_job(k, count, seed) for k in range(count)]
 
 
 def job_args(job):

@@ -586,20 +586,6 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
     }
 }
 
-__device__ inline double robust_weight(int kind, double s, double t) {
-    if (s < 0.0) s = 0.0;
-    if (kind == 1) {
-        if (t <= 0.0) return 1.0;
-        if (s <= t * t) return 1.0;
-        return t / sqrt(s);
-    }
-    if (kind == 2) {
-        if (t <= 0.0) return 1.0;
-        const double d = 1.0 + s / (t * t);
-        return 1.0 / (d * d);
-    }
-    return 1.0;
-}
 
 // Eight lanes per query: mask, nearest triangle, interpolated normal, the
 // distance / normal / slope filters in the CPU route's order
@@ -795,6 +781,14 @@ __global__ void pick_quantiles(const uint64_t* __restrict__ keys, int64_t accept
     out[1] = __longlong_as_double(static_cast<long long>(keys[accepted - 1]));
 }
 
+// The query kernel's IRLS weight, elementwise over given squared residuals.
+__global__ void robust_weights_kernel(int kind, const double* __restrict__ squared, int64_t n, double t,
+                                      double* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = robust_weight(kind, squared[i], t);
+}
+
 }  // namespace
 
 void build_vertex_faces(const int32_t* faces, int64_t nf, int64_t nv, VertexFaces& out, SortScratch& scratch,
@@ -984,6 +978,13 @@ void distance_quantiles(const uint8_t* rejection, const double* distance, int64_
     radix_sort_pairs(keys.get(), nullptr, count, 64, scratch, stream);
     pick_quantiles<<<1, 64, 0, stream>>>(keys.get(), accepted, out);
     NRICP_LAUNCH_CHECK("pick_quantiles");
+}
+
+void robust_weights(int kind, const double* squared, int64_t count, double threshold, double* out,
+                    hipStream_t stream) {
+    if (count <= 0) return;
+    robust_weights_kernel<<<blocks_for(count, 256), 256, 0, stream>>>(kind, squared, count, threshold, out);
+    NRICP_LAUNCH_CHECK("robust_weights_kernel");
 }
 
 }  // namespace nricp_hip

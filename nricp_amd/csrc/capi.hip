@@ -216,20 +216,20 @@ int nricp_hip_find_correspondences(const double* tp, int64_t tv, const int32_t* 
 }
 
 int nricp_hip_robust_weights(int32_t kernel, const double* s, int64_t n, double t, double* out) {
-    // Elementwise and tiny: evaluated where the data is (the host), with the
-    // same formulas the query kernel applies on the device.
+    // The device function the correspondence query applies (bvh.hip), so the
+    // weights returned here are the ones the solve uses.
     return guarded([&] {
         require(n >= 0 && (n == 0 || (s && out)), "robust_weights: bad arrays");
-        for (int64_t i = 0; i < n; ++i) {
-            double x = s[i] < 0.0 ? 0.0 : s[i];
-            double w = 1.0;
-            if (kernel == 1 && t > 0.0) w = x <= t * t ? 1.0 : t / std::sqrt(x);
-            if (kernel == 2 && t > 0.0) {
-                const double d = 1.0 + x / (t * t);
-                w = 1.0 / (d * d);
-            }
-            out[i] = w;
-        }
+        require(kernel >= 0 && kernel <= 2, "robust_weights: unknown kernel");
+        if (n == 0) return;
+        ensure_device();
+        hipStream_t st = op_stream();
+        StreamScope scope(st);
+        DeviceBuffer<double> squared, weights(n);
+        squared.upload(s, n, st);
+        robust_weights(kernel, squared.get(), n, t, weights.get(), st);
+        weights.download(out, n, st);
+        NRICP_CHECK(hipStreamSynchronize(st));
     });
 }
 
@@ -506,6 +506,26 @@ int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precis
             total += e[k];
         }
         out_energies[5] = total;
+    });
+}
+
+int nricp_hip_fit_rotations(const nricp_hip_energy_inputs* inputs, double relaxation, double* out_rotations) {
+    return guarded([&] {
+        require(inputs != nullptr && out_rotations != nullptr, "fit_rotations: null arguments");
+        require(relaxation >= 0.0, "fit_rotations: relaxation must be non-negative");
+        ensure_device();
+        hipStream_t st = op_stream();
+        StreamScope scope(st);
+        UploadedEnergy u;
+        upload_energy(*inputs, u, st);
+        const int64_t nn = inputs->node_count;
+        SortScratch sort;
+        LevelLayout layout;
+        build_level_layout(u.in.bind_nodes, u.in.bind_weights, u.in.bind_counts, u.in.vertex_count, u.in.edges,
+                           u.in.edge_count, nn, layout, sort, st);
+        fit_rotations(u.in, layout, relaxation, u.rot.get(), st);
+        u.rot.download(out_rotations, 9 * nn, st);
+        NRICP_CHECK(hipStreamSynchronize(st));
     });
 }
 

@@ -69,6 +69,26 @@ __device__ inline d3 mul3(const double* r, d3 v) {
             r[6] * v.x + r[7] * v.y + r[8] * v.z};
 }
 
+// IRLS weight of a squared residual s (robust_weight,
+// src/energy/robust.cpp:30-58): kind 1 Huber (1 inside the threshold t,
+// t / sqrt(s) outside), kind 2 Geman-McClure (1 / (1 + s / t^2)^2), anything
+// else or t <= 0 gives 1. The query kernel applies it to every accepted
+// correspondence; robust_weights_kernel exposes it elementwise.
+__device__ inline double robust_weight(int kind, double s, double t) {
+    if (s < 0.0) s = 0.0;
+    if (kind == 1) {
+        if (t <= 0.0) return 1.0;
+        if (s <= t * t) return 1.0;
+        return t / sqrt(s);
+    }
+    if (kind == 2) {
+        if (t <= 0.0) return 1.0;
+        const double d = 1.0 + s / (t * t);
+        return 1.0 / (d * d);
+    }
+    return 1.0;
+}
+
 // ----------------------------------------------------------- reductions ----
 
 // Wave-wide sums with DPP lane moves (register-to-register, no LDS round

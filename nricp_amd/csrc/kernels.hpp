@@ -131,6 +131,12 @@ void distance_quantiles(const uint8_t* rejection, const double* distance, int64_
                         DeviceBuffer<uint64_t>& keys, SortScratch& scratch, double* out /* p95, max */,
                         int64_t accepted, hipStream_t stream);
 
+// IRLS weights of squared residuals with the function the query kernel
+// applies to its accepted correspondences (kind 0 none, 1 Huber, 2
+// Geman-McClure; threshold <= 0 gives 1).
+void robust_weights(int kind, const double* squared, int64_t count, double threshold, double* out,
+                    hipStream_t stream);
+
 // ------------------------------------------------------- deformation -------
 
 // v' = sum over the binding's slots of w (R (v - g) + g + t), invalid nodes

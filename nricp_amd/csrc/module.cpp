@@ -366,6 +366,32 @@ EnergyArgs energy_args(const Dbl& tp, const Idx& bnode, const Dbl& bwt, const Id
     return a;
 }
 
+// The C ABI view of converted energy arguments (borrowed pointers into `a`).
+nricp_hip_energy_inputs energy_inputs(const EnergyArgs& a) {
+    nricp_hip_energy_inputs in{};
+    in.template_points = a.points.data();
+    in.vertex_count = a.nv;
+    in.bind_nodes = a.bn.data();
+    in.bind_weights = a.bw.data();
+    in.bind_counts = a.bc.data();
+    in.rotations = a.rot.data();
+    in.translations = a.trans.data();
+    in.node_positions = a.rest.data();
+    in.node_count = a.nn;
+    in.correspondence_points = a.cp.data();
+    in.correspondence_normals = a.cn.data();
+    in.correspondence_accepted = a.accepted.data();
+    in.correspondence_robust = a.robust.data();
+    in.edges = a.edges.data();
+    in.edge_count = static_cast<int64_t>(a.edges.size() / 2);
+    in.landmark_vertices = a.li.data();
+    in.landmark_targets = a.lt.data();
+    in.landmark_count = static_cast<int64_t>(a.li.size());
+    in.mask = a.mask.empty() ? nullptr : a.mask.data();
+    in.weights = a.w;
+    return in;
+}
+
 struct Usable {
     int count = 0;
     int node[4];
@@ -1521,27 +1547,7 @@ PYBIND11_MODULE(nricp_dp, m) {
                              const Idx& li, const Dbl& lt, const py::dict& w, const py::object& mk,
                              const std::string& precision) {
         const EnergyArgs a = energy_args(tp, bn, bw, bc, r, t, np, cp, cn, ca, cr, e, li, lt, w, mk);
-        nricp_hip_energy_inputs in{};
-        in.template_points = a.points.data();
-        in.vertex_count = a.nv;
-        in.bind_nodes = a.bn.data();
-        in.bind_weights = a.bw.data();
-        in.bind_counts = a.bc.data();
-        in.rotations = a.rot.data();
-        in.translations = a.trans.data();
-        in.node_positions = a.rest.data();
-        in.node_count = a.nn;
-        in.correspondence_points = a.cp.data();
-        in.correspondence_normals = a.cn.data();
-        in.correspondence_accepted = a.accepted.data();
-        in.correspondence_robust = a.robust.data();
-        in.edges = a.edges.data();
-        in.edge_count = static_cast<int64_t>(a.edges.size() / 2);
-        in.landmark_vertices = a.li.data();
-        in.landmark_targets = a.lt.data();
-        in.landmark_count = static_cast<int64_t>(a.li.size());
-        in.mask = a.mask.empty() ? nullptr : a.mask.data();
-        in.weights = a.w;
+        const nricp_hip_energy_inputs in = energy_inputs(a);
         const int64_t cap = 6 * a.nv + static_cast<int64_t>(a.edges.size() / 2) + 1;
         std::vector<double> diag(36 * a.nn), rhs(6 * a.nn), blocks(36 * cap), energies(6);
         std::vector<int32_t> pairs(2 * cap);
@@ -1557,6 +1563,24 @@ PYBIND11_MODULE(nricp_dp, m) {
           py::arg("correspondence_normals"), py::arg("correspondence_accepted"), py::arg("correspondence_robust"),
           py::arg("edges"), py::arg("landmark_indices"), py::arg("landmark_targets"), py::arg("weights"),
           py::arg("mask") = py::none(), py::arg("precision") = "double");
+
+    // The local step of the local/global route on a given state (no function
+    // of this name in the reference's module: its rotation fit runs inside
+    // the device session, src/cuda/local_global.cu:536-564).
+    m.def("fit_rotations", [](const Dbl& tp, const Idx& bn, const Dbl& bw, const Idx& bc, const Dbl& r, const Dbl& t,
+                              const Dbl& np, const Dbl& cp, const Dbl& cn, const Idx& ca, const Dbl& cr, const Idx& e,
+                              const Idx& li, const Dbl& lt, const py::dict& w, const py::object& mk, double relaxation) {
+        const EnergyArgs a = energy_args(tp, bn, bw, bc, r, t, np, cp, cn, ca, cr, e, li, lt, w, mk);
+        const nricp_hip_energy_inputs in = energy_inputs(a);
+        py::array_t<double> out({static_cast<py::ssize_t>(a.nn), static_cast<py::ssize_t>(3),
+                                 static_cast<py::ssize_t>(3)});
+        check(nricp_hip_fit_rotations(&in, relaxation, out.mutable_data()));
+        return out;
+    }, py::arg("template_points"), py::arg("binding_nodes"), py::arg("binding_weights"), py::arg("binding_counts"),
+          py::arg("rotations"), py::arg("translations"), py::arg("node_positions"), py::arg("correspondence_points"),
+          py::arg("correspondence_normals"), py::arg("correspondence_accepted"), py::arg("correspondence_robust"),
+          py::arg("edges"), py::arg("landmark_indices"), py::arg("landmark_targets"), py::arg("weights"),
+          py::arg("mask") = py::none(), py::arg("relaxation") = 0.2);
 
     m.def("solve_normal_equations", [](const Dbl& diag, const Idx& edges, const Dbl& blocks, const Dbl& rhs,
                                        double lambda, bool scaling, double tol, int max_iter,

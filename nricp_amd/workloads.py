@@ -430,17 +430,22 @@ def config_job(index: int, seed: int = 0) -> WrapJob:
 
 
 def batch_jobs(count: int = 64, seed: int = 1000, lo: int = 2000, hi: int = 50_000,
-               kind: str = "humanoid") -> list[WrapJob]:
+               kind: str = "humanoid", indices=None) -> list[WrapJob]:
     """Uneven batch: template sizes log-uniform in [lo, hi], target 2-3x the
     template's triangles, nodes ~ vertices / 25, one seed per job. "humanoid"
     generates each job as the 50k humanoid configuration does (4 levels);
     "sphere" uses geodesic spheres (3 levels), which are much faster to
-    generate."""
+    generate. ``indices`` selects jobs of the batch by position: every job's
+    size and seed are drawn first, so a selected job is the same job as in the
+    whole batch."""
     rng = np.random.default_rng(seed)
-    jobs = []
-    for k in range(count):
+    sizes = []
+    for _ in range(count):
         v = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
-        ratio = rng.uniform(2.0, 3.0)
+        sizes.append((v, rng.uniform(2.0, 3.0)))
+    jobs = []
+    for k in (range(count) if indices is None else indices):
+        v, ratio = sizes[k]
         if kind == "humanoid":
             nodes = max(8, int(round(v / 25)))
             job = humanoid_job(seed + k, template_vertices=int(v), target_triangles=int(2 * v * ratio),
@@ -455,3 +460,13 @@ def batch_jobs(count: int = 64, seed: int = 1000, lo: int = 2000, hi: int = 50_0
         else:
             raise ValueError(f"unknown batch kind {kind!r}")
     return jobs
+
+
+def batch_sizes(count: int = 64, seed: int = 1000, lo: int = 2000, hi: int = 50_000) -> list[float]:
+    """The template vertex counts batch_jobs draws, in job order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        out.append(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        rng.uniform(2.0, 3.0)
+    return out

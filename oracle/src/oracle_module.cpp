@@ -644,6 +644,22 @@ PYBIND11_MODULE(nricp_oracle, m) {
           py::arg("correspondence_normals"), py::arg("correspondence_accepted"), py::arg("correspondence_robust"),
           py::arg("edges"), py::arg("landmark_indices"), py::arg("landmark_targets"), py::arg("weights"),
           py::arg("mask") = py::none());
+    // The route-C local step alone (src/cuda/local_global.cu:188-302,536-564).
+    m.def("fit_rotations", [](const Dbl& tp, const Idx& bn, const Dbl& bw, const Idx& bc, const Dbl& r, const Dbl& t,
+                              const Dbl& np, const Dbl& cp, const Dbl& cn, const Idx& ca, const Dbl& cr, const Idx& e,
+                              const Idx& li, const Dbl& lt, const py::dict& w, const py::object& mk, double relaxation) {
+        EnergyCall c = energy_call(tp, bn, bw, bc, np, cp, cn, ca, cr, e, li, lt, w, mk);
+        c.in.mask = c.mask.empty() ? nullptr : &c.mask;
+        Deformation d = deformation(r, t, np);
+        fit_rotations(d, c.in, c.w, relaxation);
+        std::vector<M3> out(d.x.size());
+        for (size_t j = 0; j < d.x.size(); ++j) out[j] = d.x[j].r;
+        return from_mats(out);
+    }, py::arg("template_points"), py::arg("binding_nodes"), py::arg("binding_weights"), py::arg("binding_counts"),
+          py::arg("rotations"), py::arg("translations"), py::arg("node_positions"), py::arg("correspondence_points"),
+          py::arg("correspondence_normals"), py::arg("correspondence_accepted"), py::arg("correspondence_robust"),
+          py::arg("edges"), py::arg("landmark_indices"), py::arg("landmark_targets"), py::arg("weights"),
+          py::arg("mask") = py::none(), py::arg("relaxation") = 0.2);
     m.def("solve_normal_equations", [](const Dbl& diag, const Idx& e, const Dbl& blocks, const Dbl& rhs, double lam,
                                         bool scaling) {
         System s = system_of(diag, e, blocks, rhs);

@@ -71,8 +71,9 @@ def direct_solve(diag, edges, blocks, rhs, damping):
 
 # 40 and 100 nodes run in one workgroup (one and two node slots per lane
 # group); 1,500 and 3,000 take the grid kernels (24 groups of one slot, 24
-# groups of two).
-@pytest.mark.parametrize("n", [40, 100, 1500, 3000])
+# groups of two); 8,000 is the 200k-vertex job's finest level (63 groups of
+# two slots).
+@pytest.mark.parametrize("n", [40, 100, 1500, 3000, 8000])
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_rows_of_every_length_match_a_direct_solve(device, n, precision):
     diag, edges, blocks, rhs, degree = random_system(n, seed=n)
