@@ -153,8 +153,9 @@ typedef struct nricp_hip_energy_inputs {
 /* Fused system and energies (assemble_and_energy, src/solver/fused_assembly.cpp:159-300;
  * binding fused_system, src/python/bindings.cpp:1311-1353). Writes diagonal
  * (N, 6, 6), rhs (N, 6) = -J^T r, the touched pairs (a < b, ascending) with
- * their blocks J_a^T J_b, and energies[6] (five kinds, total). precision: 0
- * FP64 blocks, 1 FP32 blocks (mixed mode). */
+ * their blocks J_a^T J_b, and energies[6] (five kinds, total). precision:
+ * DOUBLE FP64 blocks, MIXED the FP64 blocks rounded to FP32, FLOAT blocks
+ * computed and accumulated in FP32 (returned widened to FP64). */
 int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precision,
                            double* out_diagonal, double* out_rhs, int32_t* out_pairs,
                            double* out_pair_blocks, int64_t capacity, int64_t* out_pair_count,
@@ -170,8 +171,10 @@ int nricp_hip_fit_rotations(const nricp_hip_energy_inputs* inputs, double relaxa
 /* Damped solve of the block system (damped + solve_direct,
  * src/solver/normal_equations.cpp:81-96, src/solver/direct_solve.cpp:10-64;
  * binding solve_normal_equations, src/python/bindings.cpp:1355-1380), by
- * block-Jacobi PCG to `tolerance` (relative residual). out_info[4] =
- * iterations, relative residual, converged, breakdown. */
+ * block-Jacobi PCG to `tolerance` (relative residual). precision: DOUBLE,
+ * MIXED (FP32 iterations on the blocks rounded to FP32, refinement against the
+ * given FP64 blocks) or FLOAT (the blocks rounded to FP32 are the system).
+ * out_info[4] = iterations, relative residual, converged, breakdown. */
 int nricp_hip_solve_system(const double* diagonal, int64_t node_count, const int32_t* pairs,
                            const double* pair_blocks, int64_t pair_count, const double* rhs,
                            double damping, int32_t diagonal_scaling, int32_t precision,
@@ -214,8 +217,20 @@ typedef struct nricp_hip_level_settings {
 #define NRICP_HIP_ROUTE_A 0  /* FP64 Gauss-Newton (route A semantics) */
 #define NRICP_HIP_ROUTE_B 1  /* device Gauss-Newton (route B) */
 #define NRICP_HIP_ROUTE_C 2  /* local/global: Horn fits + translation solve (route C) */
+/* Arithmetic of the Gauss-Newton system (no reference counterpart; the
+ * reference's route B is all-FP32, src/cuda/assemble.cu:164-311,
+ * src/cuda/pcg_solver.cu:63-241):
+ *   DOUBLE  FP64 assembly and PCG.
+ *   MIXED   FP64 assembly, stored in FP64 and rounded to FP32; PCG iterations
+ *           in FP32 on the FP32 copy, FP64 refinement whose residual uses the
+ *           FP64 matrix, so a solve converges to the FP64 system.
+ *   FLOAT   assembly blocks computed and accumulated in FP32, PCG iterations
+ *           in FP32, FP64 refinement of that FP32 system's residual.
+ * The right-hand side, energies, decisions and rotation fits are FP64 in all
+ * three. */
 #define NRICP_HIP_PRECISION_DOUBLE 0
 #define NRICP_HIP_PRECISION_MIXED 1
+#define NRICP_HIP_PRECISION_FLOAT 2
 
 /* RouteSettings + GraphSettings (src/solver/route_a.h:28-43, src/core/graph.h:11-16). */
 typedef struct nricp_hip_settings {

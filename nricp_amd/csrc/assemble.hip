@@ -199,54 +199,83 @@ energy_rows(AssemblyInputs in, double* __restrict__ scale, double* __restrict__ 
 __device__ constexpr int U(int r, int c) { return r * 6 - (r * (r - 1)) / 2 + (c - r); }
 constexpr int kUpper = 21;
 
-__device__ inline void add_offset_rows(double* acc, double c, d3 a, const double* r) {
-    const double c2 = c * c;
-    const double aa = dot(a, a);
-    const double av[3] = {a.x, a.y, a.z};
+// Block part (Acc arithmetic) and right-hand-side part (FP64) of a 3-row
+// offset block with arm a, scale c and residual r.
+template <typename Acc>
+__device__ inline void add_offset_block(Acc* blk, Acc c, const Acc (&av)[3]) {
+    const Acc c2 = c * c;
+    const Acc aa = (av[0] * av[0] + av[1] * av[1]) + av[2] * av[2];
     for (int x = 0; x < 3; ++x)
-        for (int y = x; y < 3; ++y) acc[U(x, y)] += c2 * ((x == y ? aa : 0.0) - av[x] * av[y]);
+        for (int y = x; y < 3; ++y) blk[U(x, y)] += c2 * ((x == y ? aa : Acc(0)) - av[x] * av[y]);
     // [a]x = [[0,-az,ay],[az,0,-ax],[-ay,ax,0]]
-    acc[U(0, 4)] += c2 * -a.z;
-    acc[U(0, 5)] += c2 * a.y;
-    acc[U(1, 3)] += c2 * a.z;
-    acc[U(1, 5)] += c2 * -a.x;
-    acc[U(2, 3)] += c2 * -a.y;
-    acc[U(2, 4)] += c2 * a.x;
-    acc[U(3, 3)] += c2;
-    acc[U(4, 4)] += c2;
-    acc[U(5, 5)] += c2;
-    const d3 rv{r[0], r[1], r[2]};
-    const d3 m = cross(a, rv);
-    acc[kUpper + 0] -= c * m.x;
-    acc[kUpper + 1] -= c * m.y;
-    acc[kUpper + 2] -= c * m.z;
-    acc[kUpper + 3] -= c * rv.x;
-    acc[kUpper + 4] -= c * rv.y;
-    acc[kUpper + 5] -= c * rv.z;
+    blk[U(0, 4)] += c2 * -av[2];
+    blk[U(0, 5)] += c2 * av[1];
+    blk[U(1, 3)] += c2 * av[2];
+    blk[U(1, 5)] += c2 * -av[0];
+    blk[U(2, 3)] += c2 * -av[1];
+    blk[U(2, 4)] += c2 * av[0];
+    blk[U(3, 3)] += c2;
+    blk[U(4, 4)] += c2;
+    blk[U(5, 5)] += c2;
 }
 
-__device__ inline void add_plane_row(double* acc, double c, d3 a, d3 n, double value) {
+__device__ inline void add_offset_rhs(double* rhs, double c, d3 a, const double* r) {
+    const d3 rv{r[0], r[1], r[2]};
+    const d3 m = cross(a, rv);
+    rhs[0] -= c * m.x;
+    rhs[1] -= c * m.y;
+    rhs[2] -= c * m.z;
+    rhs[3] -= c * rv.x;
+    rhs[4] -= c * rv.y;
+    rhs[5] -= c * rv.z;
+}
+
+template <typename Acc>
+__device__ inline void add_offset_rows(Acc* blk, double* rhs, double c, d3 a, const double* r) {
+    const Acc av[3] = {static_cast<Acc>(a.x), static_cast<Acc>(a.y), static_cast<Acc>(a.z)};
+    add_offset_block<Acc>(blk, static_cast<Acc>(c), av);
+    add_offset_rhs(rhs, c, a, r);
+}
+
+template <typename Acc>
+__device__ inline void add_plane_row(Acc* blk, double* rhs, double c, d3 a, d3 n, double value) {
     const d3 an = cross(a, n);
     const double q[6] = {c * an.x, c * an.y, c * an.z, c * n.x, c * n.y, c * n.z};
-    for (int x = 0; x < 6; ++x) {
-        for (int y = x; y < 6; ++y) acc[U(x, y)] += q[x] * q[y];
-        acc[kUpper + x] -= q[x] * value;
+    if constexpr (sizeof(Acc) == sizeof(double)) {
+        for (int x = 0; x < 6; ++x)
+            for (int y = x; y < 6; ++y) blk[U(x, y)] += q[x] * q[y];
+    } else {
+        // The Jacobian row in FP32: arm, normal and scale rounded, then the
+        // cross product and the outer product in FP32.
+        const Acc cf = static_cast<Acc>(c);
+        const Acc af[3] = {static_cast<Acc>(a.x), static_cast<Acc>(a.y), static_cast<Acc>(a.z)};
+        const Acc nf[3] = {static_cast<Acc>(n.x), static_cast<Acc>(n.y), static_cast<Acc>(n.z)};
+        const Acc qf[6] = {cf * (af[1] * nf[2] - af[2] * nf[1]), cf * (af[2] * nf[0] - af[0] * nf[2]),
+                           cf * (af[0] * nf[1] - af[1] * nf[0]), cf * nf[0], cf * nf[1], cf * nf[2]};
+        for (int x = 0; x < 6; ++x)
+            for (int y = x; y < 6; ++y) blk[U(x, y)] += qf[x] * qf[y];
     }
+    for (int x = 0; x < 6; ++x) rhs[x] -= q[x] * value;
 }
 
 // Blocks = false: the right-hand side only, with its rotation half written as
-// zero (the translation system of the local/global route).
-template <typename Scalar, bool Blocks>
+// zero (the translation system of the local/global route). The block is
+// accumulated in Acc (FP64, or FP32 for the all-FP32 assembly), the
+// right-hand side always in FP64. `copy32`, when given, receives an FP32
+// copy of the (FP64) block.
+template <typename Scalar, bool Blocks, typename Acc>
 __global__ void __launch_bounds__(kThreads)
 node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
             const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
             const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
-            Scalar* __restrict__ bsr, double* __restrict__ rhs) {
+            Scalar* __restrict__ bsr, double* __restrict__ rhs, float* __restrict__ copy32) {
     const int lane = threadIdx.x % kWave;
     const int64_t node = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
     if (node >= in.node_count) return;
-    double acc[kUpper + 6];
-    for (int k = 0; k < kUpper + 6; ++k) acc[k] = 0.0;
+    Acc blk[kUpper];
+    double g[6];
+    for (int k = 0; k < kUpper; ++k) blk[k] = Acc(0);
+    for (int k = 0; k < 6; ++k) g[k] = 0.0;
     for (int32_t k = goff[node] + lane; k < goff[node + 1]; k += kWave) {
         const int32_t ent = gent[k];
         const int64_t v = ent >> 2;
@@ -254,9 +283,9 @@ node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
         const double w = in.bind_weights[4 * v + s];
         const d3 a = ld3(arm, 4 * v + s);
         const double sp = scale[3 * v], sl = scale[3 * v + 1], sm = scale[3 * v + 2];
-        if (sp >= 0.0) add_offset_rows(acc, sp * w, a, resid + 7 * v);
-        if (sl >= 0.0) add_plane_row(acc, sl * w, a, ld3(in.corr_normals, v), resid[7 * v + 3]);
-        if (sm >= 0.0) add_offset_rows(acc, sm * w, a, resid + 7 * v + 4);
+        if (sp >= 0.0) add_offset_rows<Acc>(blk, g, sp * w, a, resid + 7 * v);
+        if (sl >= 0.0) add_plane_row<Acc>(blk, g, sl * w, a, ld3(in.corr_normals, v), resid[7 * v + 3]);
+        if (sm >= 0.0) add_offset_rows<Acc>(blk, g, sm * w, a, resid + 7 * v + 4);
     }
     if (in.w.smooth > 0.0) {
         const double sc = sqrt(in.w.smooth);
@@ -270,15 +299,15 @@ node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
                              apply_node(in.rotations, in.translations, in.rest, b, gb)) * sc;
             const double r[3] = {diff.x, diff.y, diff.z};
             if ((ent & 1) == 0) {
-                add_offset_rows(acc, sc, mul3(in.rotations + 9 * a, gb - ga), r);
+                add_offset_rows<Acc>(blk, g, sc, mul3(in.rotations + 9 * a, gb - ga), r);
             } else {
-                const double s2 = sc * sc;
-                acc[U(3, 3)] += s2;
-                acc[U(4, 4)] += s2;
-                acc[U(5, 5)] += s2;
-                acc[kUpper + 3] += sc * r[0];
-                acc[kUpper + 4] += sc * r[1];
-                acc[kUpper + 5] += sc * r[2];
+                const Acc s2 = static_cast<Acc>(sc * sc);
+                blk[U(3, 3)] += s2;
+                blk[U(4, 4)] += s2;
+                blk[U(5, 5)] += s2;
+                g[3] += sc * r[0];
+                g[4] += sc * r[1];
+                g[5] += sc * r[2];
             }
         }
     }
@@ -293,74 +322,88 @@ node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             for (int y = x; y < 3; ++y) {
                 double s = 0.0;
                 for (int r = 0; r < 3; ++r) s += J[r * 3 + x] * J[r * 3 + y];
-                acc[U(x, y)] += sc2 * s;
+                blk[U(x, y)] += static_cast<Acc>(sc2 * s);
             }
-            double g = 0.0;
-            for (int r = 0; r < 3; ++r) g += J[r * 3 + x] * wv[r];
-            acc[kUpper + x] -= sc2 * g;
+            double gr = 0.0;
+            for (int r = 0; r < 3; ++r) gr += J[r * 3 + x] * wv[r];
+            g[x] -= sc2 * gr;
         }
         const d3 t = ld3(in.translations, node);
-        acc[U(3, 3)] += sc2;
-        acc[U(4, 4)] += sc2;
-        acc[U(5, 5)] += sc2;
-        acc[kUpper + 3] -= sc2 * t.x;
-        acc[kUpper + 4] -= sc2 * t.y;
-        acc[kUpper + 5] -= sc2 * t.z;
+        blk[U(3, 3)] += static_cast<Acc>(sc2);
+        blk[U(4, 4)] += static_cast<Acc>(sc2);
+        blk[U(5, 5)] += static_cast<Acc>(sc2);
+        g[3] -= sc2 * t.x;
+        g[4] -= sc2 * t.y;
+        g[5] -= sc2 * t.z;
     }
-    for (int k = 0; k < kUpper + 6; ++k) acc[k] = wave_sum(acc[k]);
+    for (int k = 0; k < kUpper; ++k) blk[k] = wave_sum(blk[k]);
+    for (int k = 0; k < 6; ++k) g[k] = wave_sum(g[k]);
     if (lane == 0) {
         if (Blocks) {
             Scalar* out = bsr + 36 * static_cast<int64_t>(diag_pos[node]);
             for (int x = 0; x < 6; ++x)
                 for (int y = 0; y < 6; ++y)
-                    out[6 * x + y] = static_cast<Scalar>(x <= y ? acc[U(x, y)] : acc[U(y, x)]);
+                    out[6 * x + y] = static_cast<Scalar>(x <= y ? blk[U(x, y)] : blk[U(y, x)]);
+            if (copy32) {
+                float* c = copy32 + 36 * static_cast<int64_t>(diag_pos[node]);
+                for (int x = 0; x < 6; ++x)
+                    for (int y = 0; y < 6; ++y) c[6 * x + y] = static_cast<float>(x <= y ? blk[U(x, y)] : blk[U(y, x)]);
+            }
         }
-        for (int x = 0; x < 6; ++x) rhs[6 * node + x] = (Blocks || x >= 3) ? acc[kUpper + x] : 0.0;
+        for (int x = 0; x < 6; ++x) rhs[6 * node + x] = (Blocks || x >= 3) ? g[x] : 0.0;
     }
 }
 
 // ------------------------------------------------------------ pass 4 -------
 
-__device__ inline void add_pair_offset(double* acc, double clo, d3 alo, double chi, d3 ahi) {
-    const double cc = clo * chi;
-    const double lo[3] = {alo.x, alo.y, alo.z}, hi[3] = {ahi.x, ahi.y, ahi.z};
-    const double d = dot(alo, ahi);
+template <typename Acc>
+__device__ inline void add_pair_offset(Acc* acc, Acc clo, const Acc (&lo)[3], Acc chi, const Acc (&hi)[3]) {
+    const Acc cc = clo * chi;
+    const Acc d = (lo[0] * hi[0] + lo[1] * hi[1]) + lo[2] * hi[2];
     for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) acc[6 * x + y] += cc * ((x == y ? d : 0.0) - hi[x] * lo[y]);
+        for (int y = 0; y < 3; ++y) acc[6 * x + y] += cc * ((x == y ? d : Acc(0)) - hi[x] * lo[y]);
     // top right [alo]x, bottom left [ahi]x^T
-    acc[6 * 0 + 4] += cc * -alo.z;
-    acc[6 * 0 + 5] += cc * alo.y;
-    acc[6 * 1 + 3] += cc * alo.z;
-    acc[6 * 1 + 5] += cc * -alo.x;
-    acc[6 * 2 + 3] += cc * -alo.y;
-    acc[6 * 2 + 4] += cc * alo.x;
-    acc[6 * 4 + 0] += cc * -ahi.z;
-    acc[6 * 5 + 0] += cc * ahi.y;
-    acc[6 * 3 + 1] += cc * ahi.z;
-    acc[6 * 5 + 1] += cc * -ahi.x;
-    acc[6 * 3 + 2] += cc * -ahi.y;
-    acc[6 * 4 + 2] += cc * ahi.x;
+    acc[6 * 0 + 4] += cc * -lo[2];
+    acc[6 * 0 + 5] += cc * lo[1];
+    acc[6 * 1 + 3] += cc * lo[2];
+    acc[6 * 1 + 5] += cc * -lo[0];
+    acc[6 * 2 + 3] += cc * -lo[1];
+    acc[6 * 2 + 4] += cc * lo[0];
+    acc[6 * 4 + 0] += cc * -hi[2];
+    acc[6 * 5 + 0] += cc * hi[1];
+    acc[6 * 3 + 1] += cc * hi[2];
+    acc[6 * 5 + 1] += cc * -hi[0];
+    acc[6 * 3 + 2] += cc * -hi[1];
+    acc[6 * 4 + 2] += cc * hi[0];
     acc[6 * 3 + 3] += cc;
     acc[6 * 4 + 4] += cc;
     acc[6 * 5 + 5] += cc;
 }
 
+template <typename Acc>
+__device__ inline void acc3(Acc (&out)[3], d3 v) {
+    out[0] = static_cast<Acc>(v.x);
+    out[1] = static_cast<Acc>(v.y);
+    out[2] = static_cast<Acc>(v.z);
+}
+
 // Eight lanes per interacting pair: the lanes split the pair's entries, then
-// sum their 36 partial block entries with shuffles in a fixed order.
+// sum their 36 partial block entries with shuffles in a fixed order. The
+// block is accumulated in Acc (FP64, or FP32 for the all-FP32 assembly);
+// `copy32`, when given, receives an FP32 copy of it.
 constexpr int kPairLanes = 8;
 
-template <typename Scalar>
+template <typename Scalar, typename Acc>
 __global__ void __launch_bounds__(kThreads)
-pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
-            const double* __restrict__ arm, int64_t groups, const int32_t* __restrict__ goff,
-            const int32_t* __restrict__ gent, const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
-            int32_t* __restrict__ touched) {
+pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ arm, int64_t groups,
+            const int32_t* __restrict__ goff, const int32_t* __restrict__ gent, const int32_t* __restrict__ pos,
+            Scalar* __restrict__ bsr, int32_t* __restrict__ touched, float* __restrict__ copy32) {
     const int sub = threadIdx.x % kPairLanes;
     const int64_t g = (static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kPairLanes;
     const bool valid = g < groups;
-    double acc[36];
+    Acc acc[36];
 #pragma unroll
-    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+    for (int k = 0; k < 36; ++k) acc[k] = Acc(0);
     int hit = 0;
     const int32_t end = valid ? goff[g + 1] : 0;
     for (int32_t k = valid ? goff[g] + sub : 0; k < end; k += kPairLanes) {
@@ -369,19 +412,23 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             const int64_t v = ent >> 4;
             const int slo = (ent >> 2) & 3, shi = ent & 3;
             const double wlo = in.bind_weights[4 * v + slo], whi = in.bind_weights[4 * v + shi];
-            const d3 alo = ld3(arm, 4 * v + slo), ahi = ld3(arm, 4 * v + shi);
+            Acc alo[3], ahi[3];
+            acc3(alo, ld3(arm, 4 * v + slo));
+            acc3(ahi, ld3(arm, 4 * v + shi));
             const double sp = scale[3 * v], sl = scale[3 * v + 1], sm = scale[3 * v + 2];
             if (sp >= 0.0) {
                 hit = 1;
-                add_pair_offset(acc, sp * wlo, alo, sp * whi, ahi);
+                add_pair_offset<Acc>(acc, static_cast<Acc>(sp * wlo), alo, static_cast<Acc>(sp * whi), ahi);
             }
             if (sl >= 0.0) {
                 hit = 1;
-                const d3 n = ld3(in.corr_normals, v);
-                const d3 nlo = cross(alo, n), nhi = cross(ahi, n);
-                const double cl = sl * wlo, ch = sl * whi;
-                const double ql[6] = {cl * nlo.x, cl * nlo.y, cl * nlo.z, cl * n.x, cl * n.y, cl * n.z};
-                const double qh[6] = {ch * nhi.x, ch * nhi.y, ch * nhi.z, ch * n.x, ch * n.y, ch * n.z};
+                Acc n[3];
+                acc3(n, ld3(in.corr_normals, v));
+                const Acc cl = static_cast<Acc>(sl * wlo), ch = static_cast<Acc>(sl * whi);
+                const Acc ql[6] = {cl * (alo[1] * n[2] - alo[2] * n[1]), cl * (alo[2] * n[0] - alo[0] * n[2]),
+                                   cl * (alo[0] * n[1] - alo[1] * n[0]), cl * n[0], cl * n[1], cl * n[2]};
+                const Acc qh[6] = {ch * (ahi[1] * n[2] - ahi[2] * n[1]), ch * (ahi[2] * n[0] - ahi[0] * n[2]),
+                                   ch * (ahi[0] * n[1] - ahi[1] * n[0]), ch * n[0], ch * n[1], ch * n[2]};
 #pragma unroll
                 for (int x = 0; x < 6; ++x)
 #pragma unroll
@@ -389,14 +436,15 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             }
             if (sm >= 0.0) {
                 hit = 1;
-                add_pair_offset(acc, sm * wlo, alo, sm * whi, ahi);
+                add_pair_offset<Acc>(acc, static_cast<Acc>(sm * wlo), alo, static_cast<Acc>(sm * whi), ahi);
             }
         } else if (in.w.smooth > 0.0) {
             const int64_t e = ~ent;
             const int64_t a = in.edges[2 * e], b = in.edges[2 * e + 1];
             hit = 1;
-            const double s2 = in.w.smooth;
-            const d3 rot = mul3(in.rotations + 9 * a, ld3(in.rest, b) - ld3(in.rest, a));
+            const Acc s2 = static_cast<Acc>(in.w.smooth);
+            Acc rot[3];
+            acc3(rot, mul3(in.rotations + 9 * a, ld3(in.rest, b) - ld3(in.rest, a)));
             // J_a J_b^T: columns 3..5 hold -s^2 [ [rot]x ; I ]; transposed
             // into the (lower, higher) block when a > b.
             const bool fwd = a < b;
@@ -407,12 +455,12 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
         else                                     \
             acc[6 * (c) + (r)] += (v);           \
     } while (0)
-            NRICP_PAIR_ADD(0, 4, -s2 * -rot.z);
-            NRICP_PAIR_ADD(0, 5, -s2 * rot.y);
-            NRICP_PAIR_ADD(1, 3, -s2 * rot.z);
-            NRICP_PAIR_ADD(1, 5, -s2 * -rot.x);
-            NRICP_PAIR_ADD(2, 3, -s2 * -rot.y);
-            NRICP_PAIR_ADD(2, 4, -s2 * rot.x);
+            NRICP_PAIR_ADD(0, 4, -s2 * -rot[2]);
+            NRICP_PAIR_ADD(0, 5, -s2 * rot[1]);
+            NRICP_PAIR_ADD(1, 3, -s2 * rot[2]);
+            NRICP_PAIR_ADD(1, 5, -s2 * -rot[0]);
+            NRICP_PAIR_ADD(2, 3, -s2 * -rot[1]);
+            NRICP_PAIR_ADD(2, 4, -s2 * rot[0]);
             NRICP_PAIR_ADD(3, 3, -s2);
             NRICP_PAIR_ADD(4, 4, -s2);
             NRICP_PAIR_ADD(5, 5, -s2);
@@ -432,6 +480,17 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
             ab[6 * x + y] = static_cast<Scalar>(acc[6 * x + y]);
             ba[6 * y + x] = static_cast<Scalar>(acc[6 * x + y]);
         }
+    if (copy32) {
+        float* cab = copy32 + 36 * static_cast<int64_t>(pos[2 * g]);
+        float* cba = copy32 + 36 * static_cast<int64_t>(pos[2 * g + 1]);
+#pragma unroll
+        for (int x = 0; x < 6; ++x)
+#pragma unroll
+            for (int y = 0; y < 6; ++y) {
+                cab[6 * x + y] = static_cast<float>(acc[6 * x + y]);
+                cba[6 * y + x] = static_cast<float>(acc[6 * x + y]);
+            }
+    }
     if (touched) touched[g] = hit != 0 ? 1 : 0;
 }
 
@@ -460,20 +519,26 @@ void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStre
     launch_energies(in, s, false, out, stream);
 }
 
+// The block arithmetic follows the matrix type: FP64 blocks (optionally with
+// an FP32 copy for the mixed form's iterations) from FP64 accumulation, FP32
+// blocks from FP32 arithmetic and accumulation. The right-hand side and the
+// energies are FP64 either way.
 template <typename Scalar>
 void assemble(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s, Scalar* bsr, double* rhs,
-              double* energies_out, hipStream_t stream) {
+              double* energies_out, hipStream_t stream, float* copy32) {
+    if (sizeof(Scalar) == sizeof(float) && copy32) throw InvalidArgument("assemble: an FP32 matrix takes no copy");
     launch_energies(in, s, true, energies_out, stream);
     const int per_block = kThreads / kWave;
-    node_blocks<Scalar, true><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+    node_blocks<Scalar, true, Scalar><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
-        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), bsr, rhs);
+        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), bsr, rhs,
+        copy32);
     NRICP_LAUNCH_CHECK("node_blocks");
     if (L.group_count > 0) {
         s.touched.resize(L.group_count);
-        pair_blocks<Scalar><<<blocks_for(L.group_count * kPairLanes, kThreads), kThreads, 0, stream>>>(
-            in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.group_count, L.group_offsets.get(),
-            L.group_entries.get(), L.pair_position.get(), bsr, s.touched.get());
+        pair_blocks<Scalar, Scalar><<<blocks_for(L.group_count * kPairLanes, kThreads), kThreads, 0, stream>>>(
+            in, s.vertex_scale.get(), s.vertex_arm.get(), L.group_count, L.group_offsets.get(), L.group_entries.get(),
+            L.pair_position.get(), bsr, s.touched.get(), copy32);
         NRICP_LAUNCH_CHECK("pair_blocks");
     }
 }
@@ -482,16 +547,16 @@ void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScr
                      double* energies_out, hipStream_t stream) {
     launch_energies(in, s, true, energies_out, stream);
     const int per_block = kThreads / kWave;
-    node_blocks<double, false><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+    node_blocks<double, false, double><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
         L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), nullptr,
-        rhs);
+        rhs, nullptr);
     NRICP_LAUNCH_CHECK("node_blocks");
 }
 
 template void assemble<double>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, double*, double*, double*,
-                               hipStream_t);
+                               hipStream_t, float*);
 template void assemble<float>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, float*, double*, double*,
-                              hipStream_t);
+                              hipStream_t, float*);
 
 }  // namespace nricp_hip

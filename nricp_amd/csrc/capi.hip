@@ -465,9 +465,18 @@ int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precis
         AssemblyScratch scratch;
         DeviceBuffer<double> rhs(6 * nn), energy(5);
         std::vector<double> blocks(36 * std::max<int64_t>(layout.nnzb, 1));
-        if (precision == NRICP_HIP_PRECISION_MIXED) {
+        require(precision == NRICP_HIP_PRECISION_DOUBLE || precision == NRICP_HIP_PRECISION_MIXED ||
+                    precision == NRICP_HIP_PRECISION_FLOAT,
+                "fused_system: unknown precision");
+        if (precision != NRICP_HIP_PRECISION_DOUBLE) {
             DeviceBuffer<float> bsr(36 * std::max<int64_t>(layout.nnzb, 1));
-            assemble<float>(u.in, layout, scratch, bsr.get(), rhs.get(), energy.get(), st);
+            if (precision == NRICP_HIP_PRECISION_FLOAT) {
+                assemble<float>(u.in, layout, scratch, bsr.get(), rhs.get(), energy.get(), st);
+            } else {
+                DeviceBuffer<double> b64(36 * std::max<int64_t>(layout.nnzb, 1));
+                assemble<double>(u.in, layout, scratch, b64.get(), rhs.get(), energy.get(), st, bsr.get());
+                NRICP_CHECK(hipStreamSynchronize(st));  // b64 is released at the end of this scope
+            }
             std::vector<float> b32(bsr.size());
             bsr.download(b32.data(), bsr.size(), st);
             NRICP_CHECK(hipStreamSynchronize(st));
@@ -585,11 +594,17 @@ int nricp_hip_solve_system(const double* diag, int64_t nn, const int32_t* pairs,
         r.upload(rhs, 6 * nn, st);
         PcgWorkspace ws;
         const int it = max_iter > 0 ? max_iter : 1;
-        if (precision == NRICP_HIP_PRECISION_MIXED) {
+        require(precision == NRICP_HIP_PRECISION_DOUBLE || precision == NRICP_HIP_PRECISION_MIXED ||
+                    precision == NRICP_HIP_PRECISION_FLOAT,
+                "solve_system: unknown precision");
+        if (precision != NRICP_HIP_PRECISION_DOUBLE) {
             std::vector<float> v32(vals.begin(), vals.end());
             DeviceBuffer<float> b32;
             b32.upload(v32, st);
-            pcg_solve_mixed(layout, b32.get(), r.get(), nn, damping, scaling != 0, tol, it, x.get(), ws, st);
+            const bool mixed = precision == NRICP_HIP_PRECISION_MIXED;
+            if (mixed) b64.upload(vals, st);
+            pcg_solve_mixed(layout, b32.get(), mixed ? b64.get() : nullptr, r.get(), nn, damping, scaling != 0, tol,
+                            it, x.get(), ws, st);
             x.download(out_delta, 6 * nn, st);
             ws.outcome.download(out_info, 4, st);
             NRICP_CHECK(hipStreamSynchronize(st));

@@ -269,9 +269,12 @@ struct AssemblyScratch {
 void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream);
 // The system: diagonal and pair blocks into `bsr` (nnzb x 36, at the layout's
 // positions), rhs = -J^T r (N x 6), plus the energies of the same state.
+// FP64 blocks are accumulated in FP64 (and `copy32`, when given, receives
+// them rounded to FP32); FP32 blocks are computed and accumulated in FP32.
+// The right-hand side and the energies are FP64 in both.
 template <typename Scalar>
 void assemble(const AssemblyInputs& in, const LevelLayout& layout, AssemblyScratch& s, Scalar* bsr, double* rhs,
-              double* energies_out, hipStream_t stream);
+              double* energies_out, hipStream_t stream, float* copy32 = nullptr);
 
 // The right-hand side -J^T r of the state with its rotation half zero: the
 // translation system of the local/global route (the energies of the same
@@ -320,11 +323,13 @@ struct PcgWorkspace {
 void pcg_solve(const LevelLayout& layout, const double* bsr, const double* rhs, int64_t node_count,
                double lambda, bool diagonal_scaling, double tolerance, int max_iterations, double* solution,
                PcgWorkspace& ws, hipStream_t stream);
-// Mixed precision: FP32 matrix and inner iterations, FP64 outer residual
-// refinement (r = b - A x in FP64 arithmetic) until the FP64 relative residual
-// reaches `tolerance`.
-void pcg_solve_mixed(const LevelLayout& layout, const float* bsr, const double* rhs, int64_t node_count,
-                     double lambda, bool diagonal_scaling, double tolerance, int max_iterations,
+// FP32 inner iterations on an FP32 matrix, FP64 outer residual refinement
+// (r = b - A x in FP64 arithmetic) until the FP64 relative residual reaches
+// `tolerance`. With `bsr64` (the same blocks in FP64: the mixed form) the
+// refinement residual is that FP64 system's, so the solve converges to the
+// FP64 system; without it (the all-FP32 form) to the FP32 matrix's system.
+void pcg_solve_mixed(const LevelLayout& layout, const float* bsr, const double* bsr64, const double* rhs,
+                     int64_t node_count, double lambda, bool diagonal_scaling, double tolerance, int max_iterations,
                      double* solution, PcgWorkspace& ws, hipStream_t stream);
 
 }  // namespace nricp_hip

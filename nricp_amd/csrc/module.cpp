@@ -12,7 +12,9 @@
 //
 // New options are keyword arguments appended with defaults: graph_neighbours
 // (the reference hard-codes 4 neighbours, src/core/graph.cpp:69) and precision
-// ("double" or "mixed") for the device routes.
+// ("double", "mixed" or "float") for the device routes: FP64 throughout; FP32
+// iterations refined against the FP64 system; or FP32 assembly and iterations
+// refined against that FP32 system.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -692,6 +694,14 @@ std::string serialize_report(const std::string& route, const std::string& precis
 
 // ------------------------------------------------------------- prepare ----
 
+// "double", "mixed" or "float" (the precision forms of include/nricp_hip.h).
+int32_t precision_code(const std::string& p) {
+    if (p == "double") return NRICP_HIP_PRECISION_DOUBLE;
+    if (p == "mixed") return NRICP_HIP_PRECISION_MIXED;
+    if (p == "float") return NRICP_HIP_PRECISION_FLOAT;
+    throw py::value_error("precision must be 'double', 'mixed' or 'float', not '" + p + "'");
+}
+
 int robust_kind(const std::string& n) {
     if (n == "huber" || n == "Huber") return 1;
     if (n == "geman_mcclure" || n == "geman-mcclure" || n == "GemanMcClure") return 2;
@@ -760,8 +770,7 @@ std::unique_ptr<Prepared> prepare(const Dbl& tp, const Idx& tf, const Dbl& gp, c
         for (int64_t v : idx) p->lv.push_back(static_cast<int32_t>(v));
         p->lt = tt;
     }
-    if (precision != "double" && precision != "mixed")
-        throw py::value_error("precision must be 'double' or 'mixed', not '" + precision + "'");
+    const int32_t precision_form = precision_code(precision);
     nricp_hip_settings& s = p->s;
     s.levels = p->levels.data();
     s.level_settings_count = static_cast<int32_t>(p->levels.size());
@@ -780,7 +789,7 @@ std::unique_ptr<Prepared> prepare(const Dbl& tp, const Idx& tf, const Dbl& gp, c
     s.graph_neighbours = neighbours;
     s.pcg_tolerance = pcg_tol;
     s.pcg_max_iterations = pcg_max;
-    s.precision = precision == "mixed" ? NRICP_HIP_PRECISION_MIXED : NRICP_HIP_PRECISION_DOUBLE;
+    s.precision = precision_form;
     s.route = route;
     return p;
 }
@@ -1552,7 +1561,7 @@ PYBIND11_MODULE(nricp_dp, m) {
         std::vector<double> diag(36 * a.nn), rhs(6 * a.nn), blocks(36 * cap), energies(6);
         std::vector<int32_t> pairs(2 * cap);
         int64_t count = 0;
-        check(nricp_hip_fused_system(&in, precision == "mixed" ? NRICP_HIP_PRECISION_MIXED : NRICP_HIP_PRECISION_DOUBLE,
+        check(nricp_hip_fused_system(&in, precision_code(precision),
                                      diag.data(), rhs.data(), pairs.data(), blocks.data(), cap, &count,
                                      energies.data()));
         py::dict d = system_dict(diag.data(), rhs.data(), a.nn, pairs, blocks, count);
@@ -1596,7 +1605,7 @@ PYBIND11_MODULE(nricp_dp, m) {
         std::vector<double> delta(6 * s.n), info(4);
         check(nricp_hip_solve_system(s.diag.data(), s.n, pairs.data(), off.data(), static_cast<int64_t>(s.pairs.size()),
                                      s.rhs.data(), lambda, scaling ? 1 : 0,
-                                     precision == "mixed" ? NRICP_HIP_PRECISION_MIXED : NRICP_HIP_PRECISION_DOUBLE, tol,
+                                     precision_code(precision), tol,
                                      max_iter, delta.data(), info.data()));
         // Residual of the damped system, as the reference reports it.
         std::vector<double> ax(6 * s.n, 0.0);
@@ -1649,7 +1658,9 @@ PYBIND11_MODULE(nricp_dp, m) {
 
     const char* route_doc =
         "Fit the template onto the target and return the result plus the run report. Distances are in the "
-        "caller's units; template and target must already be aligned.";
+        "caller's units; template and target must already be aligned. precision (device routes B and C): "
+        "'double' (FP64 throughout), 'mixed' (FP32 PCG iterations, FP64 refinement against the FP64 system) or "
+        "'float' (FP32 assembly blocks and PCG iterations, FP64 refinement of that FP32 system).";
     m.def("solve_route_a", [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                               const py::object& mk, const py::object& li, const py::object& lt, double dist,
                               double mincos, double slope, double d0, double ddec, double dinc, int att, bool diag,

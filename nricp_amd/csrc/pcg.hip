@@ -90,23 +90,34 @@ __device__ inline bool spd_inverse6(const double (&a)[36], double jitter, double
 // src/cuda/pcg_solver.cu:15-46). The jitter only shapes the preconditioner;
 // the system keeps its damping.
 template <typename S>
-__global__ void precondition_kernel(const S* __restrict__ bsr, const int32_t* __restrict__ diag_pos, int64_t nn,
-                                    double lambda, int scaling, S* __restrict__ damp, S* __restrict__ minv,
+__global__ void precondition_kernel(const S* __restrict__ bsr, const double* __restrict__ bsr64,
+                                    const int32_t* __restrict__ diag_pos, int64_t nn, double lambda, int scaling,
+                                    S* __restrict__ damp, double* __restrict__ damp64, S* __restrict__ minv,
                                     unsigned* __restrict__ barrier) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     // The solve's grid-barrier counter starts at zero (the solve runs next on
     // the same stream).
     if (i == 0) barrier[0] = barrier[1] = 0u;
     if (i >= nn) return;
-    const S* blk = bsr + 36 * static_cast<int64_t>(diag_pos[i]);
+    // With an FP64 copy of the matrix (the mixed form) the damping and the
+    // preconditioner come from its diagonal block, and the FP64 damping is
+    // kept for the refinement residual.
     double a[36];
+    if (bsr64) {
+        const double* blk = bsr64 + 36 * static_cast<int64_t>(diag_pos[i]);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) a[k] = blk[k];
+    } else {
+        const S* blk = bsr + 36 * static_cast<int64_t>(diag_pos[i]);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) a[k] = static_cast<double>(blk[k]);
+    }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) a[6 * r + c] = static_cast<double>(blk[6 * r + c]);
         double d = 0.0;
         if (lambda > 0.0) d = lambda * (scaling ? fmax(a[6 * r + r], 1e-12) : 1.0);
         damp[6 * i + r] = static_cast<S>(d);
+        if (damp64) damp64[6 * i + r] = d;
         a[6 * r + r] += d;
     }
     double inv[36];
@@ -239,11 +250,11 @@ struct MatrixView {
 // copies), then its product with the column node's 6-vector into acc. An
 // FP32 block is nine loads; an FP64 block goes row by row (three loads each)
 // to bound the registers it holds.
-template <typename S, typename V, typename Acc>
+template <typename S, typename V, typename Acc, bool Abs = false>
 __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&acc)[6]) {
     Acc xv[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) xv[c] = static_cast<Acc>(x[c]);
+    for (int c = 0; c < 6; ++c) xv[c] = Abs ? fabs(static_cast<Acc>(x[c])) : static_cast<Acc>(x[c]);
     if constexpr (sizeof(S) == 4) {
         S v[36];
 #pragma unroll
@@ -258,7 +269,8 @@ __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&ac
         for (int r = 0; r < 6; ++r) {
             Acc t = Acc(0);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) t += static_cast<Acc>(v[6 * r + c]) * xv[c];
+            for (int c = 0; c < 6; ++c)
+                t += (Abs ? fabs(static_cast<Acc>(v[6 * r + c])) : static_cast<Acc>(v[6 * r + c])) * xv[c];
             acc[r] += t;
         }
     } else {
@@ -269,19 +281,19 @@ __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&ac
             const S v[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
             Acc t = Acc(0);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) t += static_cast<Acc>(v[k]) * xv[k];
+            for (int k = 0; k < 6; ++k) t += (Abs ? fabs(static_cast<Acc>(v[k])) : static_cast<Acc>(v[k])) * xv[k];
             acc[r] += t;
         }
     }
 }
 
-template <typename S, typename V, typename Acc>
+template <typename S, typename V, typename Acc, bool Abs = false>
 __device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* __restrict__ x, Acc (&acc)[6]) {
     const int32_t rel = b - m.begin;
     if (rel >= 0 && rel < m.staged)
-        add_block<S, V, Acc>(m.local + 36 * rel, x, acc);
+        add_block<S, V, Acc, Abs>(m.local + 36 * rel, x, acc);
     else
-        add_block<S, V, Acc>(m.global + 36 * static_cast<int64_t>(b), x, acc);
+        add_block<S, V, Acc, Abs>(m.global + 36 * static_cast<int64_t>(b), x, acc);
 }
 
 // (A + diag(damp)) u for the six rows of one node, evaluated by the node's
@@ -295,31 +307,40 @@ __device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* 
 // lane of the wave must call it (the sum crosses lanes); `valid` false adds
 // zeros. Returns row `lane` of the product (lanes 6 and 7 get row 0). Acc is
 // the accumulation precision (the matrix precision for the iteration, FP64
-// for the refinement residual).
-template <typename S, typename V, typename Acc>
+// for the refinement residual). Abs: the same sum over absolute values,
+// (|A| + |D|) |u|, the scale of the product's rounding error.
+template <typename S, typename V, typename Acc, bool Abs = false>
 __device__ inline Acc node_product(const RowShare& share, const int32_t* __restrict__ cols, const MatrixView<S>& m,
                                    S damp, V self, const V* __restrict__ u, bool valid, int lane) {
     Acc acc[6] = {Acc(0), Acc(0), Acc(0), Acc(0), Acc(0), Acc(0)};
     if (share.first >= 0) {
-        add_block_at<S, V, Acc>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
+        add_block_at<S, V, Acc, Abs>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
         int32_t b = share.first + 8;
         // The FP64 kernels have no registers to spare for the second column.
         if constexpr (sizeof(S) == 4) {
             if (b < share.end)
-                add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(share.second_column), acc);
+                add_block_at<S, V, Acc, Abs>(m, b, u + 6 * static_cast<int64_t>(share.second_column), acc);
             b += 8;
         }
         for (; b < share.end; b += 8)
-            add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
+            add_block_at<S, V, Acc, Abs>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
     }
 #pragma unroll
     for (int r = 0; r < 6; ++r) acc[r] = octet_sum(acc[r]);
     Acc out = acc[0];
 #pragma unroll
     for (int r = 1; r < 6; ++r) out = lane == r ? acc[r] : out;
-    if (valid && lane < 6) out += static_cast<Acc>(damp) * static_cast<Acc>(self);
+    if (valid && lane < 6)
+        out += Abs ? fabs(static_cast<Acc>(damp)) * fabs(static_cast<Acc>(self))
+                   : static_cast<Acc>(damp) * static_cast<Acc>(self);
     return out;
 }
+
+// Rounding-error factor of an FP64 residual row b - (A + D) x: a row sums at
+// most a few hundred products, each off by at most one unit roundoff, so
+// |fl(r) - r| <= gamma (|b| + (|A| + |D|) |x|) componentwise with this gamma
+// (n u / (1 - n u) for n = 256, u = 2^-53).
+constexpr double kResidualGamma = 256.0 * 1.1102230246251565e-16;
 
 // Preconditioned pipelined CG (Ghysels and Vanroose, 2014): the iteration's
 // two inner products and its matrix product are independent, so the
@@ -337,7 +358,8 @@ __device__ inline Acc node_product(const RowShare& share, const int32_t* __restr
 template <int K, typename S, bool Grid>
 __global__ void __launch_bounds__(kPcgThreads)
 pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
-           const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ rhs, int64_t nn,
+           const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ bsr64,
+           const double* __restrict__ damp64, const double* __restrict__ rhs, int64_t nn,
            double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
            double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
            unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
@@ -351,6 +373,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     bool own[K], live[K];
     RowShare share[K];
     S dampk[K];
+    double damp64k[K];  // FP64 damping of the refinement residual (with bsr64)
     S mrow[K][6];
     double b64[K], x64[K];
     S x[K], r[K], u[K], w[K], p[K], s[K], q[K], z[K];
@@ -362,6 +385,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         own[k] = row_lane && live[k];
         share[k] = row_share(row_off, cols, node[k], live[k], lane);
         dampk[k] = own[k] ? damp[6 * node[k] + lane] : S(0);
+        damp64k[k] = (bsr64 && own[k]) ? damp64[6 * node[k] + lane] : 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) mrow[k][c] = own[k] ? minv[36 * node[k] + 6 * lane + c] : S(0);
         b64[k] = own[k] ? rhs[6 * node[k] + lane] : 0.0;
@@ -465,7 +489,15 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
                 alpha = gamma / (delta - beta * gamma / alpha_prev);
             }
             if (!(alpha > 0.0) || !isfinite(alpha)) {
-                breakdown = true;
+                // The recurrences have lost their consistency (the pipelined
+                // form's extra recurrences amplify rounding, in FP32 most): end
+                // the cycle and restart from the true residual, unless no step
+                // was taken from it.
+                if (cycle_it == 0) {
+                    breakdown = true;
+                } else {
+                    inner_done = true;
+                }
                 break;
             }
 #pragma unroll
@@ -493,21 +525,36 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             if (own[k]) xbuf[6 * node[k] + lane] = x64[k];
         }
         sync_all<Grid>(barrier);
-        double rr = 0.0;
+        double rr = 0.0, ee = 0.0;
         double r64[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double v =
-                node_product<S, double, double>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
+            // The residual of the FP64 system when its matrix is given (the
+            // mixed form: FP32 iterations refine the FP64 solution), else of
+            // the iteration's own matrix; and the scale of its rounding error.
+            double v, a;
+            if (bsr64) {
+                const MatrixView<double> m64{bsr64, nullptr, 0, 0};
+                v = node_product<double, double, double>(share[k], cols, m64, damp64k[k], x64[k], xbuf, live[k],
+                                                         lane);
+                a = node_product<double, double, double, true>(share[k], cols, m64, damp64k[k], x64[k], xbuf,
+                                                               live[k], lane);
+            } else {
+                v = node_product<S, double, double>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
+                a = node_product<S, double, double, true>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
+            }
             r64[k] = own[k] ? b64[k] - v : 0.0;
             rr += r64[k] * r64[k];
+            const double e = own[k] ? fabs(b64[k]) + a : 0.0;
+            ee += e * e;
         }
         {
-            const double v[1] = {rr};
-            double o[1];
-            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            const double v[2] = {rr, ee};
+            double o[2];
+            launch_sum<2, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
             phase ^= 1;
             red[0] = o[0];
+            red[1] = o[1];
         }
         const double tnorm = sqrt(red[0]);
         rel = bnorm > 0.0 ? tnorm / bnorm : 0.0;
@@ -515,8 +562,15 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             converged = true;
             break;
         }
-        // A cycle that no longer reduces the true residual ends the solve.
-        if (!(tnorm < cycle_norm)) break;
+        // A cycle that no longer reduces the true residual ends the solve. If
+        // the residual is then within its own rounding error, the solution is
+        // as accurate as FP64 evaluates it (a tolerance below that floor
+        // cannot be met by any solver); the solve counts as converged, as a
+        // direct solve of the same system would.
+        if (!(tnorm < cycle_norm)) {
+            converged = tnorm <= kResidualGamma * sqrt(red[1]);
+            break;
+        }
         cycle_norm = tnorm;
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = static_cast<S>(r64[k]);
@@ -540,13 +594,14 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
 // the co-resident count and kPcgThreads, which the fold of the partial sums
 // assumes), growing K beyond that.
 template <typename S>
-void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t nn, double lambda, bool scaling,
-                double tol, double inner_tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp, S* minv,
-                DeviceBuffer<S>& vbuf, hipStream_t stream) {
+void launch_pcg(const LevelLayout& L, const S* bsr, const double* bsr64, const double* rhs, int64_t nn, double lambda,
+                bool scaling, double tol, double inner_tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp,
+                double* damp64, S* minv, DeviceBuffer<S>& vbuf, hipStream_t stream) {
     if (nn <= 0) return;
     ws.barrier.resize(2);
-    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, L.diag_position.get(), nn, lambda,
-                                                                     scaling ? 1 : 0, damp, minv, ws.barrier.get());
+    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, bsr64, L.diag_position.get(), nn, lambda,
+                                                                     scaling ? 1 : 0, damp, damp64, minv,
+                                                                     ws.barrier.get());
     NRICP_LAUNCH_CHECK("precondition_kernel");
     constexpr int kmax = 2;
     const int64_t passes = (nn + kNodesPerPass - 1) / kNodesPerPass;
@@ -589,8 +644,10 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* rhs, int64_t n
     double* slots = ws.reduce.get();
     unsigned* bar = ws.barrier.get();
     double* out = ws.outcome.get();
-    void* args[] = {&row_off, &cols, const_cast<S**>(&bsr), &damp, &minv, const_cast<double**>(&rhs), &n, &tol,
-                    &inner_tol, &max_iter, &vb, &xbuf, &slots, &bar, &solution, &out};
+    double* d64 = damp64;
+    void* args[] = {&row_off, &cols, const_cast<S**>(&bsr), &damp, &minv, const_cast<double**>(&bsr64), &d64,
+                    const_cast<double**>(&rhs), &n, &tol, &inner_tol, &max_iter, &vb, &xbuf, &slots, &bar,
+                    &solution, &out};
     if (ws.kernel_start) NRICP_CHECK(hipEventRecord(ws.kernel_start, stream));
     if (grid) {
         launch_persistent(kernel, groups, kPcgThreads, args, stream);
@@ -607,18 +664,23 @@ void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64
                double tol, int max_iter, double* solution, PcgWorkspace& ws, hipStream_t stream) {
     ws.damp.resize(6 * nn);
     ws.minv.resize(36 * nn);
-    launch_pcg<double>(L, bsr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
-                       ws.minv.get(), ws.u, stream);
+    launch_pcg<double>(L, bsr, nullptr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
+                       nullptr, ws.minv.get(), ws.u, stream);
 }
 
-void pcg_solve_mixed(const LevelLayout& L, const float* bsr, const double* rhs, int64_t nn, double lambda,
-                     bool scaling, double tol, int max_iter, double* solution, PcgWorkspace& ws, hipStream_t stream) {
+void pcg_solve_mixed(const LevelLayout& L, const float* bsr, const double* bsr64, const double* rhs, int64_t nn,
+                     double lambda, bool scaling, double tol, int max_iter, double* solution, PcgWorkspace& ws,
+                     hipStream_t stream) {
     ws.w32.resize(6 * nn);  // damping vector (FP32)
     ws.minv32.resize(36 * nn);
-    // FP32 cycles stop at 1e-5 of the cycle's starting residual, well above
-    // where the FP32 recurrence loses accuracy; FP64 refinement does the rest.
-    launch_pcg<float>(L, bsr, rhs, nn, lambda, scaling, tol, std::max(tol, 1e-5), max_iter, solution, ws,
-                      ws.w32.get(), ws.minv32.get(), ws.u32, stream);
+    if (bsr64) ws.damp.resize(6 * nn);  // FP64 damping of the refinement residual
+    // FP32 cycles stop at 1e-4 of the cycle's starting residual: the gap
+    // between the pipelined recurrence residual and the true one grows with
+    // the iterations of a cycle and reaches 1e-3 within a few dozen FP32
+    // iterations on ill-conditioned systems, so the cycles restart (residual
+    // replacement) well before it matters; FP64 refinement does the rest.
+    launch_pcg<float>(L, bsr, bsr64, rhs, nn, lambda, scaling, tol, std::max(tol, 1e-4), max_iter, solution, ws,
+                      ws.w32.get(), bsr64 ? ws.damp.get() : nullptr, ws.minv32.get(), ws.u32, stream);
 }
 
 }  // namespace nricp_hip

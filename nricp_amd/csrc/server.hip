@@ -37,7 +37,7 @@ int64_t estimate_job_bytes(const nricp_hip_problem& pb, const nricp_hip_settings
     for (int l = 1; l < std::max(1, s.graph_levels); ++l) n *= std::max(1, s.growth);
     n = std::min(n, v);
     const double levels = std::max(1, s.graph_levels);
-    const double bytes = v * (900.0 + 60.0 * levels) + t * 260.0 + tv * 80.0 + n * 8000.0 + 64.0 * (1 << 20);
+    const double bytes = v * (900.0 + 60.0 * levels) + t * 260.0 + tv * 80.0 + n * 10000.0 + 64.0 * (1 << 20);
     return static_cast<int64_t>(bytes);
 }
 

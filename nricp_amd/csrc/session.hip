@@ -268,7 +268,8 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         throw InvalidArgument("the target needs vertices and faces");
     if (s.route != NRICP_HIP_ROUTE_A && s.route != NRICP_HIP_ROUTE_B && s.route != NRICP_HIP_ROUTE_C)
         throw InvalidArgument("unknown route");
-    if (s.precision != NRICP_HIP_PRECISION_DOUBLE && s.precision != NRICP_HIP_PRECISION_MIXED)
+    if (s.precision != NRICP_HIP_PRECISION_DOUBLE && s.precision != NRICP_HIP_PRECISION_MIXED &&
+        s.precision != NRICP_HIP_PRECISION_FLOAT)
         throw InvalidArgument("unknown precision");
     report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
     report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
@@ -517,10 +518,10 @@ void Session::begin_level() {
     build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
                        L.edge_count, L.node_count, layout_, sort_, st);
     clock_.end(st);
-    if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
-        bsr32_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
-    else
-        bsr_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
+    // FP64 blocks for the double and mixed forms, FP32 blocks for the mixed
+    // form's iterations and the all-FP32 form.
+    if (settings_.precision != NRICP_HIP_PRECISION_FLOAT) bsr_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
+    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE) bsr32_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
     view_ = LevelView{};
     view_.level = static_cast<int>(level_);
     view_.nodes = static_cast<int>(L.node_count);
@@ -573,8 +574,10 @@ void Session::solve(double damping) {
     // set-up before it is a separate, much shorter launch).
     pcg_.kernel_start = pcg_.kernel_stop = nullptr;
     clock_.external("pcg", &pcg_.kernel_start, &pcg_.kernel_stop);
-    if (settings_.precision == NRICP_HIP_PRECISION_MIXED)
-        pcg_solve_mixed(layout_, bsr32_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
+    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE)
+        pcg_solve_mixed(layout_, bsr32_.get(),
+                        settings_.precision == NRICP_HIP_PRECISION_MIXED ? bsr_.get() : nullptr, rhs_.get(), nn,
+                        damping, settings_.diagonal_damping != 0,
                         settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
     else
         pcg_solve(layout_, bsr_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
@@ -591,12 +594,15 @@ bool Session::inner_step() {
     if (!assembled_) {
         const AssemblyInputs in = inputs(rot_.get(), trans_.get());
         clock_.begin("assemble", st);
-        if (settings_.precision == NRICP_HIP_PRECISION_MIXED) {
+        if (settings_.precision == NRICP_HIP_PRECISION_FLOAT) {
             assemble<float>(in, layout_, assembly_, bsr32_.get(), rhs_.get(), energy_cur_.get(), st);
             if (local_global) restrict_to_translations<float>(layout_, bsr32_.get(), st);
         } else {
-            assemble<double>(in, layout_, assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st);
+            const bool mixed = settings_.precision == NRICP_HIP_PRECISION_MIXED;
+            assemble<double>(in, layout_, assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st,
+                             mixed ? bsr32_.get() : nullptr);
             if (local_global) restrict_to_translations<double>(layout_, bsr_.get(), st);
+            if (local_global && mixed) restrict_to_translations<float>(layout_, bsr32_.get(), st);
         }
         clock_.end(st);
         // The local/global route keeps this matrix for the whole
@@ -604,8 +610,10 @@ bool Session::inner_step() {
         assembled_ = true;
         // Compulsory traffic of one assembly: per vertex its rest point,
         // binding, correspondence, normal, weight and flags (134 B); per node
-        // R, t, g in and the diagonal block and rhs out; per pair one block out.
-        const double sb = settings_.precision == NRICP_HIP_PRECISION_MIXED ? 4.0 : 8.0;
+        // R, t, g in and the diagonal block and rhs out; per pair one block out
+        // (the mixed form writes each block twice, FP64 and FP32).
+        const double sb = settings_.precision == NRICP_HIP_PRECISION_FLOAT ? 4.0
+                          : settings_.precision == NRICP_HIP_PRECISION_MIXED ? 12.0 : 8.0;
         const double N = static_cast<double>(nn), P = static_cast<double>(layout_.group_count);
         counters_[kAssembleLaunches] += 1.0;
         counters_[kAssembleBytes] += 134.0 * static_cast<double>(nv_) + N * (120.0 + 36.0 * sb + 48.0) + P * 36.0 * sb;
@@ -648,13 +656,15 @@ bool Session::inner_step() {
         // matrix once (N diagonal and P pair blocks of 36 values) and the
         // exchanged 6-vector written and read once; per solve the
         // preconditioner written and read once, the rhs read and the solution
-        // written (FP64). Candidate energy pass: its inputs once.
+        // written (FP64); the mixed form's refinement residual reads the FP64
+        // matrix at least once more. Candidate energy pass: its inputs once.
         {
-            const double s = settings_.precision == NRICP_HIP_PRECISION_MIXED ? 4.0 : 8.0;
+            const double s = settings_.precision == NRICP_HIP_PRECISION_DOUBLE ? 8.0 : 4.0;
             const double P = static_cast<double>(layout_.group_count), N = static_cast<double>(nn);
             counters_[kPcgLaunches] += 1.0;
             counters_[kPcgIterations] += h[10];
             counters_[kPcgBytes] += h[10] * s * (36.0 * (N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
+            if (settings_.precision == NRICP_HIP_PRECISION_MIXED) counters_[kPcgBytes] += 8.0 * 36.0 * (N + P);
             counters_[kEnergyLaunches] += 1.0;
             counters_[kEnergyBytes] += 134.0 * static_cast<double>(nv_) + 120.0 * N +
                                        8.0 * static_cast<double>(graph_->levels[level_].edge_count) + 40.0;

@@ -4,14 +4,15 @@
   target, 500 nodes, 3 levels x 10 GN steps) and humanoid 50k (50k-vertex
   capsule body onto a posed, bumped, re-meshed target with 2 % holes, 2,000
   nodes, 4 levels x 10): complete device wraps -- route A (FP64, solves to
-  1e-12) and route B in both forms at the benchmark's PCG tolerance (1e-8) --
+  1e-12) and route B in its three forms at the benchmark's PCG tolerance
+  (1e-8) --
   against a complete oracle wrap: vertices within 1e-5 (RMS) and 1e-4 (max)
   of the box diagonal, identical level sizes and finest nodes, identical
   final correspondence counts.
 * geodesic 200k (198,812-vertex template, 800,000-triangle target with normal
   noise, 8,000 nodes): the oracle's whole wrap takes minutes, so here the
   properties that do not need it -- the device template graph bit for bit
-  against the oracle's at full size, and the mixed form within the same
+  against the oracle's at full size, and the FP32 forms within the same
   bound of the FP64 form. (The 8,000-node system's PCG launch shape against a
   direct solve is in test_pcg_rows.py; the bench reports this job's error
   against a complete oracle wrap.)
@@ -70,7 +71,7 @@ COUNTS = ("total", "accepted", "rejected_distance", "rejected_normal", "rejected
 
 
 @pytest.mark.parametrize("config", [0, 1])
-@pytest.mark.parametrize("form", ["route_a", "mixed", "double"])
+@pytest.mark.parametrize("form", ["route_a", "mixed", "float", "double"])
 def test_full_size_wrap_matches_the_oracle(device, oracle, config, form):
     job = _job(config)
     o = _oracle_wrap(oracle, config)
@@ -105,21 +106,22 @@ def test_200k_template_graph_is_bit_identical_to_the_oracle(device, oracle):
         assert np.array_equal(left["edges"], right["edges"])
 
 
-def test_200k_mixed_form_stays_within_the_bound_of_the_fp64_form(device):
+def test_200k_fp32_forms_stay_within_the_bound_of_the_fp64_form(device):
     job = _job(2)
     out = {}
-    for form in ("mixed", "double"):
+    for form in ("mixed", "float", "double"):
         args, kwargs = _call(job, precision=form, pcg_tolerance=BENCH_TOLERANCE, pcg_max_iterations=5000)
         out[form] = device.solve_route_b(*args, **kwargs)
-    rms, worst = _error(out["mixed"]["vertices"], out["double"]["vertices"], job.template_points)
-    print(f"{job.name}: mixed vs double rms {rms:.3e} max {worst:.3e} of the diagonal")
-    assert rms <= 1e-5 and worst <= 1e-4, (rms, worst)
-    assert np.array_equal(out["mixed"]["node_positions"], out["double"]["node_positions"])
-    # The fit reaches the surface: most correspondences accepted, mean
-    # distance small against the diagonal.
-    rep = json.loads(out["mixed"]["report"])
-    assert rep["correspondence"]["accepted"] == len(job.template_points)
-    assert rep["correspondence"]["distance_mean"] < 1e-2
+    for form in ("mixed", "float"):
+        rms, worst = _error(out[form]["vertices"], out["double"]["vertices"], job.template_points)
+        print(f"{job.name}: {form} vs double rms {rms:.3e} max {worst:.3e} of the diagonal")
+        assert rms <= 1e-5 and worst <= 1e-4, (form, rms, worst)
+        assert np.array_equal(out[form]["node_positions"], out["double"]["node_positions"])
+        # The fit reaches the surface: every correspondence accepted, mean
+        # distance small against the diagonal.
+        rep = json.loads(out[form]["report"])
+        assert rep["correspondence"]["accepted"] == len(job.template_points)
+        assert rep["correspondence"]["distance_mean"] < 1e-2
 
 
 def _batch_selection():

@@ -288,6 +288,33 @@ def test_fused_system_matches_the_oracle(device, oracle, masked):
         assert d["energies"][key] == pytest.approx(value, rel=1e-12, abs=1e-300), key
 
 
+@pytest.mark.parametrize("precision", ["mixed", "float"])
+def test_fp32_fused_systems_match_the_oracle_to_fp32_accuracy(device, oracle, precision):
+    """The mixed form's FP32 blocks are the FP64 ones rounded (half an FP32
+    ulp of each entry); the all-FP32 form computes and accumulates them in
+    FP32 (a few ulps of the largest entry). The right-hand side and the
+    energies stay FP64 in both."""
+    points, _ = sphere_patch()
+    state = graph_state(oracle, points, 13, seed=8)
+    robust = np.random.default_rng(9).uniform(0.2, 1.0, size=len(points))
+    args = energy_args(points, state, None, robust)
+    d = device.fused_system(*args, all_weights(), None, precision)
+    o = oracle.fused_system(*args, all_weights(), None)
+    assert np.array_equal(d["edges"], o["edges"])
+    scale = np.abs(o["diagonal"]).max()
+    if precision == "mixed":
+        f32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)  # noqa: E731
+        assert np.array_equal(d["diagonal"], f32(o["diagonal"]))
+        assert np.array_equal(d["edge_blocks"], f32(o["edge_blocks"]))
+    else:
+        assert np.allclose(d["diagonal"], o["diagonal"], rtol=0, atol=1e-6 * scale)
+        assert np.allclose(d["edge_blocks"], o["edge_blocks"], rtol=0, atol=1e-6 * scale)
+        assert not np.array_equal(d["diagonal"], o["diagonal"])
+    assert np.allclose(d["rhs"], o["rhs"], rtol=0, atol=1e-12 * max(np.abs(o["rhs"]).max(), 1.0))
+    for key, value in o["energies"].items():
+        assert d["energies"][key] == pytest.approx(value, rel=1e-12, abs=1e-300), key
+
+
 def test_solve_matches_the_direct_solve(device, oracle):
     points, _ = sphere_patch()
     state = graph_state(oracle, points, 13, seed=10)
@@ -365,7 +392,7 @@ def test_route_b_mixed_precision_matches_the_oracle(device, oracle):
     assert json.loads(d["report"])["precision"] == "float"
 
 
-@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("precision", ["double", "mixed", "float"])
 def test_route_c_matches_the_oracle(device, oracle, precision):
     template, faces, target = _jittered_pair(seed=11)
     kwargs = dict(graph_levels=2, nodes_initial=80, growth=2)

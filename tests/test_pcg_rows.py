@@ -7,9 +7,10 @@ longer rows read the remaining columns from memory. The systems below hold
 all three kinds, at sizes that select the single-group kernel (one and two
 node slots per lane group) and the grid kernels (one and two slots). Every
 solve is checked against a sparse direct solve of the damped system it
-solves: the FP64 form the given system, the mixed form the system whose
-matrix and damping are rounded to FP32 (its FP64 refinement drives the true
-residual of that system to the tolerance).
+solves: the FP64 and mixed forms the given system (the mixed form iterates on
+the matrix rounded to FP32 and refines against the FP64 one), the all-FP32
+form the system whose matrix and damping are rounded to FP32 (its FP64
+refinement drives the true residual of that system to the tolerance).
 """
 
 from __future__ import annotations
@@ -74,12 +75,12 @@ def direct_solve(diag, edges, blocks, rhs, damping):
 # groups of two); 8,000 is the 200k-vertex job's finest level (63 groups of
 # two slots).
 @pytest.mark.parametrize("n", [40, 100, 1500, 3000, 8000])
-@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("precision", ["double", "mixed", "float"])
 def test_rows_of_every_length_match_a_direct_solve(device, n, precision):
     diag, edges, blocks, rhs, degree = random_system(n, seed=n)
     row_blocks = degree + 1
     assert (row_blocks <= 8).any() and ((row_blocks > 8) & (row_blocks <= 16)).any() and (row_blocks > 16).any()
-    if precision == "double":
+    if precision != "float":
         expected = direct_solve(diag, edges, blocks, rhs, DAMPING)
     else:
         f32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)  # noqa: E731
@@ -88,7 +89,7 @@ def test_rows_of_every_length_match_a_direct_solve(device, n, precision):
     assert r["ok"]
     got = np.asarray(r["delta"]).reshape(-1)
     np.testing.assert_allclose(got, expected, rtol=0, atol=1e-10 * np.abs(expected).max())
-    # The reported residual is the given FP64 system's: for the mixed form it
-    # is the FP32 rounding of the matrix (about 6e-8 relative) times |x|.
-    bound = 1e-11 if precision == "double" else 1e-6
+    # The reported residual is the given FP64 system's: for the all-FP32 form
+    # it is the FP32 rounding of the matrix (about 6e-8 relative) times |x|.
+    bound = 1e-6 if precision == "float" else 1e-11
     assert r["residual_norm"] <= bound * np.linalg.norm(rhs)
