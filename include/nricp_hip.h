@@ -292,9 +292,10 @@ int nricp_hip_session_step(nricp_hip_session* session, nricp_hip_progress* out_p
 int nricp_hip_session_reset(nricp_hip_session* session);
 /* Work counters since creation, in this order: PCG launches, PCG iterations,
  * PCG algorithmic bytes, assembly launches, assembly bytes, energy launches,
- * energy bytes, query launches, query rows, Gauss-Newton solves, and -- only
- * while profiling is on -- hierarchy nodes opened, triangles tested and the
- * largest per-query step count (opens + tests) of the closest-point queries.
+ * energy bytes, query launches, query rows, Gauss-Newton solves, then --
+ * counted only while profiling is on -- hierarchy nodes opened, triangles
+ * tested and the largest per-query step count (opens + tests) of the
+ * closest-point queries, and last the most PCG iterations one solve took.
  * Returns at most `capacity` values; *out_count receives the number available. */
 int nricp_hip_session_counters(const nricp_hip_session* session, double* out, int32_t capacity,
                                int32_t* out_count);

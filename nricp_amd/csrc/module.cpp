@@ -910,7 +910,7 @@ public:
         static const char* names[] = {"pcg_launches", "pcg_iterations", "pcg_bytes", "assemble_launches",
                                       "assemble_bytes", "energy_launches", "energy_bytes", "query_launches",
                                       "query_rows", "gauss_newton_solves", "query_node_opens",
-                                      "query_leaf_tests", "query_max_steps"};
+                                      "query_leaf_tests", "query_max_steps", "pcg_max_iterations"};
         double v[16];
         int32_t n = 0;
         check(nricp_hip_session_counters(handle_, v, 16, &n));

@@ -663,6 +663,7 @@ bool Session::inner_step() {
             const double P = static_cast<double>(layout_.group_count), N = static_cast<double>(nn);
             counters_[kPcgLaunches] += 1.0;
             counters_[kPcgIterations] += h[10];
+            counters_[kPcgMaxIterations] = std::max(counters_[kPcgMaxIterations], h[10]);
             counters_[kPcgBytes] += h[10] * s * (36.0 * (N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
             if (settings_.precision == NRICP_HIP_PRECISION_MIXED) counters_[kPcgBytes] += 8.0 * 36.0 * (N + P);
             counters_[kEnergyLaunches] += 1.0;

@@ -102,6 +102,7 @@ enum Counter : int {
     kQueryNodeOpens,
     kQueryLeafTests,
     kQueryMaxSteps,
+    kPcgMaxIterations,  // most iterations one solve took
     kCounterCount
 };
 

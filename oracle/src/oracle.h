@@ -301,6 +301,7 @@ struct RouteSettings {
     double initial_damping = 1e-3, damping_decrease = 0.3, damping_increase = 4.0;
     int max_damping_attempts = 6;
     bool diagonal_damping = true;
+    bool progress = false;  // report each finished outer iteration on stderr (long runs)
 };
 struct LevelView {
     int level = 0, nodes = 0, outer = 0, inner = 0;

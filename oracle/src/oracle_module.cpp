@@ -276,7 +276,8 @@ LevelSettings level_settings(const py::dict& d) {
 py::dict solve_route(bool local_global, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& levels,
                  const py::object& mask, const py::object& lmi, const py::object& lmt, double dist,
                  double mincos, double slope, double d0, double ddec, double dinc, int attempts,
-                 bool diag, int graph_levels, int n0, int growth, double, int, int neighbours) {
+                 bool diag, int graph_levels, int n0, int growth, double, int, int neighbours,
+                 bool progress = false) {
     Mesh t{pts(tp, "template_points"), faces(tf, "template_faces")};
     raise_if(validate_mesh(t), "invalid template mesh");
     Mesh g{pts(gp, "target_points"), faces(gf, "target_faces")};
@@ -299,6 +300,7 @@ py::dict solve_route(bool local_global, const Dbl& tp, const Idx& tf, const Dbl&
     s.damping_increase = dinc;
     s.max_damping_attempts = attempts;
     s.diagonal_damping = diag;
+    s.progress = progress;
     const std::vector<unsigned char> m = mask_of(mask);
     std::vector<Landmark> lms;
     if (!lmi.is_none() && !lmt.is_none()) {
@@ -682,11 +684,12 @@ PYBIND11_MODULE(nricp_oracle, m) {
     m.def("solve_route_a",
           [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l, const py::object& mk,
              const py::object& li, const py::object& lt, double dist, double mincos, double slope, double d0,
-             double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double tol, int maxit, int nb) {
+             double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double tol, int maxit, int nb,
+             bool progress) {
               return solve_route(false, tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag,
-                                 gl, n0, g, tol, maxit, nb);
+                                 gl, n0, g, tol, maxit, nb, progress);
           },
-          ROUTE_ARGS);
+          ROUTE_ARGS, py::arg("progress") = false);
     // The local/global route's FP64 restatement (test checker for route C).
     m.def("solve_route_c",
           [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l, const py::object& mk,

@@ -1,6 +1,7 @@
 // FP64 CPU oracle, part 2: residual blocks, fused assembly, normal equations,
 // sparse LDL^T solve, the route-A schedule and the JSON report.
 // TEST INFRASTRUCTURE ONLY: see oracle.h for who may use it.
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <iomanip>
@@ -707,6 +708,8 @@ RouteResult solve_route_a(const Mesh& tmpl, const TargetBvh& target, const Templ
             view.energy_end = end;
             view.corr_mean = found.counts.mean;
             view.outer += 1;
+            if (s.progress)
+                std::fprintf(stderr, "[oracle] level %zu outer %d done\n", li, outer + 1);
             if (outer > 0 && rel_change(prev, end) <= ls.outer_tol) {
                 view.converged = true;
                 break;
