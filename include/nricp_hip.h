@@ -181,6 +181,31 @@ int nricp_hip_solve_system(const double* diagonal, int64_t node_count, const int
                            double tolerance, int32_t max_iterations, double* out_delta,
                            double* out_info);
 
+/* The damped block solve without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same solve as nricp_hip_solve_system, but the system is already on the
+ * device in block-sparse row form, and every workspace vector comes from one
+ * static array per GPU (about 43 MB, held until the process ends).
+ * Device arrays: row_offsets (node_count + 1), columns and blocks (block_count
+ * and block_count x 36 FP64, row-major, both (a, b) and (b, a) of every pair
+ * stored), diag_positions (node_count: the diagonal block's index), rhs and
+ * out_delta (node_count x 6). The caller guarantees a consistent structure
+ * (offsets ascending, columns and positions in range). out_info (host, 4
+ * doubles) as nricp_hip_solve_system. precision: DOUBLE, MIXED (FP32
+ * iterations, FP64 refinement) or FLOAT (the blocks rounded to FP32 are the
+ * system). Sizes above NRICP_HIP_NOALLOC_MAX_NODES nodes or
+ * NRICP_HIP_NOALLOC_MAX_BLOCKS blocks are rejected (invalid argument). stream
+ * is a hipStream_t (NULL: the default stream); the call is synchronous. Calls
+ * that overlap on one GPU (from different threads) race on its array. */
+#define NRICP_HIP_NOALLOC_MAX_NODES 16384
+#define NRICP_HIP_NOALLOC_MAX_BLOCKS 262144
+int nricp_hip_solve_bsr_noalloc(const int32_t* row_offsets, const int32_t* columns,
+                                const int32_t* diag_positions, int64_t node_count, int64_t block_count,
+                                const double* blocks, const double* rhs, double damping,
+                                int32_t diagonal_scaling, int32_t precision, double tolerance,
+                                int32_t max_iterations, double* out_delta, double* out_info, void* stream);
+
 /* ------------------------------------------------------- template graph ---- */
 
 /* Template graph on the device (build_template_graph, src/core/graph.cpp:41-73;

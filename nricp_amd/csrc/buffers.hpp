@@ -36,6 +36,9 @@ void stream_retired(hipStream_t s);
 // A synchronised stream about to be destroyed: its grid-barrier launches
 // leave the admission ledger (their events go with them).
 void launches_retired(hipStream_t s);
+// A synchronised stream that lives on: its grid-barrier launches leave the
+// ledger and their events are kept for reuse.
+void launches_completed(hipStream_t s);
 
 // Orders the buffers the calling thread allocates in this scope on `s`.
 class StreamScope {

@@ -618,6 +618,17 @@ int nricp_hip_solve_system(const double* diag, int64_t nn, const int32_t* pairs,
     });
 }
 
+int nricp_hip_solve_bsr_noalloc(const int32_t* row_offsets, const int32_t* columns, const int32_t* diag_positions,
+                                int64_t node_count, int64_t block_count, const double* blocks, const double* rhs,
+                                double damping, int32_t diagonal_scaling, int32_t precision, double tolerance,
+                                int32_t max_iterations, double* out_delta, double* out_info, void* stream) {
+    return guarded([&] {
+        solve_bsr_noalloc(row_offsets, columns, diag_positions, node_count, block_count, blocks, rhs, damping,
+                          diagonal_scaling != 0, precision, tolerance, max_iterations, out_delta, out_info,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
 int nricp_hip_graph_build(const double* points, int64_t nv, int32_t levels, int32_t n0, int32_t growth, int32_t k,
                           int32_t neighbours, nricp_hip_graph** out) {
     return guarded([&] {

@@ -332,4 +332,37 @@ void pcg_solve_mixed(const LevelLayout& layout, const float* bsr, const double* 
                      int64_t node_count, double lambda, bool diagonal_scaling, double tolerance, int max_iterations,
                      double* solution, PcgWorkspace& ws, hipStream_t stream);
 
+// The same solve on caller-provided memory (nothing allocated): the
+// block-sparse rows of the full symmetric matrix (row offsets N + 1, block
+// columns, the diagonal block's position in each row), FP64 blocks, and --
+// for the mixed form -- the same blocks rounded to FP32 (`bsr32`; null runs
+// the FP64 form). Workspace sizes, in elements of the form's iteration type
+// (FP64, or FP32 with bsr32) unless noted: damp 6N, minv 36N, vectors 12N;
+// damp64 6N doubles (mixed only); xbuf 6N doubles; slots 6 x
+// pcg_workspace_groups doubles; barrier 2 unsigned; outcome 4 doubles
+// (PcgOutcome). Throws for systems above one persistent launch.
+struct PcgWorkspaceView {
+    void* damp;
+    double* damp64;
+    void* minv;
+    void* vectors;
+    double* xbuf;
+    double* slots;
+    unsigned* barrier;
+    double* outcome;
+};
+int64_t pcg_workspace_groups(int64_t node_count, bool fp32);
+void pcg_solve_on(const int32_t* row_offsets, const int32_t* columns, const int32_t* diag_position, int64_t node_count,
+                  const double* bsr, const float* bsr32, const double* rhs, double lambda, bool diagonal_scaling,
+                  double tolerance, int max_iterations, double* solution, const PcgWorkspaceView& workspace,
+                  hipStream_t stream);
+
+// The solve above on a system already on the device, with its workspace in
+// the library's static per-device array (noalloc.hip): no device memory is
+// allocated. Synchronous; `info` (4 doubles, host) receives the PcgOutcome.
+void solve_bsr_noalloc(const int32_t* row_offsets, const int32_t* columns, const int32_t* diag_positions,
+                       int64_t node_count, int64_t block_count, const double* blocks, const double* rhs, double damping,
+                       bool diagonal_scaling, int precision, double tolerance, int max_iterations, double* delta,
+                       double* info, hipStream_t stream);
+
 }  // namespace nricp_hip

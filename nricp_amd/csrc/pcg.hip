@@ -593,69 +593,84 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
 // ones spread one slot per lane group over as many groups as needed (at most
 // the co-resident count and kPcgThreads, which the fold of the partial sums
 // assumes), growing K beyond that.
+struct PcgShape {
+    int k = 1, groups = 1;
+    bool grid = false;
+};
+
+template <typename S>
+PcgShape pcg_shape(int64_t nn) {
+    constexpr int kmax = 2;
+    const int64_t passes = (nn + kNodesPerPass - 1) / kNodesPerPass;
+    PcgShape sh;
+    if (passes <= kmax) {
+        while (sh.k < passes) sh.k *= 2;
+        return sh;
+    }
+    sh.grid = true;
+    const int limit = std::min(
+        persistent_group_budget(reinterpret_cast<const void*>(pcg_kernel<2, S, true>), kPcgThreads), kPcgThreads);
+    int64_t need = passes;
+    // A grid barrier costs more the more groups arrive at it; past
+    // kBarrierGroups groups two node slots per lane group pay off.
+    while ((need > limit || need > kBarrierGroups) && sh.k < kmax) {
+        sh.k *= 2;
+        need = (passes + sh.k - 1) / sh.k;
+    }
+    if (need > limit) throw InvalidArgument("linear system too large for one persistent launch");
+    sh.groups = static_cast<int>(need);
+    return sh;
+}
+
+// The solve on caller-provided workspace: damp (6N, S), damp64 (6N, only
+// with bsr64), minv (36N, S), vbuf (12N, S), xbuf (6N), slots (6 x groups),
+// barrier (2), outcome (4). Nothing is allocated here.
+template <typename S>
+void launch_pcg_raw(const int32_t* row_off, const int32_t* cols, const int32_t* diag_pos, int64_t nn, const S* bsr,
+                    const double* bsr64, const double* rhs, double lambda, bool scaling, double tol, double inner_tol,
+                    int max_iter, double* solution, const PcgShape& sh, S* damp, double* damp64, S* minv, S* vbuf,
+                    double* xbuf, double* slots, unsigned* bar, double* out, hipEvent_t start, hipEvent_t stop,
+                    hipStream_t stream) {
+    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, bsr64, diag_pos, nn, lambda,
+                                                                     scaling ? 1 : 0, damp, damp64, minv, bar);
+    NRICP_LAUNCH_CHECK("precondition_kernel");
+    const void* kernel = nullptr;
+    if (sh.grid) {
+        kernel = sh.k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, true>)
+                           : reinterpret_cast<const void*>(pcg_kernel<2, S, true>);
+    } else {
+        kernel = sh.k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, false>)
+                           : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
+    }
+    int64_t n = nn;
+    double* d64 = damp64;
+    void* args[] = {const_cast<int32_t**>(&row_off), const_cast<int32_t**>(&cols), const_cast<S**>(&bsr), &damp, &minv,
+                    const_cast<double**>(&bsr64), &d64, const_cast<double**>(&rhs), &n, &tol, &inner_tol, &max_iter,
+                    &vbuf, &xbuf, &slots, &bar, &solution, &out};
+    if (start) NRICP_CHECK(hipEventRecord(start, stream));
+    if (sh.grid) {
+        launch_persistent(kernel, sh.groups, kPcgThreads, args, stream);
+    } else {
+        NRICP_CHECK(hipLaunchKernel(kernel, dim3(1), dim3(kPcgThreads), args, 0, stream));
+    }
+    if (stop) NRICP_CHECK(hipEventRecord(stop, stream));
+}
+
 template <typename S>
 void launch_pcg(const LevelLayout& L, const S* bsr, const double* bsr64, const double* rhs, int64_t nn, double lambda,
                 bool scaling, double tol, double inner_tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp,
                 double* damp64, S* minv, DeviceBuffer<S>& vbuf, hipStream_t stream) {
     if (nn <= 0) return;
+    const PcgShape sh = pcg_shape<S>(nn);
     ws.barrier.resize(2);
-    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, bsr64, L.diag_position.get(), nn, lambda,
-                                                                     scaling ? 1 : 0, damp, damp64, minv,
-                                                                     ws.barrier.get());
-    NRICP_LAUNCH_CHECK("precondition_kernel");
-    constexpr int kmax = 2;
-    const int64_t passes = (nn + kNodesPerPass - 1) / kNodesPerPass;
-    int k = 1;
-    bool grid = true;
-    int groups = 1;
-    if (passes <= kmax) {
-        while (k < passes) k *= 2;
-        grid = false;
-    } else {
-        const int limit = std::min(
-            persistent_group_budget(reinterpret_cast<const void*>(pcg_kernel<2, S, true>), kPcgThreads), kPcgThreads);
-        int64_t need = passes;
-        // A grid barrier costs more the more groups arrive at it; past
-        // kBarrierGroups groups two node slots per lane group pay off.
-        while ((need > limit || need > kBarrierGroups) && k < kmax) {
-            k *= 2;
-            need = (passes + k - 1) / k;
-        }
-        if (need > limit) throw InvalidArgument("linear system too large for one persistent launch");
-        groups = static_cast<int>(need);
-    }
-    const void* kernel = nullptr;
-    if (grid) {
-        kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, true>)
-                        : reinterpret_cast<const void*>(pcg_kernel<2, S, true>);
-    } else {
-        kernel = k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, false>)
-                        : reinterpret_cast<const void*>(pcg_kernel<2, S, false>);
-    }
-    ws.reduce.resize(6 * static_cast<int64_t>(groups));
+    ws.reduce.resize(6 * static_cast<int64_t>(sh.groups));
     ws.outcome.resize(4);
     ws.xr.resize(6 * nn);
     vbuf.resize(12 * nn);
-    const int32_t* row_off = L.row_offsets.get();
-    const int32_t* cols = L.columns.get();
-    int64_t n = nn;
-    S* vb = vbuf.get();
-    double* xbuf = ws.xr.get();
-    double* slots = ws.reduce.get();
-    unsigned* bar = ws.barrier.get();
-    double* out = ws.outcome.get();
-    double* d64 = damp64;
-    void* args[] = {&row_off, &cols, const_cast<S**>(&bsr), &damp, &minv, const_cast<double**>(&bsr64), &d64,
-                    const_cast<double**>(&rhs), &n, &tol, &inner_tol, &max_iter, &vb, &xbuf, &slots, &bar,
-                    &solution, &out};
-    if (ws.kernel_start) NRICP_CHECK(hipEventRecord(ws.kernel_start, stream));
-    if (grid) {
-        launch_persistent(kernel, groups, kPcgThreads, args, stream);
-    } else {
-        NRICP_CHECK(hipLaunchKernel(kernel, dim3(1), dim3(kPcgThreads), args, 0, stream));
-    }
-    if (ws.kernel_stop) NRICP_CHECK(hipEventRecord(ws.kernel_stop, stream));
-    ws.last_groups = groups;
+    launch_pcg_raw<S>(L.row_offsets.get(), L.columns.get(), L.diag_position.get(), nn, bsr, bsr64, rhs, lambda,
+                      scaling, tol, inner_tol, max_iter, solution, sh, damp, damp64, minv, vbuf.get(), ws.xr.get(),
+                      ws.reduce.get(), ws.barrier.get(), ws.outcome.get(), ws.kernel_start, ws.kernel_stop, stream);
+    ws.last_groups = sh.groups;
 }
 
 }  // namespace
@@ -666,6 +681,30 @@ void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64
     ws.minv.resize(36 * nn);
     launch_pcg<double>(L, bsr, nullptr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
                        nullptr, ws.minv.get(), ws.u, stream);
+}
+
+int64_t pcg_workspace_groups(int64_t nn, bool fp32) {
+    if (nn <= 0) return 0;
+    return fp32 ? pcg_shape<float>(nn).groups : pcg_shape<double>(nn).groups;
+}
+
+void pcg_solve_on(const int32_t* row_off, const int32_t* cols, const int32_t* diag_pos, int64_t nn, const double* bsr,
+                  const float* bsr32, const double* rhs, double lambda, bool scaling, double tol, int max_iter,
+                  double* solution, const PcgWorkspaceView& w, hipStream_t stream) {
+    if (nn <= 0) return;
+    if (bsr32) {
+        const PcgShape sh = pcg_shape<float>(nn);
+        launch_pcg_raw<float>(row_off, cols, diag_pos, nn, bsr32, bsr, rhs, lambda, scaling, tol, std::max(tol, 1e-4),
+                              max_iter, solution, sh, static_cast<float*>(w.damp), w.damp64,
+                              static_cast<float*>(w.minv), static_cast<float*>(w.vectors), w.xbuf, w.slots, w.barrier,
+                              w.outcome, nullptr, nullptr, stream);
+    } else {
+        const PcgShape sh = pcg_shape<double>(nn);
+        launch_pcg_raw<double>(row_off, cols, diag_pos, nn, bsr, nullptr, rhs, lambda, scaling, tol, tol, max_iter,
+                               solution, sh, static_cast<double*>(w.damp), nullptr, static_cast<double*>(w.minv),
+                               static_cast<double*>(w.vectors), w.xbuf, w.slots, w.barrier, w.outcome, nullptr,
+                               nullptr, stream);
+    }
 }
 
 void pcg_solve_mixed(const LevelLayout& L, const float* bsr, const double* bsr64, const double* rhs, int64_t nn,

@@ -405,6 +405,22 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
     r.in_flight.push_back(Admitted{device, charge, done, stream});
 }
 
+void launches_completed(hipStream_t stream) {
+    Residency& r = residency();
+    std::lock_guard<std::mutex> lock(r.mutex);
+    // The caller has synchronised the stream: its launches are complete, and
+    // their events go back to the spares for the next admissions.
+    for (auto it = r.in_flight.begin(); it != r.in_flight.end();) {
+        if (it->stream == stream) {
+            r.used[it->device] -= it->charge;
+            r.spare.push_back(Spare{it->device, it->done, it->stream});
+            it = r.in_flight.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 void launches_retired(hipStream_t stream) {
     Residency& r = residency();
     std::lock_guard<std::mutex> lock(r.mutex);
