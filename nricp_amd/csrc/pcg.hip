@@ -250,11 +250,11 @@ struct MatrixView {
 // copies), then its product with the column node's 6-vector into acc. An
 // FP32 block is nine loads; an FP64 block goes row by row (three loads each)
 // to bound the registers it holds.
-template <typename S, typename V, typename Acc, bool Abs = false>
+template <typename S, typename V, typename Acc>
 __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&acc)[6]) {
     Acc xv[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) xv[c] = Abs ? fabs(static_cast<Acc>(x[c])) : static_cast<Acc>(x[c]);
+    for (int c = 0; c < 6; ++c) xv[c] = static_cast<Acc>(x[c]);
     if constexpr (sizeof(S) == 4) {
         S v[36];
 #pragma unroll
@@ -269,8 +269,7 @@ __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&ac
         for (int r = 0; r < 6; ++r) {
             Acc t = Acc(0);
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                t += (Abs ? fabs(static_cast<Acc>(v[6 * r + c])) : static_cast<Acc>(v[6 * r + c])) * xv[c];
+            for (int c = 0; c < 6; ++c) t += static_cast<Acc>(v[6 * r + c]) * xv[c];
             acc[r] += t;
         }
     } else {
@@ -281,19 +280,19 @@ __device__ inline void add_block(const S* blk, const V* __restrict__ x, Acc (&ac
             const S v[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
             Acc t = Acc(0);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) t += (Abs ? fabs(static_cast<Acc>(v[k])) : static_cast<Acc>(v[k])) * xv[k];
+            for (int k = 0; k < 6; ++k) t += static_cast<Acc>(v[k]) * xv[k];
             acc[r] += t;
         }
     }
 }
 
-template <typename S, typename V, typename Acc, bool Abs = false>
+template <typename S, typename V, typename Acc>
 __device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* __restrict__ x, Acc (&acc)[6]) {
     const int32_t rel = b - m.begin;
     if (rel >= 0 && rel < m.staged)
-        add_block<S, V, Acc, Abs>(m.local + 36 * rel, x, acc);
+        add_block<S, V, Acc>(m.local + 36 * rel, x, acc);
     else
-        add_block<S, V, Acc, Abs>(m.global + 36 * static_cast<int64_t>(b), x, acc);
+        add_block<S, V, Acc>(m.global + 36 * static_cast<int64_t>(b), x, acc);
 }
 
 // (A + diag(damp)) u for the six rows of one node, evaluated by the node's
@@ -307,33 +306,51 @@ __device__ inline void add_block_at(const MatrixView<S>& m, int32_t b, const V* 
 // lane of the wave must call it (the sum crosses lanes); `valid` false adds
 // zeros. Returns row `lane` of the product (lanes 6 and 7 get row 0). Acc is
 // the accumulation precision (the matrix precision for the iteration, FP64
-// for the refinement residual). Abs: the same sum over absolute values,
-// (|A| + |D|) |u|, the scale of the product's rounding error.
-template <typename S, typename V, typename Acc, bool Abs = false>
+// for the refinement residual).
+template <typename S, typename V, typename Acc>
 __device__ inline Acc node_product(const RowShare& share, const int32_t* __restrict__ cols, const MatrixView<S>& m,
                                    S damp, V self, const V* __restrict__ u, bool valid, int lane) {
     Acc acc[6] = {Acc(0), Acc(0), Acc(0), Acc(0), Acc(0), Acc(0)};
     if (share.first >= 0) {
-        add_block_at<S, V, Acc, Abs>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
+        add_block_at<S, V, Acc>(m, share.first, u + 6 * static_cast<int64_t>(share.column), acc);
         int32_t b = share.first + 8;
         // The FP64 kernels have no registers to spare for the second column.
         if constexpr (sizeof(S) == 4) {
             if (b < share.end)
-                add_block_at<S, V, Acc, Abs>(m, b, u + 6 * static_cast<int64_t>(share.second_column), acc);
+                add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(share.second_column), acc);
             b += 8;
         }
         for (; b < share.end; b += 8)
-            add_block_at<S, V, Acc, Abs>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
+            add_block_at<S, V, Acc>(m, b, u + 6 * static_cast<int64_t>(cols[b]), acc);
     }
 #pragma unroll
     for (int r = 0; r < 6; ++r) acc[r] = octet_sum(acc[r]);
     Acc out = acc[0];
 #pragma unroll
     for (int r = 1; r < 6; ++r) out = lane == r ? acc[r] : out;
-    if (valid && lane < 6)
-        out += Abs ? fabs(static_cast<Acc>(damp)) * fabs(static_cast<Acc>(self))
-                   : static_cast<Acc>(damp) * static_cast<Acc>(self);
+    if (valid && lane < 6) out += static_cast<Acc>(damp) * static_cast<Acc>(self);
     return out;
+}
+
+// Row `lane` of (|A| + |D|) |u| for one node: the lane walks every block of
+// the node's row itself (one accumulator, no lane exchanges), so this rarely
+// needed product holds few registers. Lanes past the node's six rows, and
+// rows of absent nodes, give 0.
+template <typename S>
+__device__ inline double abs_row_product(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
+                                         const MatrixView<S>& m, S damp, double self, const double* __restrict__ u,
+                                         int64_t node, bool own, int lane) {
+    if (!own) return 0.0;
+    double acc = fabs(static_cast<double>(damp)) * fabs(self);
+    const int32_t end = row_off[node + 1];
+    for (int32_t b = row_off[node]; b < end; ++b) {
+        const int32_t rel = b - m.begin;
+        const S* blk = (rel >= 0 && rel < m.staged) ? m.local + 36 * rel : m.global + 36 * static_cast<int64_t>(b);
+        const double* x = u + 6 * static_cast<int64_t>(cols[b]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc += fabs(static_cast<double>(blk[6 * lane + c])) * fabs(x[c]);
+    }
+    return acc;
 }
 
 // Rounding-error factor of an FP64 residual row b - (A + D) x: a row sums at
@@ -525,36 +542,27 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             if (own[k]) xbuf[6 * node[k] + lane] = x64[k];
         }
         sync_all<Grid>(barrier);
-        double rr = 0.0, ee = 0.0;
+        double rr = 0.0;
         double r64[K];
+        // The residual of the FP64 system when its matrix is given (the mixed
+        // form: FP32 iterations refine the FP64 solution), else of the
+        // iteration's own matrix.
+        const MatrixView<double> m64{bsr64, nullptr, 0, 0};
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            // The residual of the FP64 system when its matrix is given (the
-            // mixed form: FP32 iterations refine the FP64 solution), else of
-            // the iteration's own matrix; and the scale of its rounding error.
-            double v, a;
-            if (bsr64) {
-                const MatrixView<double> m64{bsr64, nullptr, 0, 0};
-                v = node_product<double, double, double>(share[k], cols, m64, damp64k[k], x64[k], xbuf, live[k],
-                                                         lane);
-                a = node_product<double, double, double, true>(share[k], cols, m64, damp64k[k], x64[k], xbuf,
-                                                               live[k], lane);
-            } else {
-                v = node_product<S, double, double>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
-                a = node_product<S, double, double, true>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
-            }
+            const double v =
+                bsr64 ? node_product<double, double, double>(share[k], cols, m64, damp64k[k], x64[k], xbuf, live[k],
+                                                             lane)
+                      : node_product<S, double, double>(share[k], cols, mat, dampk[k], x64[k], xbuf, live[k], lane);
             r64[k] = own[k] ? b64[k] - v : 0.0;
             rr += r64[k] * r64[k];
-            const double e = own[k] ? fabs(b64[k]) + a : 0.0;
-            ee += e * e;
         }
         {
-            const double v[2] = {rr, ee};
-            double o[2];
-            launch_sum<2, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            const double v[1] = {rr};
+            double o[1];
+            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
             phase ^= 1;
             red[0] = o[0];
-            red[1] = o[1];
         }
         const double tnorm = sqrt(red[0]);
         rel = bnorm > 0.0 ? tnorm / bnorm : 0.0;
@@ -563,12 +571,27 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             break;
         }
         // A cycle that no longer reduces the true residual ends the solve. If
-        // the residual is then within its own rounding error, the solution is
-        // as accurate as FP64 evaluates it (a tolerance below that floor
-        // cannot be met by any solver); the solve counts as converged, as a
-        // direct solve of the same system would.
-        if (!(tnorm < cycle_norm)) {
-            converged = tnorm <= kResidualGamma * sqrt(red[1]);
+        // the residual is then within its own rounding error,
+        // gamma (|b| + (|A| + |D|) |x|), the solution is as accurate as FP64
+        // evaluates it (a tolerance below that floor cannot be met by any
+        // solver); the solve counts as converged, as a direct solve of the same
+        // system would. Every group takes the same branch (the sums are the
+        // same bits everywhere), so the extra sum's barrier is matched.
+        if (__builtin_expect(!(tnorm < cycle_norm), 0)) {
+            double ee = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double a =
+                    bsr64 ? abs_row_product<double>(row_off, cols, m64, damp64k[k], x64[k], xbuf, node[k], own[k], lane)
+                          : abs_row_product<S>(row_off, cols, mat, dampk[k], x64[k], xbuf, node[k], own[k], lane);
+                const double e = own[k] ? fabs(b64[k]) + a : 0.0;
+                ee += e * e;
+            }
+            const double v[1] = {ee};
+            double o[1];
+            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            phase ^= 1;
+            converged = tnorm <= kResidualGamma * sqrt(o[0]);
             break;
         }
         cycle_norm = tnorm;
