@@ -486,6 +486,9 @@ def main(argv=None) -> int:
         print(text, flush=True)
         if args.json_out:
             Path(args.json_out).write_text(text + "\n")
+    # The library's cached device memory and events go back before the
+    # runtime (and a profiler, when one is attached) shuts down.
+    dp.release_cache()
     return 0
 
 

@@ -47,6 +47,13 @@ int nricp_hip_device_info(int32_t* available, char* description, size_t capacity
  * benchmark brackets its timed region with it). */
 int nricp_hip_device_synchronize(void);
 
+/* Drains the devices, returns the device memory the library caches to the
+ * runtime and destroys its internal events; memory released afterwards is
+ * left to the process's end. For a caller about to tear the HIP runtime down
+ * (no reference counterpart); no other call of the library may run
+ * meanwhile, and the library should not be used afterwards. */
+int nricp_hip_release_cache(void);
+
 /* build_info() values (src/core/build_info.h:10-15): keys "version",
  * "build_type", "cxx_compiler", "hip_runtime", "offload_arch",
  * "accelerated_precision". Returns "" for an unknown key. */

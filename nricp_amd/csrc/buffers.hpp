@@ -39,6 +39,13 @@ void launches_retired(hipStream_t s);
 // A synchronised stream that lives on: its grid-barrier launches leave the
 // ledger and their events are kept for reuse.
 void launches_completed(hipStream_t s);
+// The device drained: the ledger's events are destroyed.
+void launches_teardown();
+// Drains every device, returns every cached block to the runtime and destroys
+// the launch ledger's events; blocks released afterwards are left to the
+// process's end. For callers about to tear the HIP runtime down; no other
+// library call may run meanwhile.
+void release_all();
 
 // Orders the buffers the calling thread allocates in this scope on `s`.
 class StreamScope {

@@ -94,6 +94,10 @@ int nricp_hip_device_info(int32_t* available, char* description, size_t capacity
     });
 }
 
+int nricp_hip_release_cache(void) {
+    return guarded([&] { release_all(); });
+}
+
 int nricp_hip_device_synchronize(void) {
     return guarded([&] { NRICP_CHECK(hipDeviceSynchronize()); });
 }

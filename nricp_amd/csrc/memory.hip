@@ -11,39 +11,26 @@
 // another stream at once would let that stream's kernels overwrite it while
 // the first stream's are still queued. A retired stream (synchronised, then
 // destroyed) has no work left, so its blocks become idle and any stream may
-// take them. Memory returns to the device only when an allocation would
-// otherwise fail.
+// take them (the policy itself is StreamOrderedPool, stream_pool.hpp). Memory
+// returns to the device when an allocation would otherwise fail, or when the
+// caller asks for it (release_all, before it tears the runtime down).
 #include <hip/hip_runtime.h>
 
-#include <map>
+#include <atomic>
 #include <mutex>
-#include <set>
-#include <tuple>
-#include <vector>
 
 #include "buffers.hpp"
+#include "stream_pool.hpp"
 
 namespace nricp_hip {
 namespace {
 
-// Powers of two up to 1 MiB, then whole MiB.
-int64_t size_class(int64_t bytes) {
-    if (bytes <= 256) return 256;
-    constexpr int64_t kMiB = int64_t{1} << 20;
-    if (bytes <= kMiB) {
-        int64_t c = 256;
-        while (c < bytes) c <<= 1;
-        return c;
-    }
-    return (bytes + kMiB - 1) / kMiB * kMiB;
-}
-
-using Key = std::tuple<int, hipStream_t, int64_t>;  // (device, stream or idle, class)
+using Pool = StreamOrderedPool<hipStream_t, void*>;
 
 struct Cache {
     std::mutex mutex;
-    std::multimap<Key, void*> free;
-    std::set<hipStream_t> live;  // streams created and not yet retired
+    Pool pool;
+    std::atomic<bool> closed{false};  // after release_all: later releases are left to the process's end
 };
 
 // Never destroyed: buffers with static storage may be released after every
@@ -56,44 +43,25 @@ Cache& cache() {
 thread_local int64_t t_bytes = 0;
 thread_local hipStream_t t_stream = nullptr;
 
-// Takes a block of `cls` bytes for `stream` (its own free list first, then
-// the idle one); nullptr if none is cached. Caller holds the mutex.
-void* take(int device, hipStream_t stream, int64_t cls) {
-    auto& f = cache().free;
-    auto it = stream ? f.find(Key{device, stream, cls}) : f.end();
-    if (it == f.end()) it = f.find(Key{device, nullptr, cls});
-    if (it == f.end()) return nullptr;
-    void* p = it->second;
-    f.erase(it);
-    return p;
-}
-
 // Returns every cached block of `device` to the runtime, after the device
 // has drained (blocks on a stream's free list may still have work queued).
 void trim(int device) {
     (void)hipDeviceSynchronize();
     std::lock_guard<std::mutex> lock(cache().mutex);
-    for (auto it = cache().free.begin(); it != cache().free.end();) {
-        if (std::get<0>(it->first) == device) {
-            (void)hipFree(it->second);
-            it = cache().free.erase(it);
-        } else {
-            ++it;
-        }
-    }
+    cache().pool.drain(device, [](int, void* p) { (void)hipFree(p); });
 }
 
 }  // namespace
 
 void* device_allocate(int64_t bytes, int64_t* granted, int* device_out, hipStream_t* stream_out) {
-    const int64_t cls = size_class(bytes);
+    const int64_t cls = Pool::size_class(bytes);
     int device = 0;
     NRICP_CHECK(hipGetDevice(&device));
     const hipStream_t stream = t_stream;
     void* p = nullptr;
     {
         std::lock_guard<std::mutex> lock(cache().mutex);
-        p = take(device, stream, cls);
+        if (!cache().closed && !cache().pool.take(device, stream, cls, &p)) p = nullptr;
     }
     if (!p) {
         hipError_t e = hipMalloc(&p, static_cast<size_t>(cls));
@@ -114,6 +82,7 @@ void* device_allocate(int64_t bytes, int64_t* granted, int* device_out, hipStrea
 void device_release(void* p, int64_t bytes, int device, hipStream_t stream) {
     if (!p) return;
     t_bytes -= bytes;
+    if (cache().closed) return;  // after release_all: the process's end frees it
     if (!stream) {
         // Ordered on no known stream: the block is idle once the device has
         // drained.
@@ -124,9 +93,7 @@ void device_release(void* p, int64_t bytes, int device, hipStream_t stream) {
         if (current != device) (void)hipSetDevice(current);
     }
     std::lock_guard<std::mutex> lock(cache().mutex);
-    // A stream retired since the allocation has finished all its work.
-    const hipStream_t owner = stream && cache().live.count(stream) ? stream : nullptr;
-    cache().free.emplace(Key{device, owner, bytes}, p);
+    cache().pool.release(device, stream, bytes, p);
 }
 
 int64_t thread_device_bytes() { return t_bytes; }
@@ -139,23 +106,33 @@ hipStream_t swap_allocation_stream(hipStream_t s) {
 
 void stream_created(hipStream_t s) {
     std::lock_guard<std::mutex> lock(cache().mutex);
-    cache().live.insert(s);
+    cache().pool.stream_created(s);
 }
 
 void stream_retired(hipStream_t s) {
     std::lock_guard<std::mutex> lock(cache().mutex);
-    cache().live.erase(s);
-    auto& f = cache().free;
-    std::vector<std::pair<Key, void*>> moved;
-    for (auto it = f.begin(); it != f.end();) {
-        if (std::get<1>(it->first) == s) {
-            moved.emplace_back(Key{std::get<0>(it->first), nullptr, std::get<2>(it->first)}, it->second);
-            it = f.erase(it);
-        } else {
-            ++it;
-        }
+    cache().pool.stream_retired(s);
+}
+
+void release_all() {
+    Cache& c = cache();
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+    int current = 0;
+    (void)hipGetDevice(&current);
+    for (int d = 0; d < count; ++d) {
+        if (hipSetDevice(d) == hipSuccess) (void)hipDeviceSynchronize();
     }
-    for (auto& m : moved) f.emplace(m.first, m.second);
+    {
+        std::lock_guard<std::mutex> lock(c.mutex);
+        c.closed = true;
+        c.pool.drain(-1, [](int device, void* p) {
+            (void)hipSetDevice(device);
+            (void)hipFree(p);
+        });
+    }
+    (void)hipSetDevice(current);
+    launches_teardown();
 }
 
 }  // namespace nricp_hip

@@ -1086,6 +1086,9 @@ PYBIND11_MODULE(nricp_dp, m) {
         return info;
     });
 
+    // Returns the library's cached device memory before the process ends (no
+    // reference counterpart; the library is not used afterwards).
+    m.def("release_cache", [] { check(nricp_hip_release_cache()); });
     m.def("device_synchronize", [] {
         int st;
         {

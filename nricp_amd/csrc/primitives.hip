@@ -405,6 +405,16 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
     r.in_flight.push_back(Admitted{device, charge, done, stream});
 }
 
+void launches_teardown() {
+    Residency& r = residency();
+    std::lock_guard<std::mutex> lock(r.mutex);
+    for (const Admitted& a : r.in_flight) (void)hipEventDestroy(a.done);
+    for (const Spare& sp : r.spare) (void)hipEventDestroy(sp.event);
+    r.in_flight.clear();
+    r.spare.clear();
+    r.used.clear();
+}
+
 void launches_completed(hipStream_t stream) {
     Residency& r = residency();
     std::lock_guard<std::mutex> lock(r.mutex);

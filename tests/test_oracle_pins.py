@@ -254,3 +254,54 @@ def test_route_a_without_anchor_reproduces_a_rigid_motion(impl):
     # The fitted node rotations are the motion's.
     np.testing.assert_allclose(np.asarray(out["rotations"]), np.repeat(R[None], len(out["rotations"]), 0),
                                rtol=0, atol=1e-3)
+
+
+# ---------------------------------------------------- level transfer -------
+
+def _transfer_case(seed, angles):
+    rng = np.random.default_rng(seed)
+    from_nodes = rng.normal(size=(len(angles), 3))
+    to_nodes = rng.normal(size=(60, 3))
+    axis = np.array([0.3, -0.4, 0.85])
+    axis /= np.linalg.norm(axis)
+    rotations = np.stack([rotation_about(axis, a) for a in angles])
+    translations = rng.normal(scale=0.2, size=(len(angles), 3))
+    return from_nodes, to_nodes, axis, np.asarray(angles, dtype=np.float64), rotations, translations
+
+
+def _expected_transfer(oracle, from_nodes, to_nodes, axis, angles, rotations, translations):
+    """Closed form of the level warm start (src/core/deformation.cpp:141-203)
+    for rotations about one axis: each new node's quaternion is the
+    normalised weighted sum of its coarse nodes' quaternions, signed towards
+    the heaviest one, so its angle is 2 atan2(sum w sin(a/2), sum w cos(a/2))
+    over the angles unwrapped to within pi of the heaviest node's; its
+    translation is sum w T_j(g) - g."""
+    b = oracle.bind_vertices(to_nodes, from_nodes, 4)
+    R_out, t_out = [], []
+    for i, g in enumerate(to_nodes):
+        count = int(b["counts"][i])
+        nodes, weights = b["nodes"][i][:count], b["weights"][i][:count]
+        heavy = nodes[int(np.argmax(weights))]
+        a = angles[nodes] + 2 * np.pi * np.round((angles[heavy] - angles[nodes]) / (2 * np.pi))
+        angle = 2 * np.arctan2(np.sum(weights * np.sin(a / 2)), np.sum(weights * np.cos(a / 2)))
+        R_out.append(rotation_about(axis, angle))
+        moved = [rotations[j] @ (g - from_nodes[j]) + from_nodes[j] + translations[j] for j in nodes]
+        t_out.append(np.sum(weights[:, None] * np.asarray(moved), axis=0) - g)
+    return np.asarray(R_out), np.asarray(t_out)
+
+
+@pytest.mark.parametrize("kind", ["shared rotation", "one axis", "across the half turn"])
+def test_level_transfer_follows_the_weighted_quaternion_blend(impl, oracle, kind):
+    rng = np.random.default_rng(31)
+    if kind == "shared rotation":
+        angles = np.full(20, 0.7)
+    elif kind == "one axis":
+        angles = rng.uniform(-1.2, 1.2, size=20)
+    else:  # angles either side of +-pi: the sign alignment decides the blend
+        angles = np.where(rng.random(20) < 0.5, rng.uniform(2.9, 3.1, size=20), rng.uniform(-3.1, -2.9, size=20))
+    case = _transfer_case(40, angles)
+    from_nodes, to_nodes, _, _, rotations, translations = case
+    R_exp, t_exp = _expected_transfer(oracle, *case)
+    out = impl.transfer_level(from_nodes, rotations, translations, to_nodes)
+    np.testing.assert_allclose(np.asarray(out["rotations"]), R_exp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.asarray(out["translations"]), t_exp, rtol=0, atol=1e-12)
