@@ -352,6 +352,9 @@ struct PcgWorkspaceView {
     double* outcome;
 };
 int64_t pcg_workspace_groups(int64_t node_count, bool fp32);
+// The most nodes one solve takes on the current device (one persistent
+// launch: 16,384 on a 256-compute-unit MI355X).
+int64_t pcg_max_nodes();
 void pcg_solve_on(const int32_t* row_offsets, const int32_t* columns, const int32_t* diag_position, int64_t node_count,
                   const double* bsr, const float* bsr32, const double* rhs, double lambda, bool diagonal_scaling,
                   double tolerance, int max_iterations, double* solution, const PcgWorkspaceView& workspace,

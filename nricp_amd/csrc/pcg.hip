@@ -706,6 +706,15 @@ void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64
                        nullptr, ws.minv.get(), ws.u, stream);
 }
 
+int64_t pcg_max_nodes() {
+    // Both kernel precisions share the launch bound: two node slots per lane
+    // group, at most the co-resident group count and kPcgThreads groups.
+    const int limit = std::min(
+        persistent_group_budget(reinterpret_cast<const void*>(pcg_kernel<2, float, true>), kPcgThreads),
+        persistent_group_budget(reinterpret_cast<const void*>(pcg_kernel<2, double, true>), kPcgThreads));
+    return static_cast<int64_t>(2) * std::min(limit, kPcgThreads) * kNodesPerPass;
+}
+
 int64_t pcg_workspace_groups(int64_t nn, bool fp32) {
     if (nn <= 0) return 0;
     return fp32 ? pcg_shape<float>(nn).groups : pcg_shape<double>(nn).groups;

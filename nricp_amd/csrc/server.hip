@@ -77,6 +77,7 @@ int64_t Server::submit(const nricp_hip_problem& pb, const nricp_hip_settings& s)
         !pb.target_faces)
         throw InvalidArgument("server: the job needs both meshes");
     if (s.level_settings_count > 0 && !s.levels) throw InvalidArgument("server: level settings pointer is null");
+    check_level_sizes(pb, s, nullptr);  // a job too large to solve fails here, not after admission
     auto job = std::make_shared<ServerJob>();
     job->template_points = copy_of(pb.template_points, 3 * pb.template_vertex_count);
     job->template_faces = copy_of(pb.template_faces, 3 * pb.template_face_count);

@@ -254,6 +254,24 @@ KernelClock::~KernelClock() {
     for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
 }
 
+// ------------------------------------------------------------ admission --
+
+void check_level_sizes(const nricp_hip_problem& pb, const nricp_hip_settings& s, const DeviceGraph* graph) {
+    int64_t finest = 0;
+    if (graph) {
+        for (const GraphLevel& L : graph->levels) finest = std::max(finest, L.node_count);
+    } else {
+        const int lv = std::max(1, s.graph_levels);
+        for (int l = 0; l < lv; ++l)
+            finest = std::max(finest, level_count_for(lv, s.nodes_initial, s.growth, pb.template_vertex_count, l));
+    }
+    const int64_t limit = pcg_max_nodes();
+    if (finest > limit)
+        throw InvalidArgument("the finest graph level has " + std::to_string(finest) + " nodes; one solve on this "
+                              "device takes at most " + std::to_string(limit) + " (lower nodes_initial, growth or "
+                              "graph_levels)");
+}
+
 // ---------------------------------------------------------------- session --
 
 Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const DeviceGraph* graph) : settings_(s) {
@@ -273,6 +291,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         throw InvalidArgument("unknown precision");
     report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
     report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
+    check_level_sizes(pb, s, graph);
     hipStream_t st = stream_.get();
     StreamScope scope(st);  // the session's buffers are ordered on its stream
     nv_ = pb.template_vertex_count;

@@ -44,6 +44,12 @@ void build_graph_level(const double* points, int64_t vertex_count, const int32_t
                        int max_nodes_per_vertex, int neighbours, GraphLevel& level, SortScratch& scratch,
                        hipStream_t stream);
 
+// Throws InvalidArgument when the finest level of the job's graph (the given
+// one, or the one its settings ask for) has more nodes than one solve takes on
+// this device (pcg_max_nodes), so the job fails before any work, not midway.
+void check_level_sizes(const nricp_hip_problem& problem, const nricp_hip_settings& settings,
+                       const DeviceGraph* graph);
+
 // Report of a run (RunReport, src/core/report.h:12-59).
 struct LevelView {
     int level = 0, nodes = 0, outer = 0, inner = 0;

@@ -284,3 +284,24 @@ def test_the_device_graph_builder_is_the_oracles_bit_for_bit(device, oracle):
                 assert np.array_equal(np.asarray(left[key]), np.asarray(right[key])), key
             for key in ("nodes", "weights", "counts"):
                 assert np.array_equal(np.asarray(left["bindings"][key]), np.asarray(right["bindings"][key])), key
+
+
+@pytest.mark.gpu
+def test_a_level_too_large_for_one_solve_fails_at_construction(device):
+    """One solve takes at most 16,384 nodes on this device (a persistent
+    launch of two node slots per lane group); a schedule whose finest level
+    is larger is refused when the session or the server job is created, not
+    after the coarse levels have run."""
+    from nricp_amd import workloads
+
+    template, faces = workloads.geodesic_sphere(41)  # 16,812 vertices
+    assert len(template) > 16_384
+    levels = workloads.level_settings(1, outer=1)
+    with pytest.raises(ValueError, match="at most 16384"):
+        device.RouteBSession(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_500)
+    with device.Server(max_active=1) as server:
+        with pytest.raises(ValueError, match="at most 16384"):
+            server.submit(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_500)
+    # Just inside the limit the session is created.
+    session = device.RouteBSession(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_384)
+    assert not session.finished()
