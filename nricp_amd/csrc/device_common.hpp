@@ -181,10 +181,11 @@ __device__ inline void block_sum(const double (&v)[Count], double* lds /* Count 
 //     event recorded after it completes) leave a compute unit for every
 //     group on every XCD (a launch of g groups is charged ceil(g / xcds) *
 //     xcds, the most any XCD receives when groups are dealt to XCDs in turn);
-//   * every other kernel runs to completion without waiting on anything (the
-//     sampler is one workgroup), so the groups of an admitted launch that are
-//     not yet resident become resident once the ordinary kernels ahead of them
-//     retire.
+//   * every other kernel runs to completion without waiting on anything
+//     outside itself (the sampler is one workgroup; a sort pass's workgroup
+//     waits only on workgroups of its own launch that started before it), so
+//     the groups of an admitted launch that are not yet resident become
+//     resident once the ordinary kernels ahead of them retire.
 int persistent_group_budget(const void* kernel, int threads);  // groups one persistent launch may use
 void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream);
 

@@ -1,10 +1,15 @@
 // Stable LSD radix sort, exclusive scan, CSR offsets and run flags for gfx950.
 //
-// Sort: 8-bit digits. Each pass is a per-tile digit histogram in LDS, one
-// exclusive scan over the digit-major table of tile counts, and a stable
-// scatter in which every wave ranks its keys with eight
-// ballots (the lanes holding the same digit) and waves are ordered through LDS.
-// A tile is 256 threads x 8 rounds = 2048 keys.
+// Sort: 8-bit digits, one launch per digit pass after one histogram launch
+// per sort (a single-pass "onesweep" scheme: Merrill and Garland's decoupled
+// look-back, applied per digit). The histogram launch counts every pass's
+// digits in one read of the keys (the multiset of keys is the same in every
+// pass). Each pass's launch takes its tiles in the order workgroups start
+// (a ticket), counts its tile's digits, publishes them, and walks back over
+// the tiles before it for each digit's running offset; then it scatters its
+// keys stably (every wave ranks its keys with eight ballots, the lanes
+// holding the same digit, and waves are ordered through LDS). A tile is 256
+// threads x 8 rounds = 2048 keys.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,46 +32,120 @@ constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 8;
 constexpr int kTile = kSortThreads * kSortRounds;
 constexpr int kRadix = 256;
-
-__global__ void __launch_bounds__(kSortThreads)
-digit_histogram(const uint64_t* __restrict__ keys, int64_t n, int shift, uint32_t* __restrict__ counts,
-                int tiles) {
-    __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t begin = static_cast<int64_t>(blockIdx.x) * kTile;
-    const int64_t end = begin + kTile < n ? begin + kTile : n;
-    for (int64_t i = begin + threadIdx.x; i < end; i += kSortThreads) {
-        atomicAdd(&h[(keys[i] >> shift) & 0xff], 1u);
-    }
-    __syncthreads();
-    counts[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
-}
+constexpr int kSortHistogramBlocks = 256;
 
 __device__ inline uint64_t lanemask_lt() {
     const int lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Every pass's digit counts in one read of the keys: per-workgroup counts in
+// LDS, then one atomic add per nonzero bin into this sort's table (zeroed by
+// the previous sort's histogram launch, as this launch's workgroup 0 zeroes
+// the other table -- counts and tickets -- for the next sort).
 __global__ void __launch_bounds__(kSortThreads)
-digit_scatter(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
-              const uint32_t* __restrict__ offsets, int tiles, uint64_t* __restrict__ out_keys,
-              uint32_t* __restrict__ out_vals) {
+sort_histogram(const uint64_t* __restrict__ keys, int64_t n, int passes, uint32_t last_mask,
+               uint32_t* __restrict__ table, uint32_t* __restrict__ next_table) {
+    __shared__ uint32_t h[kSortMaxPasses][kRadix];
+    for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < kSortTableWords; k += kSortThreads) next_table[k] = 0;
+    __syncthreads();
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSortThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kSortThreads) {
+        const uint64_t key = keys[i];
+        for (int p = 0; p < passes; ++p)
+            atomicAdd(&h[p][(key >> (8 * p)) & (p == passes - 1 ? last_mask : 0xffu)], 1u);
+    }
+    __syncthreads();
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t c = h[p][threadIdx.x];
+        if (c) atomicAdd(&table[p * kRadix + threadIdx.x], c);
+    }
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; returns
+// the block total through `total`.
+template <typename T>
+__device__ inline T block_exclusive(T v, T* wave_totals, T& total);
+
+// Look-back words, one per (tile, digit): the pass's epoch in the high 32
+// bits (a word from any other pass or sort never matches), then a flag (the
+// count is the digit's inclusive prefix up to this tile, else this tile's own
+// count) and the count in the low 31 bits.
+constexpr uint64_t kPrefixFlag = 1ull << 31;
+constexpr uint64_t kCountMask = kPrefixFlag - 1;
+// A wait on a tile that started earlier always ends (that tile is resident
+// and waits only on tiles before it); the cap only bounds a wait that a
+// malformed launch could leave open.
+constexpr uint32_t kLookbackSpinCap = 1u << 20;
+
+__device__ inline void publish(uint64_t* word, uint64_t value) {
+    __hip_atomic_store(word, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+onesweep_scatter(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
+                 uint32_t digit_mask, const uint32_t* __restrict__ counts, uint32_t* __restrict__ ticket,
+                 uint64_t* __restrict__ state,
+                 uint32_t epoch, uint64_t* __restrict__ out_keys, uint32_t* __restrict__ out_vals) {
     constexpr int waves = kSortThreads / 64;
     __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t tile_count[kRadix];
     __shared__ uint32_t wave_count[waves][kRadix];
-    base[threadIdx.x] = offsets[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x];
-    const int wave = threadIdx.x / 64;
-    const uint64_t below = lanemask_lt();
-    const int64_t tile_begin = static_cast<int64_t>(blockIdx.x) * kTile;
-    for (int round = 0; round < kSortRounds; ++round) {
-        for (int w = 0; w < waves; ++w) wave_count[w][threadIdx.x] = 0;
-        __syncthreads();
-        const int64_t i = tile_begin + static_cast<int64_t>(round) * kSortThreads + threadIdx.x;
+    __shared__ uint32_t wave_totals[waves];
+    __shared__ uint32_t tile_slot;
+    const int d = threadIdx.x;
+    if (d == 0) tile_slot = atomicAdd(ticket, 1u);
+    tile_count[d] = 0;
+    uint32_t total;
+    uint32_t offset = block_exclusive(counts[d], wave_totals, total);  // digits below d, whole input
+    __syncthreads();
+    const int64_t tile = tile_slot;
+    const int64_t tile_begin = tile * kTile;
+    uint64_t key[kSortRounds];
+    uint32_t val[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int64_t i = tile_begin + static_cast<int64_t>(r) * kSortThreads + d;
         const bool valid = i < n;
-        const uint64_t key = valid ? keys[i] : 0ull;
-        const uint32_t val = (valid && vals) ? vals[i] : 0u;
-        const uint32_t digit = static_cast<uint32_t>((key >> shift) & 0xff);
+        key[r] = valid ? keys[i] : 0ull;
+        val[r] = (valid && vals) ? vals[i] : 0u;
+        if (valid) atomicAdd(&tile_count[(key[r] >> shift) & digit_mask], 1u);
+    }
+    __syncthreads();
+    // Thread d: digit d's offset among the tiles before this one.
+    const uint32_t mine = tile_count[d];
+    const uint64_t tag = static_cast<uint64_t>(epoch) << 32;
+    uint64_t* word = state + tile * kRadix + d;
+    if (tile == 0) {
+        publish(word, tag | kPrefixFlag | mine);
+    } else {
+        publish(word, tag | mine);
+        uint32_t before = 0;
+        for (int64_t u = tile - 1; u >= 0; --u) {
+            const uint64_t* w = state + u * kRadix + d;
+            uint64_t s = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (uint32_t spin = 0; static_cast<uint32_t>(s >> 32) != epoch && spin < kLookbackSpinCap; ++spin) {
+                __builtin_amdgcn_s_sleep(1);
+                s = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            before += static_cast<uint32_t>(s & kCountMask);
+            if (s & kPrefixFlag) break;
+        }
+        publish(word, tag | kPrefixFlag | (before + mine));
+        offset += before;
+    }
+    base[d] = offset;
+    const int wave = d / 64;
+    const uint64_t below = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        for (int w = 0; w < waves; ++w) wave_count[w][d] = 0;
+        __syncthreads();
+        const int64_t i = tile_begin + static_cast<int64_t>(r) * kSortThreads + d;
+        const bool valid = i < n;
+        const uint32_t digit = static_cast<uint32_t>((key[r] >> shift) & digit_mask);
         uint64_t same = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -78,18 +157,27 @@ digit_scatter(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ va
         if (valid && rank == 0) wave_count[wave][digit] = static_cast<uint32_t>(__popcll(same));
         __syncthreads();
         if (valid) {
-            uint32_t before = 0;
-            for (int w = 0; w < wave; ++w) before += wave_count[w][digit];
-            const uint32_t pos = base[digit] + before + rank;
-            out_keys[pos] = key;
-            if (vals) out_vals[pos] = val;
+            uint32_t ahead = 0;
+            for (int w = 0; w < wave; ++w) ahead += wave_count[w][digit];
+            const uint32_t pos = base[digit] + ahead + rank;
+            out_keys[pos] = key[r];
+            if (vals) out_vals[pos] = val[r];
         }
         __syncthreads();
-        uint32_t total = 0;
-        for (int w = 0; w < waves; ++w) total += wave_count[w][threadIdx.x];
-        base[threadIdx.x] += total;
-        __syncthreads();
+        uint32_t round_total = 0;
+        for (int w = 0; w < waves; ++w) round_total += wave_count[w][d];
+        base[d] += round_total;
     }
+}
+
+__global__ void fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void copy_pairs(const uint64_t* __restrict__ k, const uint32_t* __restrict__ v, int64_t n,
@@ -463,24 +551,53 @@ int bits_for(uint64_t v) {
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, SortScratch& s,
                       hipStream_t stream) {
     if (n <= 1 || key_bits <= 0) return;
-    if (n > 0xffffffffLL) throw InvalidArgument("radix sort: more than 2^32 keys");
-    const int tiles = blocks_for(n, kTile);
+    if (n > static_cast<int64_t>(kCountMask)) throw InvalidArgument("radix sort: 2^31 keys or more");
+    key_bits = std::min(key_bits, 64);
+    const int passes = (key_bits + 7) / 8;
+    // The last pass's digit holds only the key bits left (so bits above
+    // key_bits never order anything).
+    const uint32_t last_mask = (1u << (key_bits - 8 * (passes - 1))) - 1u;
+    const int64_t tiles = (n + kTile - 1) / kTile;
     s.keys_alt.resize(n);
     if (vals) s.vals_alt.resize(n);
-    s.counts.resize(static_cast<int64_t>(kRadix) * tiles);
+    if (s.tables.size() == 0) {
+        // Both tables start at zero; from then on each sort's histogram
+        // launch clears the table the next sort uses.
+        s.tables.resize(2 * kSortTableWords);
+        fill_u32<<<blocks_for(2 * kSortTableWords, 256), 256, 0, stream>>>(s.tables.get(), 2 * kSortTableWords, 0u);
+        NRICP_LAUNCH_CHECK("fill_u32");
+        s.parity = 0;
+    }
+    const int64_t words = tiles * kRadix;
+    if (words > s.state_words) {
+        // Fresh look-back words hold no epoch (epochs start at 1).
+        s.state.resize(words);
+        fill_u64<<<blocks_for(words, 256), 256, 0, stream>>>(s.state.get(), words, 0ull);
+        NRICP_LAUNCH_CHECK("fill_u64");
+        s.state_words = words;
+    }
+    uint32_t* table = s.tables.get() + s.parity * kSortTableWords;
+    uint32_t* next = s.tables.get() + (1 - s.parity) * kSortTableWords;
+    s.parity = 1 - s.parity;
+    const int hist_blocks = static_cast<int>(std::min<int64_t>(tiles, kSortHistogramBlocks));
+    sort_histogram<<<hist_blocks, kSortThreads, 0, stream>>>(keys, n, passes, last_mask, table, next);
+    NRICP_LAUNCH_CHECK("sort_histogram");
     uint64_t* kin = keys;
     uint32_t* vin = vals;
     uint64_t* kout = s.keys_alt.get();
     uint32_t* vout = vals ? s.vals_alt.get() : nullptr;
-    const int passes = (key_bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        digit_histogram<<<tiles, kSortThreads, 0, stream>>>(kin, n, shift, s.counts.get(), tiles);
-        NRICP_LAUNCH_CHECK("digit_histogram");
-        scan_exclusive<uint32_t>(s.counts.get(), s.counts.get(), static_cast<int64_t>(kRadix) * tiles, nullptr,
-                                 s.scan_partials, stream);
-        digit_scatter<<<tiles, kSortThreads, 0, stream>>>(kin, vin, n, shift, s.counts.get(), tiles, kout, vout);
-        NRICP_LAUNCH_CHECK("digit_scatter");
+        if (++s.epoch == 0) {
+            // 2^32 passes on this scratch: clear the words before reusing epochs.
+            fill_u64<<<blocks_for(s.state_words, 256), 256, 0, stream>>>(s.state.get(), s.state_words, 0ull);
+            NRICP_LAUNCH_CHECK("fill_u64");
+            s.epoch = 1;
+        }
+        onesweep_scatter<<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>(
+            kin, vin, n, 8 * p, p == passes - 1 ? last_mask : 0xffu, table + p * kRadix,
+            table + kSortMaxPasses * kRadix + p, s.state.get(), s.epoch,
+            kout, vout);
+        NRICP_LAUNCH_CHECK("onesweep_scatter");
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

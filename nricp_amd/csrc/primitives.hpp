@@ -12,15 +12,26 @@
 
 namespace nricp_hip {
 
+// Digit passes of one sort (64-bit keys, 8-bit digits) and the words of one
+// sort's table: the digit counts of every pass, then one tile ticket per pass.
+constexpr int kSortMaxPasses = 8;
+constexpr int kSortTableWords = kSortMaxPasses * 256 + kSortMaxPasses;
+
+// Working memory of the sort and the scans. One scratch serves one stream at
+// a time: sorts on it run in that stream's order.
 struct SortScratch {
     DeviceBuffer<uint64_t> keys_alt;
     DeviceBuffer<uint32_t> vals_alt;
-    DeviceBuffer<uint32_t> counts;
+    DeviceBuffer<uint32_t> tables;   // two sort tables, used in turn
+    DeviceBuffer<uint64_t> state;    // look-back words, 256 per tile
+    int64_t state_words = 0;         // words of `state` cleared since it was allocated
+    uint32_t epoch = 0;              // the last digit pass's tag
+    int parity = 0;                  // the table the next sort uses
     DeviceBuffer<int64_t> scan_partials;  // tile sums of the scans
 };
 
 // Sorts keys[0..n) (and vals alongside, when non-null) by the low `key_bits`
-// bits, stably. The result is left in keys/vals.
+// bits, stably (n < 2^31). The result is left in keys/vals.
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, SortScratch& scratch,
                       hipStream_t stream);
 
