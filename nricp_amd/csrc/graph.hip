@@ -113,6 +113,10 @@ __global__ void retract_kernel(const double* __restrict__ R, const double* __res
 //     margin far above rounding), no point of the block can get closer, so
 //     the block is skipped. The wave sweeps the blocks that remain, updates
 //     their points' distances and recomputes their maxima.
+//   * The arg-max reductions run on FP32 keys (each maximum rounded down,
+//     which keeps the order) and compare exactly only the lanes that tie on
+//     the key, so a reduction step is one DPP move and one FP32 max rather
+//     than an FP64 compare-and-select of value and index.
 //
 // Late in the run a seed reaches a few blocks, so a step is one sweep of a few
 // hundred points plus two reductions. Distances are the CPU route's own
@@ -179,6 +183,48 @@ __device__ inline int owner_lane(int32_t mine, int32_t i) {
 // p of lane `lane` (the same lane for every lane of the wave).
 __device__ inline d3 lane_d3(d3 p, int lane) {
     return d3{read_lane(p.x, lane), read_lane(p.y, lane), read_lane(p.z, lane)};
+}
+
+// Maximum of `key` over the first `rows` rows of sixteen lanes, by FP32 max
+// over DPP moves: every row reduces to its maximum, then the rows are read as
+// scalars.
+__device__ inline float keys_max(float key, int rows) {
+    key = fmaxf(key, dpp_move<kDppQuadSwap1>(key));
+    key = fmaxf(key, dpp_move<kDppQuadSwap2>(key));
+    key = fmaxf(key, dpp_move<kDppRowHalfMirror>(key));
+    key = fmaxf(key, dpp_move<kDppRowMirror>(key));
+    float m = read_lane(key, 0);
+    for (int r = 1; r < rows; ++r) m = fmaxf(m, read_lane(key, 16 * r));
+    return m;
+}
+
+// The lane holding the (value, index) maximum of the lanes in `lanes` (the
+// whole wave, or its first row), each lane's value given with its key, the
+// value rounded down to FP32. Rounding is monotone, so the maximum's key is
+// the largest key: the keys are reduced in FP32 (one DPP move and one max per
+// step), and only the lanes whose key equals the largest -- usually one -- are
+// compared exactly. The same lane for every lane of the wave.
+__device__ inline int argmax_lane(double v, int32_t i, float key, uint64_t lanes) {
+    const float top = keys_max(key, lanes == ~0ull ? 4 : 1);
+    uint64_t cand = __ballot(key == top) & lanes;
+    int lane = __ffsll(static_cast<unsigned long long>(cand)) - 1;
+    cand &= cand - 1;
+    if (cand) {
+        double bv = read_lane(v, lane);
+        int32_t bi = __builtin_amdgcn_readlane(i, lane);
+        while (cand) {
+            const int l = __ffsll(static_cast<unsigned long long>(cand)) - 1;
+            cand &= cand - 1;
+            const double ov = read_lane(v, l);
+            const int32_t oi = __builtin_amdgcn_readlane(i, l);
+            if (beats(ov, oi, bv, bi)) {
+                bv = ov;
+                bi = oi;
+                lane = l;
+            }
+        }
+    }
+    return lane;
 }
 
 // Builds the blocked layout: records (x, y, z, running distance to the first
@@ -284,9 +330,10 @@ __device__ inline void sample_sweep(double4* __restrict__ rec, const int32_t* __
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int32_t own = vi[b];
-        wave_argmax(v[b], vi[b]);
-        vp[b] = lane_d3(vp[b], owner_lane(own, vi[b]));
+        const int at = argmax_lane(v[b], vi[b], __double2float_rd(v[b]), ~0ull);
+        v[b] = read_lane(v[b], at);
+        vi[b] = __builtin_amdgcn_readlane(vi[b], at);
+        vp[b] = lane_d3(vp[b], at);
     }
 }
 
@@ -302,6 +349,7 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
     __builtin_amdgcn_s_setprio(3);
     __shared__ double slot_v[2][kSampleWaves];
     __shared__ int32_t slot_i[2][kSampleWaves];
+    __shared__ float slot_k[2][kSampleWaves];
     __shared__ double slot_p[2][kSampleWaves][3];
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
@@ -320,26 +368,26 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
         bp = d3{q.y, q.z, q.w};
         bi = best_index[block];
     }
+    float bk = __double2float_rd(bv);  // key of the block maximum
     for (int32_t s = begin; s < end; ++s) {
         const int buf = s & 1;
-        double v = bv;
-        int32_t vi = bi;
-        wave_argmax(v, vi);
-        const d3 vp = lane_d3(bp, owner_lane(bi, vi));
+        const int at = argmax_lane(bv, bi, bk, ~0ull);
+        const double v = read_lane(bv, at);
+        const int32_t vi = __builtin_amdgcn_readlane(bi, at);
+        const d3 vp = lane_d3(bp, at);
         if (lane == 0) {
             slot_v[buf][wave] = v;
             slot_i[buf][wave] = vi;
+            slot_k[buf][wave] = __double2float_rd(v);
             slot_p[buf][wave][0] = vp.x;
             slot_p[buf][wave][1] = vp.y;
             slot_p[buf][wave][2] = vp.z;
         }
         __syncthreads();
-        // Each row of sixteen lanes holds the sixteen wave winners.
-        double sv = slot_v[buf][lane % kSampleWaves];
-        int32_t si = slot_i[buf][lane % kSampleWaves];
-        const int32_t mine = si;
-        row_argmax(sv, si);
-        const int from = owner_lane(mine, si) % kSampleWaves;
+        // The first row of lanes holds the sixteen wave winners.
+        const int w16 = lane % kSampleWaves;
+        const int from = argmax_lane(slot_v[buf][w16], slot_i[buf][w16], slot_k[buf][w16], 0xffffull);
+        const int32_t si = slot_i[buf][from];
         const d3 c{slot_p[buf][from][0], slot_p[buf][from][1], slot_p[buf][from][2]};
         if (threadIdx.x == 0) out[s] = si;
         // Blocks the seed may reach: squared distance from c to the box below
@@ -367,11 +415,13 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
                     bv = nv[0];
                     bi = ni[0];
                     bp = np[0];
+                    bk = __double2float_rd(bv);
                 }
                 if (lane == l1) {
                     bv = nv[1];
                     bi = ni[1];
                     bp = np[1];
+                    bk = __double2float_rd(bv);
                 }
             } else {
                 const int64_t base[1] = {static_cast<int64_t>(l0 * kSampleWaves + wave) * points_per_block};
@@ -383,6 +433,7 @@ sample_kernel(double4* __restrict__ rec, const int32_t* __restrict__ ids, const 
                     bv = nv[0];
                     bi = ni[0];
                     bp = np[0];
+                    bk = __double2float_rd(bv);
                 }
             }
         }
