@@ -21,10 +21,9 @@ blocks computed and accumulated in FP32, FP32 PCG iterations, FP64
 refinement of that FP32 system; right-hand side, energies, decisions and
 rotation fits in FP64), "mixed" (FP64 assembly, FP32 PCG iterations refined
 against the FP64 system) and "double" (plain FP64). Every system is solved to
-a 1e-8 relative true residual (the schedule's PCG tolerance). The schedule's
-cap of 50 PCG iterations is not imposed: the line reports the most iterations
-any solve took, so whether the cap would bind is visible. The oracle solves
-each system directly.
+a 1e-8 relative true residual (the schedule's PCG tolerance) with the
+schedule's cap of 50 PCG iterations; the line reports the most iterations any
+solve took. The oracle solves each system directly.
 
 CPU baseline: the FP64 oracle (a single-threaded restatement of the reference's
 route A) on this host, run before the GPU is touched. For configs 0 and 1 it
@@ -42,6 +41,14 @@ its result.
 The timed wraps run with profiling off. Two more wraps per form then run with
 HIP-event stage timing and traversal counters on; the stage breakdown, the
 roofline figures and the query statistics come from those.
+
+Beside the headline (unless --no-extras), in the headline's form: the 64-job
+batch of uneven humanoid jobs (nricp_amd.batch: jobs/s at a fixed slot count,
+the same jobs one at a time, and whether a sample of jobs gives the same bits
+alone and in the batch), and the serving loop (nricp_amd.serve_bench: p50 /
+p95 / p99 latency of Poisson arrivals at two offered loads below saturation).
+Their jobs are generated before the GPU is touched; they run while the
+one-core oracle wrap is still going.
 
 Prints one JSON line (rank 0).
 """
@@ -285,6 +292,70 @@ def cpu_baseline(config: int, job, settings, kwargs, procs: int) -> dict:
 CPU_ARRAYS = ("vertices", "node_positions", "level_nodes", "correspondence")
 
 
+BATCH_JOBS, BATCH_SEED = 64, 1000        # nricp_amd.batch's defaults
+SERVING_POOL, SERVING_SEED = 16, 3000     # nricp_amd.serve_bench's defaults
+
+
+def generate_extras(big) -> dict:
+    """The batch's 64 jobs and the serving pool, generated in worker
+    processes before this process touches the GPU."""
+    from nricp_amd import batch as batch_mod
+    from nricp_amd import serve_bench
+
+    t0 = time.perf_counter()
+    jobs = batch_mod.make_batch(BATCH_JOBS, BATCH_SEED)
+    pool = serve_bench.build_pool(SERVING_POOL, SERVING_SEED, min(len(os.sched_getaffinity(0)), 16))
+    log(f"batch ({len(jobs)} jobs) and serving pool ({len(pool)} jobs) generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return {"batch": jobs, "pool": pool, "big": big}
+
+
+def run_extras(dp, form: str, gen: dict, slots: int, rates: str) -> dict:
+    """The 64-job batch and the serving loop, in the headline's form."""
+    from nricp_amd import batch as batch_mod
+    from nricp_amd import serve_bench
+
+    jobs = gen["batch"]
+    batch_mod.run_batch(dp, jobs[:8], 4, form)  # warm-up: code objects, allocator cache
+    wall, together = batch_mod.run_batch(dp, jobs, slots, form)
+    serial_wall, alone = batch_mod.run_batch(dp, jobs, 1, form)
+    check = min(4, len(jobs))
+    same = all(np.array_equal(together[k]["vertices"], alone[k]["vertices"]) for k in range(check))
+    sizes = [len(j.template_points) for j in jobs]
+    latency = np.array([r["queued_seconds"] + r["run_seconds"] for r in together]) * 1e3
+    log(f"batch: {len(jobs) / wall:.1f} jobs/s at {slots} slots, {len(jobs) / serial_wall:.1f} jobs/s one at a time")
+    batch = {
+        "metric": "batched uneven instances: jobs/s on one MI355X (whole batch)",
+        "workload": f"{len(jobs)} humanoid jobs, {min(sizes)}-{max(sizes)} template vertices, target 2-3x the "
+                    f"template's triangles, nodes ~ V/25, 4 levels x {OUTER} GN steps, seed {BATCH_SEED}",
+        "form": form,
+        "slots": slots,
+        "jobs_per_s": round(len(jobs) / wall, 3),
+        "serial_jobs_per_s": round(len(jobs) / serial_wall, 3),
+        "speedup_over_serial": round(serial_wall / wall, 3),
+        "latency_p50_ms": round(float(np.percentile(latency, 50)), 2),
+        "latency_p99_ms": round(float(np.percentile(latency, 99)), 2),
+        "same_bits_alone_and_in_batch": {"jobs_checked": check, "same": bool(same)},
+    }
+    pool, big = gen["pool"], gen["big"]
+    loads = []
+    with dp.Server(max_active=4) as server:
+        for j in pool + [big]:  # warm-up: every job once
+            a, kw = serve_bench.job_args(j)
+            server.result(server.submit(*a, precision=form, **kw))
+        for k, rate in enumerate(float(r) for r in rates.split(",") if r.strip()):
+            rng = np.random.default_rng([SERVING_SEED, k])
+            loads.append(serve_bench.run_load(server, pool, big, rate, 40, form, rng))
+            log(f"serving at {rate:g} jobs/s: p99 {loads[-1]['p99_ms']} ms")
+    serving = {
+        "metric": "serving: p50/p95/p99 latency (submission to result) vs offered load, 1xMI355X",
+        "workload": f"Poisson arrivals, 40 per load; {len(pool)} uneven humanoid jobs (2k-50k template vertices) + "
+                    f"5 % the 200k-vertex job; 4 jobs on the device at once",
+        "loads": loads,
+    }
+    return {"batch": batch, "serving": serving}
+
+
 def copy_calibration(mib: int) -> None:
     """Three device-to-device copies of `mib` MiB through the HIP runtime the
     library uses (its blit copy kernel): a plain copy of known size, against
@@ -333,6 +404,9 @@ def main(argv=None) -> int:
     ap.add_argument("--pcg-max-iterations", type=int, default=MAX_PCG,
                     help="PCG iteration cap per solve (default 50, the benchmark schedule's)")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--no-extras", action="store_true", help="skip the 64-job batch and the serving sample")
+    ap.add_argument("--batch-slots", type=int, default=12, help="jobs on the device at once for the batch sample")
+    ap.add_argument("--serving-rates", default="20,50", help="offered loads (jobs/s) of the serving sample")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device to device three times: a plain copy of known size, against "
                          "which rocprofv3 FETCH_SIZE / WRITE_SIZE counts are scaled")
@@ -345,6 +419,11 @@ def main(argv=None) -> int:
               "GPUs with no collectives; this build runs and measures the one-GPU share)", file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
+    extras = not args.no_extras and rank == 0
+    if extras:
+        # Concurrent jobs' streams each need a hardware queue of their own
+        # (HIP reads this once, when it initialises; the default is 4).
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
     t0 = time.perf_counter()
     job, settings, kwargs = build_job(args.config, args.seed)
@@ -368,6 +447,8 @@ def main(argv=None) -> int:
             full_pool = mp.get_context("spawn").Pool(1)
             full_wrap = full_pool.apply_async(_oracle_full_wrap, (job, settings, kwargs))
 
+    extra_jobs = generate_extras(job) if extras else None
+
     import nricp_amd
 
     dp = nricp_amd.load()
@@ -381,6 +462,7 @@ def main(argv=None) -> int:
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     results = {f: run_form(dp, job, settings, kwargs, f, args.steps, args.warmup) for f in forms}
+    extra_out = run_extras(dp, forms[0], extra_jobs, args.batch_slots, args.serving_rates) if extras else None
     if full_wrap is not None:
         log("waiting for the complete oracle wrap (one core)")
         done = full_wrap.get()
@@ -472,6 +554,8 @@ def main(argv=None) -> int:
                                                  for k, v in cpu.items() if k not in CPU_ARRAYS},
         "forms": forms_out,
     }
+    if extra_out is not None:
+        line.update(extra_out)
     if "double" in results:
         for name in results:
             if name != "double":
