@@ -292,6 +292,7 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     report_.route = s.route == NRICP_HIP_ROUTE_A ? "route_a" : (s.route == NRICP_HIP_ROUTE_B ? "route_b" : "route_c");
     report_.precision = s.precision == NRICP_HIP_PRECISION_DOUBLE ? "double" : "float";
     check_level_sizes(pb, s, graph);
+    NRICP_CHECK(hipGetDevice(&device_));
     hipStream_t st = stream_.get();
     StreamScope scope(st);  // the session's buffers are ordered on its stream
     nv_ = pb.template_vertex_count;
@@ -340,6 +341,10 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
         for (auto& e : fps_ready_) NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         NRICP_CHECK(hipEventCreateWithFlags(&order_ready_, hipEventDisableTiming));
     }
+    layouts_.resize(graph_->levels.size());
+    prep_ready_.resize(graph_->levels.size(), nullptr);
+    for (size_t l = 1; l < prep_ready_.size(); ++l)
+        NRICP_CHECK(hipEventCreateWithFlags(&prep_ready_[l], hipEventDisableTiming));
     if (s.max_nodes_per_vertex < 1) throw InvalidArgument("max_nodes_per_vertex must be at least 1");
     if (s.graph_neighbours < 0 || s.graph_neighbours > 16) throw InvalidArgument("graph_neighbours must be in [0, 16]");
     // Buffers at the finest level's size.
@@ -363,17 +368,31 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
 }
 
 Session::~Session() {
+    join_prep();
+    (void)hipStreamSynchronize(prep_stream_.get());
     (void)hipStreamSynchronize(fps_stream_.get());
     (void)hipStreamSynchronize(stream_.get());
+    for (hipEvent_t e : prep_ready_)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : fps_ready_) (void)hipEventDestroy(e);
     if (order_ready_) (void)hipEventDestroy(order_ready_);
 }
 
 KernelClock& Session::clock() {
-    // Everything timed so far, including work still on the sampling stream.
+    // Everything timed so far, including work still on the sampling stream
+    // and the levels built ahead (their own clock, folded in here).
+    join_prep();
+    NRICP_CHECK(hipStreamSynchronize(prep_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(fps_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     clock_.collect(true);
+    prep_clock_.collect(true);
+    for (auto& kv : prep_clock_.totals) {
+        auto& t = clock_.totals[kv.first];
+        t.first += kv.second.first;
+        t.second += kv.second.second;
+    }
+    prep_clock_.totals.clear();
     return clock_;
 }
 
@@ -485,10 +504,61 @@ void Session::prepare() {
     clock_.end(st);
     have_warm_ = true;
     level_ = 0;
+    // The later levels' graphs and layouts, ahead of the solve.
+    join_prep();
+    prep_recorded_ = 0;
+    prep_failed_ = false;
+    prep_error_ = nullptr;
+    if (graph_->levels.size() > 1) prep_thread_ = std::thread([this] { build_levels_ahead(); });
+}
+
+void Session::build_levels_ahead() {
+    try {
+        NRICP_CHECK(hipSetDevice(device_));
+        hipStream_t ps = prep_stream_.get();
+        StreamScope scope(ps);  // the levels' buffers are ordered on the building stream
+        prep_clock_.enable(clock_.enabled());
+        const int lv = static_cast<int>(graph_->levels.size());
+        for (int l = 1; l < lv; ++l) {
+            if (!shared_graph_) {
+                NRICP_CHECK(hipStreamWaitEvent(ps, fps_ready_[l], 0));
+                prep_clock_.begin("graph", ps);
+                build_graph_level(points_.get(), nv_, seeds_.get(),
+                                  level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, l),
+                                  settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_.levels[l],
+                                  prep_sort_, ps);
+                prep_clock_.end(ps);
+            }
+            const GraphLevel& L = graph_->levels[l];
+            prep_clock_.begin("layout", ps);
+            build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
+                               L.edge_count, L.node_count, layouts_[l], prep_sort_, ps);
+            prep_clock_.end(ps);
+            NRICP_CHECK(hipEventRecord(prep_ready_[l], ps));
+            {
+                std::lock_guard<std::mutex> lock(prep_mutex_);
+                prep_recorded_ = static_cast<size_t>(l);
+            }
+            prep_cv_.notify_all();
+        }
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lock(prep_mutex_);
+            prep_error_ = std::current_exception();
+            prep_failed_ = true;
+        }
+        prep_cv_.notify_all();
+    }
+}
+
+void Session::join_prep() {
+    if (prep_thread_.joinable()) prep_thread_.join();
 }
 
 void Session::reset() {
     // The rebuild rewrites buffers that queued work of the last run may still read.
+    join_prep();
+    NRICP_CHECK(hipStreamSynchronize(prep_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(fps_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     phase_ = Phase::Prepare;
@@ -512,7 +582,15 @@ void Session::reset() {
 
 void Session::begin_level() {
     hipStream_t st = stream_.get();
-    if (!shared_graph_) {
+    if (level_ > 0) {
+        // Built ahead (build_levels_ahead): wait until the level's event is
+        // recorded, then order the main stream after it.
+        std::unique_lock<std::mutex> lock(prep_mutex_);
+        prep_cv_.wait(lock, [this] { return prep_failed_ || prep_recorded_ >= level_; });
+        if (prep_failed_ && prep_recorded_ < level_) std::rethrow_exception(prep_error_);
+        lock.unlock();
+        NRICP_CHECK(hipStreamWaitEvent(st, prep_ready_[level_], 0));
+    } else if (!shared_graph_) {
         NRICP_CHECK(hipStreamWaitEvent(st, fps_ready_[level_], 0));
         clock_.begin("graph", st);
         const int lv = static_cast<int>(own_graph_.levels.size());
@@ -533,14 +611,16 @@ void Session::begin_level() {
         std::swap(rot_, rot_cand_);
         std::swap(trans_, trans_cand_);
     }
-    clock_.begin("layout", st);
-    build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
-                       L.edge_count, L.node_count, layout_, sort_, st);
-    clock_.end(st);
+    if (level_ == 0) {
+        clock_.begin("layout", st);
+        build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
+                           L.edge_count, L.node_count, layout(), sort_, st);
+        clock_.end(st);
+    }
     // FP64 blocks for the double and mixed forms, FP32 blocks for the mixed
     // form's iterations and the all-FP32 form.
-    if (settings_.precision != NRICP_HIP_PRECISION_FLOAT) bsr_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
-    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE) bsr32_.resize(36 * std::max<int64_t>(layout_.nnzb, 1));
+    if (settings_.precision != NRICP_HIP_PRECISION_FLOAT) bsr_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
+    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE) bsr32_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
     view_ = LevelView{};
     view_.level = static_cast<int>(level_);
     view_.nodes = static_cast<int>(L.node_count);
@@ -594,12 +674,12 @@ void Session::solve(double damping) {
     pcg_.kernel_start = pcg_.kernel_stop = nullptr;
     clock_.external("pcg", &pcg_.kernel_start, &pcg_.kernel_stop);
     if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE)
-        pcg_solve_mixed(layout_, bsr32_.get(),
+        pcg_solve_mixed(layout(), bsr32_.get(),
                         settings_.precision == NRICP_HIP_PRECISION_MIXED ? bsr_.get() : nullptr, rhs_.get(), nn,
                         damping, settings_.diagonal_damping != 0,
                         settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
     else
-        pcg_solve(layout_, bsr_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
+        pcg_solve(layout(), bsr_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
                   settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
     pcg_.kernel_start = pcg_.kernel_stop = nullptr;
 }
@@ -614,14 +694,14 @@ bool Session::inner_step() {
         const AssemblyInputs in = inputs(rot_.get(), trans_.get());
         clock_.begin("assemble", st);
         if (settings_.precision == NRICP_HIP_PRECISION_FLOAT) {
-            assemble<float>(in, layout_, assembly_, bsr32_.get(), rhs_.get(), energy_cur_.get(), st);
-            if (local_global) restrict_to_translations<float>(layout_, bsr32_.get(), st);
+            assemble<float>(in, layout(), assembly_, bsr32_.get(), rhs_.get(), energy_cur_.get(), st);
+            if (local_global) restrict_to_translations<float>(layout(), bsr32_.get(), st);
         } else {
             const bool mixed = settings_.precision == NRICP_HIP_PRECISION_MIXED;
-            assemble<double>(in, layout_, assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st,
+            assemble<double>(in, layout(), assembly_, bsr_.get(), rhs_.get(), energy_cur_.get(), st,
                              mixed ? bsr32_.get() : nullptr);
-            if (local_global) restrict_to_translations<double>(layout_, bsr_.get(), st);
-            if (local_global && mixed) restrict_to_translations<float>(layout_, bsr32_.get(), st);
+            if (local_global) restrict_to_translations<double>(layout(), bsr_.get(), st);
+            if (local_global && mixed) restrict_to_translations<float>(layout(), bsr32_.get(), st);
         }
         clock_.end(st);
         // The local/global route keeps this matrix for the whole
@@ -633,7 +713,7 @@ bool Session::inner_step() {
         // (the mixed form writes each block twice, FP64 and FP32).
         const double sb = settings_.precision == NRICP_HIP_PRECISION_FLOAT ? 4.0
                           : settings_.precision == NRICP_HIP_PRECISION_MIXED ? 12.0 : 8.0;
-        const double N = static_cast<double>(nn), P = static_cast<double>(layout_.group_count);
+        const double N = static_cast<double>(nn), P = static_cast<double>(layout().group_count);
         counters_[kAssembleLaunches] += 1.0;
         counters_[kAssembleBytes] += 134.0 * static_cast<double>(nv_) + N * (120.0 + 36.0 * sb + 48.0) + P * 36.0 * sb;
     }
@@ -652,8 +732,8 @@ bool Session::inner_step() {
                                        hipMemcpyDeviceToDevice, st));
             clock_.begin("local_step", st);
             const AssemblyInputs in = inputs(rot_.get(), trans_.get());
-            fit_rotations(in, layout_, kRotationRelaxation, rot_.get(), st);
-            translation_rhs(inputs(rot_.get(), trans_.get()), layout_, assembly_, rhs_.get(), energy_fit_.get(), st);
+            fit_rotations(in, layout(), kRotationRelaxation, rot_.get(), st);
+            translation_rhs(inputs(rot_.get(), trans_.get()), layout(), assembly_, rhs_.get(), energy_fit_.get(), st);
             clock_.end(st);
         }
         solve(damping_);
@@ -679,7 +759,7 @@ bool Session::inner_step() {
         // matrix at least once more. Candidate energy pass: its inputs once.
         {
             const double s = settings_.precision == NRICP_HIP_PRECISION_DOUBLE ? 8.0 : 4.0;
-            const double P = static_cast<double>(layout_.group_count), N = static_cast<double>(nn);
+            const double P = static_cast<double>(layout().group_count), N = static_cast<double>(nn);
             counters_[kPcgLaunches] += 1.0;
             counters_[kPcgIterations] += h[10];
             counters_[kPcgMaxIterations] = std::max(counters_[kPcgMaxIterations], h[10]);

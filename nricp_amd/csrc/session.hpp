@@ -5,8 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <condition_variable>
+#include <exception>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -139,6 +143,9 @@ private:
     const nricp_hip_level_settings& level_settings() const;
     AssemblyInputs inputs(const double* R, const double* T) const;
     void prepare();
+    void build_levels_ahead();  // body of prep_thread_
+    void join_prep();
+    LevelLayout& layout() { return layouts_[level_]; }
     void begin_level();
     void refresh();
     bool inner_step();
@@ -171,7 +178,23 @@ private:
     const DeviceGraph* graph_ = nullptr;
     const DeviceGraph* shared_graph_ = nullptr;
     std::array<double, kCounterCount> counters_{};
-    LevelLayout layout_;
+    std::vector<LevelLayout> layouts_;  // per level
+    // The graph levels after the first and their layouts depend only on the
+    // seeds, so a host thread of their own builds them on their own stream as
+    // soon as each level's seeds are written, while the solve works on the
+    // coarser levels; a level's start waits for its event. (The first level
+    // is built on the main stream: its seeds are the first wait anyway.)
+    int device_ = 0;
+    Stream prep_stream_;
+    SortScratch prep_sort_;
+    KernelClock prep_clock_;
+    std::vector<hipEvent_t> prep_ready_;  // per level (from 1): graph level and layout written
+    std::thread prep_thread_;
+    std::mutex prep_mutex_;
+    std::condition_variable prep_cv_;
+    size_t prep_recorded_ = 0;            // levels 1 .. prep_recorded_ have their events recorded
+    bool prep_failed_ = false;
+    std::exception_ptr prep_error_;
     AssemblyScratch assembly_;
     PcgWorkspace pcg_;
     // state
