@@ -521,6 +521,15 @@ void Session::build_levels_ahead() {
         const int lv = static_cast<int>(graph_->levels.size());
         for (int l = 1; l < lv; ++l) {
             if (!shared_graph_) {
+                // The level's seeds first, polled: a host thread blocked in
+                // the runtime while the sampling runs for milliseconds
+                // delays the main thread's own calls into it.
+                for (;;) {
+                    const hipError_t q = hipEventQuery(fps_ready_[l]);
+                    if (q == hipSuccess) break;
+                    if (q != hipErrorNotReady) NRICP_CHECK(q);
+                    std::this_thread::sleep_for(std::chrono::microseconds(100));
+                }
                 NRICP_CHECK(hipStreamWaitEvent(ps, fps_ready_[l], 0));
                 prep_clock_.begin("graph", ps);
                 build_graph_level(points_.get(), nv_, seeds_.get(),
