@@ -16,7 +16,8 @@
 // residuals, per-slot arms, scales) and their energies; (2) one thread per
 // edge/node adds smoothness and anchor energies; (3) one wave per node gathers
 // its diagonal block and right-hand side from its vertex rows, edges and
-// anchor; (4) eight lanes per interacting pair gather the off-diagonal block.
+// anchor; (4) eight lanes per interacting pair gather the off-diagonal block
+// ((3) and (4) share one launch, system_blocks).
 // Energies are reduced in a fixed order, so the energy of a state is the same
 // bits whether it comes from an assembly or an energy-only pass.
 #include <hip/hip_runtime.h>
@@ -164,17 +165,28 @@ __device__ inline void graph_energy_block(const AssemblyInputs& in, double* __re
 }
 
 // The five energies from every block's partials, in a fixed order (thread t
-// adds blocks t, t + 256, ...; then the block sum).
+// adds blocks t, t + 256, ...; then the block sum). With `host` (pinned host
+// memory as the device addresses it), also writes `before` (nb values), the
+// five energies and `after` (na values) there, in that order: the readback
+// that follows an energy pass, without a launch of its own.
 __global__ void __launch_bounds__(kThreads) reduce_partials(const double* __restrict__ partials, int blocks,
-                                                            double* __restrict__ out) {
+                                                            double* __restrict__ out, double* __restrict__ host,
+                                                            const double* __restrict__ before, int nb,
+                                                            const double* __restrict__ after, int na) {
     __shared__ double lds[5 * (kThreads / kWave)];
     double e[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < blocks; b += kThreads)
         for (int k = 0; k < 5; ++k) e[k] += partials[5 * b + k];
     double s[5];
     block_sum<5, kThreads>(e, lds, s);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int k = 0; k < 5; ++k) out[k] = s[k];
+        if (host) {
+            for (int i = 0; i < nb; ++i) host[i] = before[i];
+            for (int k = 0; k < 5; ++k) host[nb + k] = s[k];
+            for (int i = 0; i < na; ++i) host[nb + 5 + i] = after[i];
+        }
+    }
 }
 
 // The rows and graph parts of an energy pass in one launch: blocks
@@ -264,13 +276,14 @@ __device__ inline void add_plane_row(Acc* blk, double* rhs, double c, d3 a, d3 n
 // right-hand side always in FP64. `copy32`, when given, receives an FP32
 // copy of the (FP64) block.
 template <typename Scalar, bool Blocks, typename Acc>
-__global__ void __launch_bounds__(kThreads)
-node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
-            const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
-            const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
-            Scalar* __restrict__ bsr, double* __restrict__ rhs, float* __restrict__ copy32) {
+__device__ inline void node_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
+                                       const double* __restrict__ resid, const double* __restrict__ arm,
+                                       const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+                                       const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent,
+                                       const int32_t* __restrict__ diag_pos, Scalar* __restrict__ bsr,
+                                       double* __restrict__ rhs, float* __restrict__ copy32, int64_t block) {
     const int lane = threadIdx.x % kWave;
-    const int64_t node = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    const int64_t node = block * (kThreads / kWave) + threadIdx.x / kWave;
     if (node >= in.node_count) return;
     Acc blk[kUpper];
     double g[6];
@@ -354,6 +367,16 @@ node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
     }
 }
 
+template <typename Scalar, bool Blocks, typename Acc>
+__global__ void __launch_bounds__(kThreads)
+node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
+            const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+            const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
+            Scalar* __restrict__ bsr, double* __restrict__ rhs, float* __restrict__ copy32) {
+    node_block_rows<Scalar, Blocks, Acc>(in, scale, resid, arm, goff, gent, eoff, eent, diag_pos, bsr, rhs, copy32,
+                                         blockIdx.x);
+}
+
 // ------------------------------------------------------------ pass 4 -------
 
 template <typename Acc>
@@ -394,12 +417,13 @@ __device__ inline void acc3(Acc (&out)[3], d3 v) {
 constexpr int kPairLanes = 8;
 
 template <typename Scalar, typename Acc>
-__global__ void __launch_bounds__(kThreads)
-pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ arm, int64_t groups,
-            const int32_t* __restrict__ goff, const int32_t* __restrict__ gent, const int32_t* __restrict__ pos,
-            Scalar* __restrict__ bsr, int32_t* __restrict__ touched, float* __restrict__ copy32) {
+__device__ inline void pair_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
+                                       const double* __restrict__ arm, int64_t groups,
+                                       const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+                                       const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
+                                       int32_t* __restrict__ touched, float* __restrict__ copy32, int64_t block) {
     const int sub = threadIdx.x % kPairLanes;
-    const int64_t g = (static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kPairLanes;
+    const int64_t g = (block * kThreads + threadIdx.x) / kPairLanes;
     const bool valid = g < groups;
     Acc acc[36];
 #pragma unroll
@@ -494,7 +518,28 @@ pair_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* _
     if (touched) touched[g] = hit != 0 ? 1 : 0;
 }
 
-void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream) {
+// The node blocks (workgroups [0, node_groups)) and the pair blocks (the
+// rest) of one assembly in one launch: both only read the vertex records, so
+// their gathers overlap instead of running one launch after the other.
+template <typename Scalar>
+__global__ void __launch_bounds__(kThreads)
+system_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
+              const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+              const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
+              int64_t node_groups, int64_t pairs, const int32_t* __restrict__ poff, const int32_t* __restrict__ pent,
+              const int32_t* __restrict__ ppos, Scalar* __restrict__ bsr, double* __restrict__ rhs,
+              int32_t* __restrict__ touched, float* __restrict__ copy32) {
+    const int64_t b = blockIdx.x;
+    if (b < node_groups)
+        node_block_rows<Scalar, true, Scalar>(in, scale, resid, arm, goff, gent, eoff, eent, diag_pos, bsr, rhs,
+                                              copy32, b);
+    else
+        pair_block_rows<Scalar, Scalar>(in, scale, arm, pairs, poff, pent, ppos, bsr, touched, copy32,
+                                        b - node_groups);
+}
+
+void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream,
+                     const HostReadback* readback = nullptr) {
     const int vblocks = std::max(1, blocks_for(in.vertex_count, kThreads));
     const int gblocks = std::max(1, blocks_for(std::max(in.edge_count, in.node_count), kThreads));
     s.partials.resize(5 * static_cast<int64_t>(vblocks + gblocks));
@@ -509,14 +554,18 @@ void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records,
         energy_rows<false><<<blocks, kThreads, 0, stream>>>(in, nullptr, nullptr, nullptr, s.partials.get(), vblocks);
     }
     NRICP_LAUNCH_CHECK("energy_rows");
-    reduce_partials<<<1, kThreads, 0, stream>>>(s.partials.get(), blocks, out);
+    const HostReadback none{};
+    const HostReadback& r = readback ? *readback : none;
+    reduce_partials<<<1, kThreads, 0, stream>>>(s.partials.get(), blocks, out, r.host, r.before, r.before_count,
+                                                r.after, r.after_count);
     NRICP_LAUNCH_CHECK("reduce_partials");
 }
 
 }  // namespace
 
-void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream) {
-    launch_energies(in, s, false, out, stream);
+void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream,
+              const HostReadback* readback) {
+    launch_energies(in, s, false, out, stream, readback);
 }
 
 // The block arithmetic follows the matrix type: FP64 blocks (optionally with
@@ -529,18 +578,16 @@ void assemble(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s
     if (sizeof(Scalar) == sizeof(float) && copy32) throw InvalidArgument("assemble: an FP32 matrix takes no copy");
     launch_energies(in, s, true, energies_out, stream);
     const int per_block = kThreads / kWave;
-    node_blocks<Scalar, true, Scalar><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
+    const int64_t node_groups = blocks_for(in.node_count, per_block);
+    const int64_t pair_groups = L.group_count > 0 ? blocks_for(L.group_count * kPairLanes, kThreads) : 0;
+    s.touched.resize(std::max<int64_t>(L.group_count, 1));
+    if (node_groups + pair_groups <= 0) return;
+    system_blocks<Scalar><<<static_cast<unsigned>(node_groups + pair_groups), kThreads, 0, stream>>>(
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
-        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), bsr, rhs,
-        copy32);
-    NRICP_LAUNCH_CHECK("node_blocks");
-    if (L.group_count > 0) {
-        s.touched.resize(L.group_count);
-        pair_blocks<Scalar, Scalar><<<blocks_for(L.group_count * kPairLanes, kThreads), kThreads, 0, stream>>>(
-            in, s.vertex_scale.get(), s.vertex_arm.get(), L.group_count, L.group_offsets.get(), L.group_entries.get(),
-            L.pair_position.get(), bsr, s.touched.get(), copy32);
-        NRICP_LAUNCH_CHECK("pair_blocks");
-    }
+        L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(),
+        node_groups, L.group_count, L.group_offsets.get(), L.group_entries.get(), L.pair_position.get(), bsr, rhs,
+        s.touched.get(), copy32);
+    NRICP_LAUNCH_CHECK("system_blocks");
 }
 
 void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s, double* rhs,

@@ -187,7 +187,11 @@ __device__ inline void block_sum(const double (&v)[Count], double* lds /* Count 
 //     the groups of an admitted launch that are not yet resident become
 //     resident once the ordinary kernels ahead of them retire.
 int persistent_group_budget(const void* kernel, int threads);  // groups one persistent launch may use
-void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream);
+// `prelude`, when given, enqueues the work that precedes the launch on the
+// stream; it runs once the launch is admitted, so that work and the launch
+// reach the device back to back instead of around the admission.
+void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream,
+                       void (*prelude)(void*) = nullptr, void* prelude_arg = nullptr);
 
 // --------------------------------------------------------- grid barrier ----
 

@@ -266,7 +266,18 @@ struct AssemblyScratch {
     DeviceBuffer<int32_t> touched;      // P
 };
 // Energy only: out[0..5) = p2p, plane, smoothness, anchor, landmark.
-void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream);
+// Values copied to pinned host memory (as the device addresses it) by the
+// energy pass's last kernel: `before` (before_count values), the five
+// energies, then `after` (after_count values).
+struct HostReadback {
+    double* host = nullptr;
+    const double* before = nullptr;
+    int before_count = 0;
+    const double* after = nullptr;
+    int after_count = 0;
+};
+void energies(const AssemblyInputs& in, AssemblyScratch& s, double* out, hipStream_t stream,
+              const HostReadback* readback = nullptr);
 // The system: diagonal and pair blocks into `bsr` (nnzb x 36, at the layout's
 // positions), rhs = -J^T r (N x 6), plus the energies of the same state.
 // FP64 blocks are accumulated in FP64 (and `copy32`, when given, receives

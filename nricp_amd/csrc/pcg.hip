@@ -654,9 +654,30 @@ void launch_pcg_raw(const int32_t* row_off, const int32_t* cols, const int32_t* 
                     int max_iter, double* solution, const PcgShape& sh, S* damp, double* damp64, S* minv, S* vbuf,
                     double* xbuf, double* slots, unsigned* bar, double* out, hipEvent_t start, hipEvent_t stop,
                     hipStream_t stream) {
-    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, bsr64, diag_pos, nn, lambda,
-                                                                     scaling ? 1 : 0, damp, damp64, minv, bar);
-    NRICP_LAUNCH_CHECK("precondition_kernel");
+    // The preconditioner (and the barrier's reset), then the solve. In a
+    // grid launch it is enqueued once the launch is admitted, so the two
+    // reach the device back to back.
+    struct Prelude {
+        const S* bsr;
+        const double* bsr64;
+        const int32_t* diag_pos;
+        int64_t nn;
+        double lambda;
+        bool scaling;
+        S* damp;
+        double* damp64;
+        S* minv;
+        unsigned* bar;
+        hipEvent_t start;
+        hipStream_t stream;
+        static void run(void* p) {
+            const Prelude& a = *static_cast<const Prelude*>(p);
+            precondition_kernel<S><<<blocks_for(a.nn, 128), 128, 0, a.stream>>>(
+                a.bsr, a.bsr64, a.diag_pos, a.nn, a.lambda, a.scaling ? 1 : 0, a.damp, a.damp64, a.minv, a.bar);
+            NRICP_LAUNCH_CHECK("precondition_kernel");
+            if (a.start) NRICP_CHECK(hipEventRecord(a.start, a.stream));
+        }
+    } prelude{bsr, bsr64, diag_pos, nn, lambda, scaling, damp, damp64, minv, bar, start, stream};
     const void* kernel = nullptr;
     if (sh.grid) {
         kernel = sh.k == 1 ? reinterpret_cast<const void*>(pcg_kernel<1, S, true>)
@@ -670,10 +691,10 @@ void launch_pcg_raw(const int32_t* row_off, const int32_t* cols, const int32_t* 
     void* args[] = {const_cast<int32_t**>(&row_off), const_cast<int32_t**>(&cols), const_cast<S**>(&bsr), &damp, &minv,
                     const_cast<double**>(&bsr64), &d64, const_cast<double**>(&rhs), &n, &tol, &inner_tol, &max_iter,
                     &vbuf, &xbuf, &slots, &bar, &solution, &out};
-    if (start) NRICP_CHECK(hipEventRecord(start, stream));
     if (sh.grid) {
-        launch_persistent(kernel, sh.groups, kPcgThreads, args, stream);
+        launch_persistent(kernel, sh.groups, kPcgThreads, args, stream, &Prelude::run, &prelude);
     } else {
+        Prelude::run(&prelude);
         NRICP_CHECK(hipLaunchKernel(kernel, dim3(1), dim3(kPcgThreads), args, 0, stream));
     }
     if (stop) NRICP_CHECK(hipEventRecord(stop, stream));

@@ -453,7 +453,8 @@ int persistent_group_budget(const void* kernel, int threads) {
     return std::max(1, cus / 2);
 }
 
-void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream) {
+void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream,
+                       void (*prelude)(void*), void* prelude_arg) {
     if (groups < 1 || groups > persistent_group_budget(kernel, threads))
         throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
     int device = 0;
@@ -479,6 +480,14 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
         lock.lock();
     }
     const hipEvent_t done = take_event(r, device);
+    if (prelude) {
+        try {
+            prelude(prelude_arg);
+        } catch (...) {
+            r.spare.push_back(Spare{device, done, stream});
+            throw;
+        }
+    }
     // Launch and record under the lock, so the ledger never holds a charge
     // without its event.
     hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);

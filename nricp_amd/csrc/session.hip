@@ -748,9 +748,12 @@ bool Session::inner_step() {
         solve(damping_);
         retract(rot_.get(), trans_.get(), delta_.get(), nn, rot_cand_.get(), trans_cand_.get(), st);
         clock_.begin("energy", st);
-        energies(inputs(rot_cand_.get(), trans_cand_.get()), assembly_, energy_cand_.get(), st);
+        // The candidate's energies, and with them the current energies and
+        // the solve's outcome, to the host staging: [0, 5) current, [5, 10)
+        // candidate, [10, 14) outcome.
+        const HostReadback readback{staging_device_, energy_cur_.get(), 5, pcg_.outcome.get(), 4};
+        energies(inputs(rot_cand_.get(), trans_cand_.get()), assembly_, energy_cand_.get(), st, &readback);
         clock_.end(st);
-        to_host(st, 0, energy_cur_.get(), 5, energy_cand_.get(), 5, pcg_.outcome.get(), 4);
         sync();
         const double* h = staging_.get();
         cur = (((h[0] + h[1]) + h[2]) + h[3]) + h[4];
