@@ -440,7 +440,6 @@ int64_t stack_entries_needed(int32_t depth_max) {
 struct Nearest {
     double d2;
     int32_t triangle;
-    int32_t leaf;
     uint32_t opens, tests;  // traversal work, for profiling
 };
 
@@ -465,36 +464,34 @@ __device__ inline T octet_xor(T v) {
 }
 
 template <int K>
-__device__ inline void group_min_step(double& d2, int32_t& tri, int32_t& leaf) {
+__device__ inline void group_min_step(double& d2, int32_t& tri) {
     const double od = octet_xor<K>(d2);
     const int32_t ot = octet_xor<K>(tri);
-    const int32_t ol = octet_xor<K>(leaf);
     if (od < d2 || (od == d2 && ot < tri)) {
         d2 = od;
         tri = ot;
-        leaf = ol;
     }
 }
 
 // Lexicographic (d2, triangle) minimum over the group's eight lanes: pairs,
 // then quads, then the two quads (the order is total, so every lane keeps
 // the same winner).
-__device__ inline void group_min(double& d2, int32_t& tri, int32_t& leaf) {
-    group_min_step<1>(d2, tri, leaf);
-    group_min_step<2>(d2, tri, leaf);
-    group_min_step<7>(d2, tri, leaf);
+__device__ inline void group_min(double& d2, int32_t& tri) {
+    group_min_step<1>(d2, tri);
+    group_min_step<2>(d2, tri);
+    group_min_step<7>(d2, tri);
 }
 
 __device__ inline int group_sum(int v) { return octet_sum(v); }
 
-// How many pushing lanes of the group come before lane j in the (distance,
-// lane) order.
+// How many lanes of the group come before lane j in the (key, lane) order
+// (non-pushing lanes carry +inf keys, so a pushing lane counts only pushing
+// lanes).
 template <int K>
-__device__ inline void rank_step(double d, int push, int j, int& rank) {
-    const double od = octet_xor<K>(d);
-    const int op = octet_xor<K>(push);
+__device__ inline void rank_step(float key, int j, int& rank) {
+    const float ok = octet_xor<K>(key);
     const int other = j ^ K;
-    if (op && (od < d || (od == d && other < j))) ++rank;
+    if (ok < key || (ok == key && other < j)) ++rank;
 }
 
 template <int StackDepth>
@@ -524,7 +521,7 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
         const double d = ref != kWideEmpty ? box_d : __builtin_inf();
         // Leaf children within reach: tested in parallel, then merged.
         double cd2 = best.d2;
-        int32_t ctri = best.triangle, cleaf = best.leaf;
+        int32_t ctri = best.triangle;
         if (ref != kWideEmpty && ref < 0 && !(d > best.d2 * kPruneSlack)) {
             const int32_t leaf = ~ref;
             // The record (corners, then the triangle id) as five 16-byte
@@ -540,25 +537,27 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
             if (d2 < cd2 || (d2 == cd2 && tri < ctri)) {
                 cd2 = d2;
                 ctri = tri;
-                cleaf = leaf;
             }
             ++best.tests;
         }
-        group_min(cd2, ctri, cleaf);
+        group_min(cd2, ctri);
         best.d2 = cd2;
         best.triangle = ctri;
-        best.leaf = cleaf;
         // Internal children still within reach, pushed nearest on top.
         const bool push = ref != kWideEmpty && ref >= 0 && !(d > best.d2 * kPruneSlack);
+        // The push order (nearest on top) only steers the search, never its
+        // result, so it is ranked on the FP32 distance the stack keeps (its
+        // rounding down, finite for every box); lanes that do not push carry
+        // +inf and rank after every pushing lane.
+        const float key = push ? __double2float_rd(d) : __builtin_inff();
         int rank = 0;
-        const int pushing = push ? 1 : 0;
-        rank_step<1>(d, pushing, j, rank);
-        rank_step<2>(d, pushing, j, rank);
-        rank_step<3>(d, pushing, j, rank);
-        rank_step<4>(d, pushing, j, rank);
-        rank_step<5>(d, pushing, j, rank);
-        rank_step<6>(d, pushing, j, rank);
-        rank_step<7>(d, pushing, j, rank);
+        rank_step<1>(key, j, rank);
+        rank_step<2>(key, j, rank);
+        rank_step<3>(key, j, rank);
+        rank_step<4>(key, j, rank);
+        rank_step<5>(key, j, rank);
+        rank_step<6>(key, j, rank);
+        rank_step<7>(key, j, rank);
         const int npush = group_sum(push ? 1 : 0);
         if (sp + npush > StackDepth) {
             best.triangle = -1;
@@ -567,7 +566,7 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
         if (push) {
             const int at = sp + npush - 1 - rank;
             stack_node[at] = ref;
-            stack_dist[at] = __double2float_rd(d);
+            stack_dist[at] = key;
         }
         sp += npush;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -625,7 +624,6 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
     const d3 q = ld3(queries, i);
     Nearest best;
     best.d2 = __builtin_inf();
-    best.leaf = -1;
     best.triangle = 0x7fffffff;
     best.opens = best.tests = 0;
     // Warm start: the triangle this query matched last time is an exact
