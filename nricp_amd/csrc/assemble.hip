@@ -24,6 +24,7 @@
 
 #include "kernels.hpp"
 #include "so3.hpp"
+#include "xcd_map.hpp"
 
 namespace nricp_hip {
 namespace {
@@ -281,10 +282,12 @@ __device__ inline void node_block_rows(const AssemblyInputs& in, const double* _
                                        const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
                                        const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent,
                                        const int32_t* __restrict__ diag_pos, Scalar* __restrict__ bsr,
-                                       double* __restrict__ rhs, float* __restrict__ copy32, int64_t block) {
+                                       double* __restrict__ rhs, float* __restrict__ copy32, int64_t block,
+                                       const int32_t* __restrict__ order) {
     const int lane = threadIdx.x % kWave;
-    const int64_t node = block * (kThreads / kWave) + threadIdx.x / kWave;
-    if (node >= in.node_count) return;
+    const int64_t slot = block * (kThreads / kWave) + threadIdx.x / kWave;
+    if (slot >= in.node_count) return;
+    const int64_t node = order ? order[slot] : slot;
     Acc blk[kUpper];
     double g[6];
     for (int k = 0; k < kUpper; ++k) blk[k] = Acc(0);
@@ -372,9 +375,10 @@ __global__ void __launch_bounds__(kThreads)
 node_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
             const double* __restrict__ arm, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
             const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
-            Scalar* __restrict__ bsr, double* __restrict__ rhs, float* __restrict__ copy32) {
+            Scalar* __restrict__ bsr, double* __restrict__ rhs, float* __restrict__ copy32,
+            const int32_t* __restrict__ order) {
     node_block_rows<Scalar, Blocks, Acc>(in, scale, resid, arm, goff, gent, eoff, eent, diag_pos, bsr, rhs, copy32,
-                                         blockIdx.x);
+                                         xcd_contiguous(blockIdx.x, gridDim.x), order);
 }
 
 // ------------------------------------------------------------ pass 4 -------
@@ -421,10 +425,12 @@ __device__ inline void pair_block_rows(const AssemblyInputs& in, const double* _
                                        const double* __restrict__ arm, int64_t groups,
                                        const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
                                        const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
-                                       int32_t* __restrict__ touched, float* __restrict__ copy32, int64_t block) {
+                                       int32_t* __restrict__ touched, float* __restrict__ copy32, int64_t block,
+                                       const int32_t* __restrict__ order) {
     const int sub = threadIdx.x % kPairLanes;
-    const int64_t g = (block * kThreads + threadIdx.x) / kPairLanes;
-    const bool valid = g < groups;
+    const int64_t slot = (block * kThreads + threadIdx.x) / kPairLanes;
+    const bool valid = slot < groups;
+    const int64_t g = valid && order ? order[slot] : slot;
     Acc acc[36];
 #pragma unroll
     for (int k = 0; k < 36; ++k) acc[k] = Acc(0);
@@ -518,9 +524,12 @@ __device__ inline void pair_block_rows(const AssemblyInputs& in, const double* _
     if (touched) touched[g] = hit != 0 ? 1 : 0;
 }
 
-// The node blocks (workgroups [0, node_groups)) and the pair blocks (the
-// rest) of one assembly in one launch: both only read the vertex records, so
-// their gathers overlap instead of running one launch after the other.
+// The node blocks (node_groups workgroups) and the pair blocks (the rest) of
+// one assembly in one launch: both only read the vertex records, so their
+// gathers overlap instead of running one launch after the other. Nodes and
+// groups are taken in the layout's space order, and each XCD takes one
+// contiguous eighth of the nodes and of the groups, so the records one region
+// of the mesh needs are gathered through one L2.
 template <typename Scalar>
 __global__ void __launch_bounds__(kThreads)
 system_blocks(AssemblyInputs in, const double* __restrict__ scale, const double* __restrict__ resid,
@@ -528,14 +537,16 @@ system_blocks(AssemblyInputs in, const double* __restrict__ scale, const double*
               const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent, const int32_t* __restrict__ diag_pos,
               int64_t node_groups, int64_t pairs, const int32_t* __restrict__ poff, const int32_t* __restrict__ pent,
               const int32_t* __restrict__ ppos, Scalar* __restrict__ bsr, double* __restrict__ rhs,
-              int32_t* __restrict__ touched, float* __restrict__ copy32) {
-    const int64_t b = blockIdx.x;
-    if (b < node_groups)
+              int32_t* __restrict__ touched, float* __restrict__ copy32, const int32_t* __restrict__ node_order,
+              const int32_t* __restrict__ group_order) {
+    int kind = 0;
+    const int64_t b = xcd_split(blockIdx.x, gridDim.x, node_groups, &kind);
+    if (kind == 0)
         node_block_rows<Scalar, true, Scalar>(in, scale, resid, arm, goff, gent, eoff, eent, diag_pos, bsr, rhs,
-                                              copy32, b);
+                                              copy32, b, node_order);
     else
-        pair_block_rows<Scalar, Scalar>(in, scale, arm, pairs, poff, pent, ppos, bsr, touched, copy32,
-                                        b - node_groups);
+        pair_block_rows<Scalar, Scalar>(in, scale, arm, pairs, poff, pent, ppos, bsr, touched, copy32, b,
+                                        group_order);
 }
 
 void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream,
@@ -586,7 +597,7 @@ void assemble(const AssemblyInputs& in, const LevelLayout& L, AssemblyScratch& s
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
         L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(),
         node_groups, L.group_count, L.group_offsets.get(), L.group_entries.get(), L.pair_position.get(), bsr, rhs,
-        s.touched.get(), copy32);
+        s.touched.get(), copy32, L.node_order.get(), L.group_order.get());
     NRICP_LAUNCH_CHECK("system_blocks");
 }
 
@@ -597,7 +608,7 @@ void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScr
     node_blocks<double, false, double><<<blocks_for(in.node_count, per_block), kThreads, 0, stream>>>(
         in, s.vertex_scale.get(), s.vertex_resid.get(), s.vertex_arm.get(), L.gather_offsets.get(),
         L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), nullptr,
-        rhs, nullptr);
+        rhs, nullptr, L.node_order.get());
     NRICP_LAUNCH_CHECK("node_blocks");
 }
 

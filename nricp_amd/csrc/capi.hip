@@ -465,7 +465,7 @@ int nricp_hip_fused_system(const nricp_hip_energy_inputs* inputs, int32_t precis
         SortScratch sort;
         LevelLayout layout;
         build_level_layout(u.in.bind_nodes, u.in.bind_weights, u.in.bind_counts, u.in.vertex_count, u.in.edges,
-                           u.in.edge_count, nn, layout, sort, st);
+                           u.in.edge_count, nn, u.in.rest, layout, sort, st);
         AssemblyScratch scratch;
         DeviceBuffer<double> rhs(6 * nn), energy(5);
         std::vector<double> blocks(36 * std::max<int64_t>(layout.nnzb, 1));
@@ -535,7 +535,7 @@ int nricp_hip_fit_rotations(const nricp_hip_energy_inputs* inputs, double relaxa
         SortScratch sort;
         LevelLayout layout;
         build_level_layout(u.in.bind_nodes, u.in.bind_weights, u.in.bind_counts, u.in.vertex_count, u.in.edges,
-                           u.in.edge_count, nn, layout, sort, st);
+                           u.in.edge_count, nn, u.in.rest, layout, sort, st);
         fit_rotations(u.in, layout, relaxation, u.rot.get(), st);
         u.rot.download(out_rotations, 9 * nn, st);
         NRICP_CHECK(hipStreamSynchronize(st));

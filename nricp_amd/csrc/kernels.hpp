@@ -225,14 +225,24 @@ struct LevelLayout {
     DeviceBuffer<int32_t> columns;         // nnzb
     DeviceBuffer<int32_t> diag_position;   // N
     DeviceBuffer<int32_t> pair_position;   // P x 2: block (a,b), block (b,a)
+    // The order in which the assembly visits nodes and pair groups: nodes by
+    // the Morton code of their positions, groups by their first node's place
+    // in that order (node indices follow the sampling, so consecutive nodes
+    // lie far apart; visited in space order, the vertex records a workgroup
+    // gathers are the ones its neighbours just read).
+    DeviceBuffer<int32_t> node_order;      // N
+    DeviceBuffer<int32_t> group_order;     // P
     // scratch
     DeviceBuffer<uint64_t> keys, block_keys;
     DeviceBuffer<uint32_t> vals, block_vals;
     DeviceBuffer<int32_t> counts, offsets, flags, runs;
+    DeviceBuffer<double> order_bounds;
 };
+// `node_positions` (N x 3) sets the assembly's visiting order; null visits
+// nodes and groups in index order.
 void build_level_layout(const int32_t* bind_nodes, const double* bind_weights, const int32_t* bind_counts,
                         int64_t vertex_count, const int32_t* edges, int64_t edge_count, int64_t node_count,
-                        LevelLayout& layout, SortScratch& scratch, hipStream_t stream);
+                        const double* node_positions, LevelLayout& layout, SortScratch& scratch, hipStream_t stream);
 
 // ---------------------------------------------------------- assembly -------
 

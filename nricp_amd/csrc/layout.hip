@@ -139,6 +139,26 @@ __global__ void block_positions(const uint64_t* __restrict__ key, const uint32_t
         pair_pos[2 * index + (kind - 1)] = static_cast<int32_t>(k);
 }
 
+__global__ void iota_kernel(int32_t* __restrict__ out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<int32_t>(i);
+}
+
+// rank[order[i]] = i
+__global__ void invert_kernel(const int32_t* __restrict__ order, int64_t n, int32_t* __restrict__ rank) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) rank[order[i]] = static_cast<int32_t>(i);
+}
+
+// Key of each pair group: the rank of its first node in the node order.
+__global__ void group_rank_keys(const int32_t* __restrict__ pairs, const int32_t* __restrict__ rank, int64_t np,
+                                uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= np) return;
+    key[g] = static_cast<uint64_t>(rank[pairs[2 * g]]);
+    val[g] = static_cast<uint32_t>(g);
+}
+
 int32_t read_i32(const int32_t* p, hipStream_t s) {
     int32_t v = 0;
     NRICP_CHECK(hipMemcpyAsync(&v, p, sizeof(v), hipMemcpyDeviceToHost, s));
@@ -149,7 +169,8 @@ int32_t read_i32(const int32_t* p, hipStream_t s) {
 }  // namespace
 
 void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, int64_t nv, const int32_t* edges,
-                        int64_t ne, int64_t nn, LevelLayout& L, SortScratch& scratch, hipStream_t stream) {
+                        int64_t ne, int64_t nn, const double* node_positions, LevelLayout& L, SortScratch& scratch,
+                        hipStream_t stream) {
     if (nn <= 0) throw InvalidArgument("a level needs at least one node");
     if (nv >= (1ll << 27)) throw InvalidArgument("too many template vertices for the layout encoding");
     L.node_count = nn;
@@ -264,6 +285,36 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     block_positions<<<blocks_for(nnzb, 256), 256, 0, stream>>>(bkey, bval, nnzb, column_mask, L.columns.get(),
                                                                 L.diag_position.get(), L.pair_position.get());
     NRICP_LAUNCH_CHECK("block_positions");
+    // 7. the assembly's visiting order: nodes by the Morton code of their
+    // positions, pair groups by the rank of their first node in it (stable,
+    // so by group index within a node). Only the order in which workgroups
+    // take their work changes; every block is summed the same way.
+    L.node_order.resize(nn);
+    L.group_order.resize(std::max<int64_t>(np, 1));
+    if (!node_positions) {
+        iota_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(L.node_order.get(), nn);
+        NRICP_LAUNCH_CHECK("iota_kernel");
+        if (np > 0) {
+            iota_kernel<<<blocks_for(np, 256), 256, 0, stream>>>(L.group_order.get(), np);
+            NRICP_LAUNCH_CHECK("iota_kernel");
+        }
+        return;
+    }
+    morton_order(node_positions, nn, L.node_order.get(), scratch, L.block_keys, L.order_bounds, stream);
+    if (np > 0) {
+        L.counts.resize(std::max<int64_t>(nn, 2 * (nv + 1)));
+        int32_t* rank = L.counts.get();
+        invert_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(L.node_order.get(), nn, rank);
+        NRICP_LAUNCH_CHECK("invert_kernel");
+        L.keys.resize(std::max<int64_t>(np, 1));
+        L.vals.resize(std::max<int64_t>(np, 1));
+        group_rank_keys<<<blocks_for(np, 256), 256, 0, stream>>>(L.group_pairs.get(), rank, np, L.keys.get(),
+                                                                  L.vals.get());
+        NRICP_LAUNCH_CHECK("group_rank_keys");
+        radix_sort_pairs(L.keys.get(), L.vals.get(), np, node_bits, scratch, stream);
+        NRICP_CHECK(hipMemcpyAsync(L.group_order.get(), L.vals.get(), np * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                   stream));
+    }
 }
 
 }  // namespace nricp_hip

@@ -541,7 +541,7 @@ void Session::build_levels_ahead() {
             const GraphLevel& L = graph_->levels[l];
             prep_clock_.begin("layout", ps);
             build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
-                               L.edge_count, L.node_count, layouts_[l], prep_sort_, ps);
+                               L.edge_count, L.node_count, L.node_positions.get(), layouts_[l], prep_sort_, ps);
             prep_clock_.end(ps);
             NRICP_CHECK(hipEventRecord(prep_ready_[l], ps));
             {
@@ -623,7 +623,7 @@ void Session::begin_level() {
     if (level_ == 0) {
         clock_.begin("layout", st);
         build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
-                           L.edge_count, L.node_count, layout(), sort_, st);
+                           L.edge_count, L.node_count, L.node_positions.get(), layout(), sort_, st);
         clock_.end(st);
     }
     // FP64 blocks for the double and mixed forms, FP32 blocks for the mixed

@@ -25,3 +25,17 @@ def test_stream_ordered_pool_rules(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == "ok"
+
+
+def test_xcd_work_mapping(tmp_path):
+    """The gather kernels' workgroup -> work mapping (nricp_amd/csrc/xcd_map.hpp):
+    a bijection that gives each XCD one contiguous run of each kind of work."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.fail("g++ is needed to build the host test")
+    exe = tmp_path / "xcd_map_test"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-Wall", f"-I{ROOT / 'nricp_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "native" / "xcd_map_test.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
