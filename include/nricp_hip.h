@@ -401,6 +401,39 @@ int nricp_hip_server_result(nricp_hip_server* server, int64_t ticket, double* ou
 int nricp_hip_server_release(nricp_hip_server* server, int64_t ticket);
 int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats* out_stats);
 
+/* --------------------------------------------------------------- batch ---- */
+
+/* A batch of wrap jobs solved in lockstep through shared launches (no
+ * reference counterpart: the reference runs one RouteBSession per job on one
+ * stream, src/cuda/gpu_solver.cuh:86-126, src/cuda/host_staging.cuh:74-99; the
+ * single-job kernels the packed ones generalise are src/cuda/assemble.cu:553-662
+ * and src/cuda/pcg_solver.cu:63-241). Each job keeps its own resident session
+ * (one nricp_hip_settings per job; all jobs share one precision, and route C
+ * is refused); per graph level, every Gauss-Newton damping attempt of every
+ * job still in the level runs as one packed launch per stage -- evaluation,
+ * normals, closest points, assembly, preconditioner, one persistent PCG
+ * launch over every job's block rows with per-job convergence, retraction,
+ * energies, and the accept/reject decision taken on the device -- captured as
+ * a graph and replayed. A job's result is the same bits in any batch.
+ * `pcg_slots` (1 or 2) is the packed PCG's node slots per lane group (its
+ * tile shape); `use_graphs` 0 launches the stages directly. */
+typedef struct nricp_hip_batch nricp_hip_batch;
+typedef struct nricp_hip_batch_config {
+    int32_t pcg_slots;
+    int32_t use_graphs;
+} nricp_hip_batch_config;
+/* `settings` holds `count` entries, one per problem. */
+int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_settings* settings, int32_t count,
+                           const nricp_hip_batch_config* config, nricp_hip_batch** out_batch);
+int nricp_hip_batch_run(nricp_hip_batch* batch);
+/* Job `index` as a session handle owned by the batch (results: vertices,
+ * report, state, level nodes, counters; not to be destroyed or stepped). */
+int nricp_hip_batch_job(nricp_hip_batch* batch, int32_t index, nricp_hip_session** out_session);
+/* [ticks, graph replays, graphs captured, refresh ticks, assembly ticks,
+ * seconds, level set-up seconds, tick-loop seconds]. */
+int nricp_hip_batch_stats(const nricp_hip_batch* batch, double* out, int32_t capacity, int32_t* out_count);
+void nricp_hip_batch_destroy(nricp_hip_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

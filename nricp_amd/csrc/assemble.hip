@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "packed.hpp"
 #include "so3.hpp"
 #include "xcd_map.hpp"
 
@@ -61,7 +62,7 @@ __device__ inline void left_jacobian_inverse(d3 w, double* J) {
 // Records, the scales, residuals and arms the assembly reuses), and the
 // block's energy partials. lds: 3 x waves doubles.
 template <bool Records>
-__device__ inline void vertex_rows_block(const AssemblyInputs& in, double* __restrict__ scale,
+__device__ __forceinline__ void vertex_rows_block(const AssemblyInputs& in, double* __restrict__ scale,
                                          double* __restrict__ resid, double* __restrict__ arm,
                                          double* __restrict__ partials, int64_t block, double* lds) {
     const int64_t v = block * kThreads + threadIdx.x;
@@ -133,7 +134,7 @@ __device__ inline void vertex_rows_block(const AssemblyInputs& in, double* __res
 
 // Block `block` of the graph terms (smoothness per edge, anchor per node) and
 // its energy partials. lds: 2 x waves doubles.
-__device__ inline void graph_energy_block(const AssemblyInputs& in, double* __restrict__ partials, int64_t block,
+__device__ __forceinline__ void graph_energy_block(const AssemblyInputs& in, double* __restrict__ partials, int64_t block,
                                           double* lds) {
     const int64_t i = block * kThreads + threadIdx.x;
     double e[2] = {0.0, 0.0};
@@ -170,16 +171,20 @@ __device__ inline void graph_energy_block(const AssemblyInputs& in, double* __re
 // memory as the device addresses it), also writes `before` (nb values), the
 // five energies and `after` (na values) there, in that order: the readback
 // that follows an energy pass, without a launch of its own.
+__device__ __forceinline__ void fold_partials(const double* __restrict__ partials, int blocks, double* lds, double (&s)[5]) {
+    double e[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < blocks; b += kThreads)
+        for (int k = 0; k < 5; ++k) e[k] += partials[5 * b + k];
+    block_sum<5, kThreads>(e, lds, s);
+}
+
 __global__ void __launch_bounds__(kThreads) reduce_partials(const double* __restrict__ partials, int blocks,
                                                             double* __restrict__ out, double* __restrict__ host,
                                                             const double* __restrict__ before, int nb,
                                                             const double* __restrict__ after, int na) {
     __shared__ double lds[5 * (kThreads / kWave)];
-    double e[5] = {0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < blocks; b += kThreads)
-        for (int k = 0; k < 5; ++k) e[k] += partials[5 * b + k];
     double s[5];
-    block_sum<5, kThreads>(e, lds, s);
+    fold_partials(partials, blocks, lds, s);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 5; ++k) out[k] = s[k];
         if (host) {
@@ -277,7 +282,7 @@ __device__ inline void add_plane_row(Acc* blk, double* rhs, double c, d3 a, d3 n
 // right-hand side always in FP64. `copy32`, when given, receives an FP32
 // copy of the (FP64) block.
 template <typename Scalar, bool Blocks, typename Acc>
-__device__ inline void node_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
+__device__ __forceinline__ void node_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
                                        const double* __restrict__ resid, const double* __restrict__ arm,
                                        const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
                                        const int32_t* __restrict__ eoff, const int32_t* __restrict__ eent,
@@ -421,7 +426,7 @@ __device__ inline void acc3(Acc (&out)[3], d3 v) {
 constexpr int kPairLanes = 8;
 
 template <typename Scalar, typename Acc>
-__device__ inline void pair_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
+__device__ __forceinline__ void pair_block_rows(const AssemblyInputs& in, const double* __restrict__ scale,
                                        const double* __restrict__ arm, int64_t groups,
                                        const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
                                        const int32_t* __restrict__ pos, Scalar* __restrict__ bsr,
@@ -549,6 +554,161 @@ system_blocks(AssemblyInputs in, const double* __restrict__ scale, const double*
                                         group_order);
 }
 
+// ---------------------------------------------------------------- packed ---
+// The assembly and the energy passes for a batch of jobs (packed.hpp). A
+// job's blocks are its single-job launch's blocks, so its records, blocks and
+// energies are the same bits packed or alone.
+
+// Records: the current state's rows (jobs that refresh or assemble this
+// tick); otherwise the candidate state's energies (jobs that solve).
+template <bool Records>
+__global__ void __launch_bounds__(kThreads)
+rows_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+            const PackedBlock* __restrict__ map) {
+    __shared__ double lds[3 * (kThreads / kWave)];
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    const int32_t flags = ctl[m.job].flags;
+    if (!(flags & (Records ? (kJobRefresh | kJobAssemble) : kJobActive))) return;
+    const PackedJob& J = jobs[m.job];
+    AssemblyInputs in = J.in;
+    if (!Records) {
+        in.rotations = J.rot_cand;
+        in.translations = J.trans_cand;
+    }
+    if (m.block < J.vertex_blocks)
+        vertex_rows_block<Records>(in, J.scale, J.resid, J.arm, J.partials, m.block, lds);
+    else
+        graph_energy_block(in, J.partials + 5 * static_cast<int64_t>(J.vertex_blocks), m.block - J.vertex_blocks,
+                           lds);
+}
+
+// Scalar: the blocks' type; Copy32: the mixed form (FP64 blocks into bsr64,
+// their FP32 copy into bsr).
+template <typename Scalar, bool Copy32>
+__global__ void __launch_bounds__(kThreads)
+system_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+              const PackedBlock* __restrict__ map) {
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & (kJobRefresh | kJobAssemble))) return;
+    const PackedJob& J = jobs[m.job];
+    Scalar* bsr = Copy32 ? reinterpret_cast<Scalar*>(J.bsr64) : static_cast<Scalar*>(J.bsr);
+    float* copy32 = Copy32 ? static_cast<float*>(J.bsr) : nullptr;
+    if (m.block < J.node_blocks)
+        node_block_rows<Scalar, true, Scalar>(J.in, J.scale, J.resid, J.arm, J.gather_offsets, J.gather_entries,
+                                              J.edge_offsets, J.edge_entries, J.diag_position, bsr, J.rhs, copy32,
+                                              m.block, J.node_order);
+    else
+        pair_block_rows<Scalar, Scalar>(J.in, J.scale, J.arm, J.group_count, J.group_offsets, J.group_entries,
+                                        J.pair_position, bsr, J.touched, copy32, m.block - J.node_blocks,
+                                        J.group_order);
+}
+
+// One workgroup per job: the current state's energies from its partials.
+__global__ void __launch_bounds__(kThreads)
+reduce_current_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl) {
+    __shared__ double lds[5 * (kThreads / kWave)];
+    const int job = blockIdx.x;
+    if (!(ctl[job].flags & (kJobRefresh | kJobAssemble))) return;
+    const PackedJob& J = jobs[job];
+    double e[5];
+    fold_partials(J.partials, J.vertex_blocks + J.graph_blocks, lds, e);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) J.energy_cur[k] = e[k];
+}
+
+__device__ inline double total5(const double* e) { return (((e[0] + e[1]) + e[2]) + e[3]) + e[4]; }
+
+__device__ inline double relative_change_dev(double before, double after) {
+    return (before - after) / fmax(fabs(before), 1e-12);
+}
+
+// One workgroup per job: the candidate's energies, then the decision the
+// single-job session takes after a damping attempt (route A semantics,
+// src/solver/route_a.cpp:131-169, with the outer loop's close): accept iff
+// the solve converged and the candidate's energy is below the current one;
+// damping times the decrease (floor 1e-12) on accept, times the increase on
+// reject or a failed solve; a new inner step after an accept, a retry while
+// attempts remain, else the outer iteration closes, and the level ends when
+// its outer loop converges or runs out.
+__global__ void __launch_bounds__(kThreads)
+decide_packed(const PackedJob* __restrict__ jobs, BatchControl* __restrict__ ctl, int32_t* __restrict__ host_flags) {
+    __shared__ double lds[5 * (kThreads / kWave)];
+    const int job = blockIdx.x;
+    BatchControl& c = ctl[job];
+    const int32_t flags = c.flags;
+    if (!(flags & kJobActive)) {
+        if (threadIdx.x == 0) {
+            c.flags = flags & ~kJobAccepted;
+            host_flags[job] = c.flags;
+        }
+        return;
+    }
+    const PackedJob& J = jobs[job];
+    double cand_e[5];
+    fold_partials(J.partials, J.vertex_blocks + J.graph_blocks, lds, cand_e);
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 5; ++k) J.energy_cand[k] = cand_e[k];
+    const double cur = total5(J.energy_cur);
+    if (c.attempt == 0) {
+        for (int k = 0; k < 5; ++k) c.last_energy[k] = J.energy_cur[k];
+        c.current_energy = cur;
+        if (c.outer == 0 && c.inner == 0) c.energy_start = cur;
+    }
+    const double iterations = J.outcome[0];
+    c.pcg_launches += 1.0;
+    c.pcg_iterations += iterations;
+    c.pcg_most = fmax(c.pcg_most, iterations);
+    c.gn_steps += 1.0;
+    bool accepted = false;
+    double cand = 0.0;
+    if (J.outcome[2] != 0.0) {
+        cand = total5(cand_e);
+        accepted = cand < cur;
+    }
+    int32_t next = 0;
+    bool close = false;
+    if (accepted) {
+        for (int k = 0; k < 5; ++k) {
+            c.last_energy[k] = cand_e[k];
+            J.energy_cur[k] = cand_e[k];
+        }
+        c.current_energy = cand;
+        c.damping = fmax(c.damping * c.damping_decrease, 1e-12);
+        c.inner += 1;
+        c.view_inner += 1;
+        c.attempt = 0;
+        next = kJobAccepted;
+        if (relative_change_dev(cur, cand) <= c.inner_tolerance || c.inner >= max(1, c.max_inner))
+            close = true;
+        else
+            next |= kJobActive | kJobAssemble;
+    } else {
+        c.damping *= c.damping_increase;
+        c.attempt += 1;
+        if (c.attempt < max(1, c.max_attempts))
+            next = kJobActive;
+        else
+            close = true;
+    }
+    if (close) {
+        const double level_energy = total5(c.last_energy);
+        c.energy_end = level_energy;
+        c.view_outer += 1;
+        bool converged = false;
+        if (c.outer > 0 && relative_change_dev(c.previous_level_energy, level_energy) <= c.outer_tolerance) {
+            c.converged = 1;
+            converged = true;
+        }
+        c.previous_level_energy = level_energy;
+        c.outer += 1;
+        c.attempt = 0;
+        c.inner = 0;
+        if (!converged && c.outer < max(1, c.max_outer)) next |= kJobActive | kJobRefresh;
+    }
+    c.flags = next;
+    host_flags[job] = next;
+}
+
 void launch_energies(const AssemblyInputs& in, AssemblyScratch& s, bool records, double* out, hipStream_t stream,
                      const HostReadback* readback = nullptr) {
     const int vblocks = std::max(1, blocks_for(in.vertex_count, kThreads));
@@ -610,6 +770,43 @@ void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScr
         L.gather_entries.get(), L.node_edge_offsets.get(), L.node_edge_entries.get(), L.diag_position.get(), nullptr,
         rhs, nullptr, L.node_order.get());
     NRICP_LAUNCH_CHECK("node_blocks");
+}
+
+void packed_rows(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, bool records,
+                 hipStream_t stream) {
+    if (maps.rows_blocks <= 0) return;
+    const unsigned blocks = static_cast<unsigned>(maps.rows_blocks);
+    if (records)
+        rows_packed<true><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.rows);
+    else
+        rows_packed<false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.rows);
+    NRICP_LAUNCH_CHECK("rows_packed");
+}
+
+void packed_system(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
+                   hipStream_t stream) {
+    if (maps.system_blocks <= 0) return;
+    const unsigned blocks = static_cast<unsigned>(maps.system_blocks);
+    if (precision == NRICP_HIP_PRECISION_FLOAT)
+        system_packed<float, false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
+    else if (precision == NRICP_HIP_PRECISION_MIXED)
+        system_packed<double, true><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
+    else
+        system_packed<double, false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
+    NRICP_LAUNCH_CHECK("system_packed");
+}
+
+void packed_reduce_current(const PackedJob* jobs, const BatchControl* ctl, int32_t job_count, hipStream_t stream) {
+    if (job_count <= 0) return;
+    reduce_current_packed<<<static_cast<unsigned>(job_count), kThreads, 0, stream>>>(jobs, ctl);
+    NRICP_LAUNCH_CHECK("reduce_current_packed");
+}
+
+void packed_decide(const PackedJob* jobs, BatchControl* ctl, int32_t job_count, int32_t* host_flags,
+                   hipStream_t stream) {
+    if (job_count <= 0) return;
+    decide_packed<<<static_cast<unsigned>(job_count), kThreads, 0, stream>>>(jobs, ctl, host_flags);
+    NRICP_LAUNCH_CHECK("decide_packed");
 }
 
 template void assemble<double>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, double*, double*, double*,

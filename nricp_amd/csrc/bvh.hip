@@ -12,6 +12,8 @@
 #include <cfloat>
 
 #include "kernels.hpp"
+#include "packed.hpp"
+#include "xcd_map.hpp"
 
 namespace nricp_hip {
 namespace {
@@ -23,10 +25,9 @@ namespace {
 // route performs per vertex. A face's cross product is recomputed by each of
 // its vertices (same operations, same bits) rather than stored by a pass of
 // its own, which saves a launch per correspondence pass.
-__global__ void vertex_normals_kernel(const double* __restrict__ p, const int32_t* __restrict__ f,
-                                      const int32_t* __restrict__ off, const int32_t* __restrict__ faces, int64_t nv,
-                                      double* __restrict__ out) {
-    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void vertex_normal(const double* __restrict__ p, const int32_t* __restrict__ f,
+                                     const int32_t* __restrict__ off, const int32_t* __restrict__ faces, int64_t nv,
+                                     double* __restrict__ out, int64_t v) {
     if (v >= nv) return;
     d3 s{0.0, 0.0, 0.0};
     for (int32_t k = off[v]; k < off[v + 1]; ++k) {
@@ -35,6 +36,12 @@ __global__ void vertex_normals_kernel(const double* __restrict__ p, const int32_
         s = s + cross(b - a, c - a);
     }
     st3(out, v, unit(s));
+}
+
+__global__ void vertex_normals_kernel(const double* __restrict__ p, const int32_t* __restrict__ f,
+                                      const int32_t* __restrict__ off, const int32_t* __restrict__ faces, int64_t nv,
+                                      double* __restrict__ out) {
+    vertex_normal(p, f, off, faces, nv, out, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 __global__ void vertex_face_entries(const int32_t* __restrict__ f, int64_t nf, uint64_t* __restrict__ keys,
@@ -495,7 +502,7 @@ __device__ inline void rank_step(float key, int j, int& rank) {
 }
 
 template <int StackDepth>
-__device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
+__device__ __forceinline__ void nearest_wide(d3 p, const WideNode* __restrict__ wide, int64_t wide_count,
                                     const double* __restrict__ corners,
                                     int32_t* stack_node, float* stack_dist, int j, Nearest& best) {
     int sp = 0;
@@ -592,20 +599,21 @@ __device__ inline void nearest_wide(d3 p, const WideNode* __restrict__ wide, int
 // src/solver/route_a.cpp:19-38 for accepted rows (written by the group's
 // first lane).
 template <int StackDepth>
-__global__ void __launch_bounds__(kQueryThreads)
-query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, const double* __restrict__ corners,
-             const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
-             const double* __restrict__ vnormals, const double* __restrict__ queries,
-             const double* __restrict__ qnormals, const uint8_t* __restrict__ mask, int64_t count,
-             QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
-             double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
-             int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
-             const int32_t* __restrict__ order, uint32_t* __restrict__ stats) {
-    __shared__ int32_t stack_node[kGroupsPerBlock][StackDepth];
-    __shared__ float stack_dist[kGroupsPerBlock][StackDepth];
+__device__ __forceinline__ void query_block(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt,
+                                   const double* __restrict__ corners, const int32_t* __restrict__ leaf_triangle,
+                                   const int32_t* __restrict__ faces, const double* __restrict__ vnormals,
+                                   const double* __restrict__ queries, const double* __restrict__ qnormals,
+                                   const uint8_t* __restrict__ mask, int64_t count, const QuerySettings& s,
+                                   const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
+                                   double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
+                                   int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej,
+                                   double* __restrict__ out_u, const int32_t* __restrict__ order,
+                                   uint32_t* __restrict__ stats, int64_t block,
+                                   int32_t (&stack_node)[kGroupsPerBlock][StackDepth],
+                                   float (&stack_dist)[kGroupsPerBlock][StackDepth]) {
     const int j = threadIdx.x % kLanes;
     const int group = threadIdx.x / kLanes;
-    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes;
+    const int64_t slot = (block * kQueryThreads + threadIdx.x) / kLanes;
     if (slot >= count) return;  // the whole group leaves together
     const int64_t i = order ? static_cast<int64_t>(order[slot]) : slot;
     const bool writer = j == 0;
@@ -697,6 +705,54 @@ query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, 
                        ? robust_weight(s.robust_kernel, norm2(off), s.robust_threshold)
                        : 1.0;
     }
+}
+
+template <int StackDepth>
+__global__ void __launch_bounds__(kQueryThreads)
+query_kernel(const WideNode* __restrict__ wide, int64_t wide_count, int64_t nt, const double* __restrict__ corners,
+             const int32_t* __restrict__ leaf_triangle, const int32_t* __restrict__ faces,
+             const double* __restrict__ vnormals, const double* __restrict__ queries,
+             const double* __restrict__ qnormals, const uint8_t* __restrict__ mask, int64_t count,
+             QuerySettings s, const double* __restrict__ tpoints, const int32_t* __restrict__ warm,
+             double* __restrict__ out_p, double* __restrict__ out_n, double* __restrict__ out_d,
+             int32_t* __restrict__ out_t, uint8_t* __restrict__ out_rej, double* __restrict__ out_u,
+             const int32_t* __restrict__ order, uint32_t* __restrict__ stats) {
+    __shared__ int32_t stack_node[kGroupsPerBlock][StackDepth];
+    __shared__ float stack_dist[kGroupsPerBlock][StackDepth];
+    query_block<StackDepth>(wide, wide_count, nt, corners, leaf_triangle, faces, vnormals, queries, qnormals, mask,
+                            count, s, tpoints, warm, out_p, out_n, out_d, out_t, out_rej, out_u, order, stats,
+                            blockIdx.x, stack_node, stack_dist);
+}
+
+// ---------------------------------------------------------------- packed ---
+// Normals and correspondences of the jobs that refresh this tick (packed.hpp).
+
+__global__ void __launch_bounds__(256)
+normals_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+               const PackedBlock* __restrict__ map) {
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & kJobRefresh)) return;
+    const PackedJob& J = jobs[m.job];
+    vertex_normal(J.deformed, J.faces, J.vf_offsets, J.vf_faces, J.in.vertex_count, J.query_normals,
+                  static_cast<int64_t>(m.block) * 256 + threadIdx.x);
+}
+
+template <int StackDepth>
+__global__ void __launch_bounds__(kQueryThreads)
+query_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+             const PackedBlock* __restrict__ map) {
+    __shared__ int32_t stack_node[kGroupsPerBlock][StackDepth];
+    __shared__ float stack_dist[kGroupsPerBlock][StackDepth];
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & kJobRefresh)) return;
+    const PackedJob& J = jobs[m.job];
+    // Every refresh after the rest-template pass starts from the last
+    // triangles (the single-job session's warm start).
+    query_block<StackDepth>(J.wide, J.wide_count, J.triangle_count, J.leaf_corners, J.leaf_triangle, J.target_faces,
+                            J.target_normals, J.deformed, J.query_normals, J.mask, J.in.vertex_count, J.query,
+                            J.target_points, J.corr_triangle, J.corr_points, J.corr_normals, J.corr_distance,
+                            J.corr_triangle, J.rejection, J.robust, J.query_order, nullptr, m.block, stack_node,
+                            stack_dist);
 }
 
 // Traversal statistics of one launch (profiling): per-query (opens, tests)
@@ -948,6 +1004,37 @@ void closest_points(const Lbvh& bvh, const double* queries, const double* query_
         launch(query_kernel<128>);
     else
         launch(query_kernel<296>);
+}
+
+int32_t packed_query_stack(int32_t depth_max) {
+    const int64_t need = stack_entries_needed(depth_max);
+    return need <= 64 ? 64 : need <= 96 ? 96 : need <= 128 ? 128 : 296;
+}
+
+void packed_normals(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+    if (maps.vertex_blocks <= 0) return;
+    normals_packed<<<static_cast<unsigned>(maps.vertex_blocks), 256, 0, stream>>>(jobs, ctl, maps.vertex);
+    NRICP_LAUNCH_CHECK("normals_packed");
+}
+
+void packed_query(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+    if (maps.query_blocks <= 0) return;
+    const unsigned blocks = static_cast<unsigned>(maps.query_blocks);
+    switch (maps.query_stack) {
+        case 64:
+            query_packed<64><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            break;
+        case 96:
+            query_packed<96><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            break;
+        case 128:
+            query_packed<128><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            break;
+        default:
+            query_packed<296><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            break;
+    }
+    NRICP_LAUNCH_CHECK("query_packed");
 }
 
 void fold_query_stats(QueryStats& stats, int64_t count, hipStream_t stream) {

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "server.hpp"
 #include "session.hpp"
 
@@ -885,3 +886,64 @@ int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats*
         *out = server->server->stats();
     });
 }
+
+// ----------------------------------------------------------------- batch --
+
+struct nricp_hip_batch {
+    std::vector<nricp_hip_session> sessions;  // each owns its job's session
+    std::unique_ptr<BatchEngine> engine;      // released first (it drives the sessions)
+    ~nricp_hip_batch() { engine.reset(); }
+};
+
+int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_settings* settings, int32_t count,
+                           const nricp_hip_batch_config* config, nricp_hip_batch** out) {
+    return guarded([&] {
+        require(problems && settings && out && count > 0, "batch_create: null argument or empty batch");
+        ensure_device();
+        auto b = std::make_unique<nricp_hip_batch>();
+        b->sessions.reserve(static_cast<size_t>(count));
+        std::vector<Session*> jobs;
+        for (int32_t i = 0; i < count; ++i) {
+            b->sessions.emplace_back();
+            b->sessions.back().session = std::make_unique<Session>(problems[i], settings[i], nullptr);
+            jobs.push_back(b->sessions.back().session.get());
+        }
+        BatchConfig c;
+        if (config) {
+            c.pcg_slots = config->pcg_slots;
+            c.graphs = config->use_graphs != 0;
+        }
+        b->engine = std::make_unique<BatchEngine>(std::move(jobs), c);
+        *out = b.release();
+    });
+}
+
+int nricp_hip_batch_run(nricp_hip_batch* batch) {
+    return guarded([&] {
+        require(batch && batch->engine, "batch_run: null handle");
+        batch->engine->run();
+    });
+}
+
+int nricp_hip_batch_job(nricp_hip_batch* batch, int32_t index, nricp_hip_session** out) {
+    return guarded([&] {
+        require(batch && out, "batch_job: null argument");
+        require(index >= 0 && static_cast<size_t>(index) < batch->sessions.size(), "batch_job: index out of range");
+        *out = &batch->sessions[static_cast<size_t>(index)];
+    });
+}
+
+int nricp_hip_batch_stats(const nricp_hip_batch* batch, double* out, int32_t capacity, int32_t* count) {
+    return guarded([&] {
+        require(batch && batch->engine && count, "batch_stats: null argument");
+        const BatchStats& s = batch->engine->stats();
+        const double v[] = {static_cast<double>(s.ticks),       static_cast<double>(s.replays),
+                            static_cast<double>(s.graphs),      static_cast<double>(s.full_ticks),
+                            static_cast<double>(s.assemble_ticks), s.seconds,
+                            s.level_setup_seconds,              s.tick_seconds};
+        *count = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
+        for (int32_t i = 0; i < std::min(capacity, *count); ++i) out[i] = v[i];
+    });
+}
+
+void nricp_hip_batch_destroy(nricp_hip_batch* batch) { delete batch; }

@@ -192,6 +192,11 @@ int persistent_group_budget(const void* kernel, int threads);  // groups one per
 // reach the device back to back instead of around the admission.
 void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream,
                        void (*prelude)(void*) = nullptr, void* prelude_arg = nullptr);
+// The same admission for work the caller enqueues itself (a graph holding a
+// persistent kernel, or a packed launch whose jobs each keep `groups` groups
+// resident at most): waits until `groups` groups fit, calls enqueue(arg) and
+// records the ledger's event after it on `stream`.
+void admit_persistent(int groups, hipStream_t stream, hipError_t (*enqueue)(void*), void* arg);
 
 // --------------------------------------------------------- grid barrier ----
 

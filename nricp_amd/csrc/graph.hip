@@ -12,7 +12,9 @@
 #include <cfloat>
 
 #include "kernels.hpp"
+#include "packed.hpp"
 #include "so3.hpp"
+#include "xcd_map.hpp"
 
 namespace nricp_hip {
 namespace {
@@ -59,11 +61,10 @@ __device__ inline double qdot(quat a, quat b) { return a.w * b.w + a.x * b.x + a
 
 // ------------------------------------------------------------ evaluate -----
 
-__global__ void evaluate_kernel(const double* __restrict__ p, const int32_t* __restrict__ bn,
-                                const double* __restrict__ bw, const int32_t* __restrict__ bc, int64_t nv,
-                                const double* __restrict__ R, const double* __restrict__ T,
-                                const double* __restrict__ G, int64_t nn, double* __restrict__ out) {
-    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void evaluate_vertex(const double* __restrict__ p, const int32_t* __restrict__ bn,
+                                       const double* __restrict__ bw, const int32_t* __restrict__ bc, int64_t nv,
+                                       const double* __restrict__ R, const double* __restrict__ T,
+                                       const double* __restrict__ G, int64_t nn, double* __restrict__ out, int64_t v) {
     if (v >= nv) return;
     const d3 x = ld3(p, v);
     d3 s{0.0, 0.0, 0.0};
@@ -78,10 +79,16 @@ __global__ void evaluate_kernel(const double* __restrict__ p, const int32_t* __r
     st3(out, v, s);
 }
 
-__global__ void retract_kernel(const double* __restrict__ R, const double* __restrict__ T,
-                               const double* __restrict__ delta, int64_t nn, double* __restrict__ Ro,
-                               double* __restrict__ To) {
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ void evaluate_kernel(const double* __restrict__ p, const int32_t* __restrict__ bn,
+                                const double* __restrict__ bw, const int32_t* __restrict__ bc, int64_t nv,
+                                const double* __restrict__ R, const double* __restrict__ T,
+                                const double* __restrict__ G, int64_t nn, double* __restrict__ out) {
+    evaluate_vertex(p, bn, bw, bc, nv, R, T, G, nn, out, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+__device__ __forceinline__ void retract_node(const double* __restrict__ R, const double* __restrict__ T,
+                                    const double* __restrict__ delta, int64_t nn, double* __restrict__ Ro,
+                                    double* __restrict__ To, int64_t j) {
     if (j >= nn) return;
     const double* d = delta + 6 * j;
     const m3 e = exp_so3(d3{d[0], d[1], d[2]});
@@ -91,6 +98,50 @@ __global__ void retract_kernel(const double* __restrict__ R, const double* __res
     const d3 t = ld3(T, j) + d3{d[3], d[4], d[5]};
     for (int i = 0; i < 9; ++i) Ro[9 * j + i] = o.m[i];
     st3(To, j, t);
+}
+
+__global__ void retract_kernel(const double* __restrict__ R, const double* __restrict__ T,
+                               const double* __restrict__ delta, int64_t nn, double* __restrict__ Ro,
+                               double* __restrict__ To) {
+    retract_node(R, T, delta, nn, Ro, To, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- packed ---
+// The same per-vertex and per-node work for a batch of jobs (packed.hpp):
+// workgroup b takes the block the stage's map gives it (in the XCD-contiguous
+// order), and returns at once when its job has no work in the stage.
+
+__global__ void __launch_bounds__(256)
+evaluate_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+                const PackedBlock* __restrict__ map) {
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & kJobRefresh)) return;
+    const PackedJob& J = jobs[m.job];
+    evaluate_vertex(J.in.points, J.in.bind_nodes, J.in.bind_weights, J.in.bind_counts, J.in.vertex_count, J.rot,
+                    J.trans, J.in.rest, J.in.node_count, J.deformed, static_cast<int64_t>(m.block) * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(128)
+retract_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+               const PackedBlock* __restrict__ map) {
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & kJobActive)) return;
+    const PackedJob& J = jobs[m.job];
+    retract_node(J.rot, J.trans, J.delta, J.in.node_count, J.rot_cand, J.trans_cand,
+                 static_cast<int64_t>(m.block) * 128 + threadIdx.x);
+}
+
+// An accepted candidate becomes the state.
+__global__ void __launch_bounds__(128)
+commit_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+              const PackedBlock* __restrict__ map) {
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    if (!(ctl[m.job].flags & kJobAccepted)) return;
+    const PackedJob& J = jobs[m.job];
+    const int64_t j = static_cast<int64_t>(m.block) * 128 + threadIdx.x;
+    if (j >= J.in.node_count) return;
+    for (int i = 0; i < 9; ++i) J.rot[9 * j + i] = J.rot_cand[9 * j + i];
+    for (int i = 0; i < 3; ++i) J.trans[3 * j + i] = J.trans_cand[3 * j + i];
 }
 
 // ---------------------------------------------------------- sampling -------
@@ -765,6 +816,24 @@ void retract(const double* R, const double* T, const double* delta, int64_t nn, 
     if (nn <= 0) return;
     retract_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(R, T, delta, nn, Ro, To);
     NRICP_LAUNCH_CHECK("retract_kernel");
+}
+
+void packed_evaluate(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+    if (maps.vertex_blocks <= 0) return;
+    evaluate_packed<<<static_cast<unsigned>(maps.vertex_blocks), 256, 0, stream>>>(jobs, ctl, maps.vertex);
+    NRICP_LAUNCH_CHECK("evaluate_packed");
+}
+
+void packed_retract(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+    if (maps.node_blocks <= 0) return;
+    retract_packed<<<static_cast<unsigned>(maps.node_blocks), 128, 0, stream>>>(jobs, ctl, maps.node);
+    NRICP_LAUNCH_CHECK("retract_packed");
+}
+
+void packed_commit(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+    if (maps.node_blocks <= 0) return;
+    commit_packed<<<static_cast<unsigned>(maps.node_blocks), 128, 0, stream>>>(jobs, ctl, maps.node);
+    NRICP_LAUNCH_CHECK("commit_packed");
 }
 
 namespace {

@@ -811,6 +811,50 @@ nricp_hip_problem problem_of(const Prepared& p) {
     return pb;
 }
 
+// The result dict of a finished session (route_result_to_dict,
+// src/python/bindings.cpp:674-710): fitted vertices in caller units, the
+// finest level's node state, the report, the level sizes and, when asked,
+// the energy trace.
+py::dict session_result(nricp_hip_session* h, const Prepared& prep, bool trace) {
+    py::dict d;
+    std::vector<double> v(prep.tp.size());
+    int st;
+    {
+        py::gil_scoped_release release;
+        st = nricp_hip_session_vertices(h, v.data());
+    }
+    check(st);
+    denormalize(v, prep.norm);
+    d["vertices"] = arr2(v.data(), static_cast<py::ssize_t>(v.size() / 3), 3);
+    int64_t nn = 0;
+    int32_t levels = 0;
+    check(nricp_hip_session_sizes(h, &nn, &levels));
+    std::vector<double> R(9 * nn), T(3 * nn), G(3 * nn);
+    check(nricp_hip_session_state(h, R.data(), T.data(), G.data()));
+    py::array_t<double> rot({static_cast<py::ssize_t>(nn), static_cast<py::ssize_t>(3), static_cast<py::ssize_t>(3)});
+    if (nn > 0) std::copy(R.begin(), R.end(), rot.mutable_data());
+    d["rotations"] = rot;
+    d["translations"] = arr2(T.data(), nn, 3);
+    d["node_positions"] = arr2(G.data(), nn, 3);
+    size_t len = 0;
+    check(nricp_hip_session_report(h, nullptr, 0, &len));
+    std::string text(len + 1, '\0');
+    check(nricp_hip_session_report(h, text.data(), text.size(), &len));
+    text.resize(len);
+    d["report"] = text;
+    std::vector<int64_t> level_nodes(levels);
+    check(nricp_hip_session_level_nodes(h, level_nodes.data()));
+    d["level_nodes"] = ints1(level_nodes);
+    if (trace) {
+        int64_t count = 0;
+        check(nricp_hip_session_trace(h, nullptr, 0, &count));
+        py::array_t<double> tr(static_cast<py::ssize_t>(count));
+        if (count > 0) check(nricp_hip_session_trace(h, tr.mutable_data(), count, &count));
+        d["trace_energy"] = tr;
+    }
+    return d;
+}
+
 // The device session behind RouteBSession and the batch routes.
 class Session {
 public:
@@ -884,27 +928,7 @@ public:
         text.resize(len);
         return text;
     }
-    py::dict result() {
-        py::dict d;
-        d["vertices"] = vertices();
-        int64_t nn = 0;
-        check(nricp_hip_session_sizes(handle_, &nn, nullptr));
-        std::vector<double> R(9 * nn), T(3 * nn), G(3 * nn);
-        check(nricp_hip_session_state(handle_, R.data(), T.data(), G.data()));
-        py::array_t<double> rot({static_cast<py::ssize_t>(nn), static_cast<py::ssize_t>(3), static_cast<py::ssize_t>(3)});
-        if (nn > 0) std::copy(R.begin(), R.end(), rot.mutable_data());
-        d["rotations"] = rot;
-        d["translations"] = arr2(T.data(), nn, 3);
-        d["node_positions"] = arr2(G.data(), nn, 3);
-        d["report"] = report();
-        d["level_nodes"] = ints1(level_nodes_);
-        int64_t count = 0;
-        check(nricp_hip_session_trace(handle_, nullptr, 0, &count));
-        py::array_t<double> trace(static_cast<py::ssize_t>(count));
-        if (count > 0) check(nricp_hip_session_trace(handle_, trace.mutable_data(), count, &count));
-        d["trace_energy"] = trace;
-        return d;
-    }
+    py::dict result() { return session_result(handle_, *prep_, true); }
     void reset() { check(nricp_hip_session_reset(handle_)); }
     py::dict counters() const {
         static const char* names[] = {"pcg_launches", "pcg_iterations", "pcg_bytes", "assemble_launches",
@@ -931,6 +955,78 @@ private:
     nricp_hip_session* handle_ = nullptr;
     int status_ = 0;
     std::vector<int64_t> level_nodes_;
+};
+
+// Lockstep batches (nricp_hip_batch_*): jobs are added with the route
+// arguments, normalised as solve_route_* normalises them; run() solves them
+// all through the packed launches and returns each job's result dict (the
+// layout of solve_route_*, without the energy trace).
+class Batch {
+public:
+    Batch(int pcg_slots, bool use_graphs) {
+        config_.pcg_slots = pcg_slots;
+        config_.use_graphs = use_graphs ? 1 : 0;
+    }
+    ~Batch() { close(); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    void close() {
+        if (!handle_) return;
+        py::gil_scoped_release release;
+        nricp_hip_batch_destroy(handle_);
+        handle_ = nullptr;
+    }
+    int add(std::unique_ptr<Prepared> p) {
+        if (handle_) throw py::value_error("the batch has run; create a new one for more jobs");
+        jobs_.push_back(std::move(p));
+        return static_cast<int>(jobs_.size() - 1);
+    }
+    py::list run() {
+        if (handle_) throw py::value_error("the batch has already run");
+        if (jobs_.empty()) throw py::value_error("the batch has no jobs");
+        std::vector<nricp_hip_problem> problems;
+        std::vector<nricp_hip_settings> settings;
+        for (const auto& p : jobs_) {
+            problems.push_back(problem_of(*p));
+            settings.push_back(p->s);
+        }
+        nricp_hip_batch* h = nullptr;
+        int st;
+        {
+            py::gil_scoped_release release;
+            st = nricp_hip_batch_create(problems.data(), settings.data(), static_cast<int32_t>(jobs_.size()), &config_,
+                                        &h);
+            if (st == 0) {
+                handle_ = h;
+                st = nricp_hip_batch_run(h);
+            }
+        }
+        check(st);
+        py::list out;
+        for (size_t i = 0; i < jobs_.size(); ++i) {
+            nricp_hip_session* s = nullptr;
+            check(nricp_hip_batch_job(handle_, static_cast<int32_t>(i), &s));
+            out.append(session_result(s, *jobs_[i], false));
+        }
+        return out;
+    }
+    py::dict stats() const {
+        if (!handle_) throw py::value_error("the batch has not run");
+        static const char* names[] = {"ticks", "graph_replays", "graphs_captured", "refresh_ticks",
+                                      "assemble_ticks", "seconds", "level_setup_seconds", "tick_seconds"};
+        double v[8];
+        int32_t n = 0;
+        check(nricp_hip_batch_stats(handle_, v, 8, &n));
+        py::dict d;
+        for (int32_t i = 0; i < n && i < 8; ++i) d[names[i]] = v[i];
+        return d;
+    }
+
+private:
+    nricp_hip_batch_config config_{2, 1};
+    std::vector<std::unique_ptr<Prepared>> jobs_;
+    nricp_hip_batch* handle_ = nullptr;
 };
 
 // The request loop (nricp_hip_server_*): jobs are submitted with the route
@@ -1722,6 +1818,33 @@ PYBIND11_MODULE(nricp_dp, m) {
         .def("counters", &Session::counters)
         .def("profile", &Session::profile, py::arg("enable") = true)
         .def("kernel_time", &Session::kernel_time, py::arg("stage"));
+
+    py::class_<Batch>(m, "Batch",
+                      "A lockstep batch of wrap jobs: add() the jobs (route_a or route_b arguments, one precision "
+                      "for all), then run() solves them together -- per graph level, every damping attempt of "
+                      "every job is one shared launch per stage (closest points, assembly, one persistent PCG over "
+                      "all jobs' block rows, energies, the accept/reject decision on the device), captured as a "
+                      "graph and replayed -- and returns the result dicts in order. pcg_slots (1 or 2) is the "
+                      "packed PCG's node slots per lane group. A job's result does not depend on the batch.")
+        .def(py::init<int, bool>(), py::arg("pcg_slots") = 2, py::arg("use_graphs") = true)
+        .def("add",
+             [](Batch& self, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
+                const py::object& mk, const py::object& li, const py::object& lt, double dist, double mincos,
+                double slope, double d0, double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double pt,
+                int pm, int nb, const std::string& route, const std::string& precision) {
+                 const int r = route_of(route);
+                 if (r == NRICP_HIP_ROUTE_C) throw py::value_error("the local/global route does not run in a batch");
+                 const bool exact = r == NRICP_HIP_ROUTE_A;
+                 return self.add(prepare(tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag,
+                                         gl, n0, g, exact ? kRouteATolerance : pt, exact ? kRouteAMaxIterations : pm, nb,
+                                         r, exact ? "double" : precision));
+             },
+             ROUTE_ARGS, py::arg("route") = "route_b", py::arg("precision") = "mixed")
+        .def("run", &Batch::run)
+        .def("stats", &Batch::stats)
+        .def("close", &Batch::close)
+        .def("__enter__", [](Batch& self) -> Batch& { return self; }, py::return_value_policy::reference)
+        .def("__exit__", [](Batch& self, const py::object&, const py::object&, const py::object&) { self.close(); });
 
     py::class_<Server>(m, "Server",
                        "A request loop over wrap jobs: submit() copies a job and returns a ticket at once; jobs run "

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "packed.hpp"
+#include "xcd_map.hpp"
 
 namespace nricp_hip {
 namespace {
@@ -90,11 +92,10 @@ __device__ inline bool spd_inverse6(const double (&a)[36], double jitter, double
 // src/cuda/pcg_solver.cu:15-46). The jitter only shapes the preconditioner;
 // the system keeps its damping.
 template <typename S>
-__global__ void precondition_kernel(const S* __restrict__ bsr, const double* __restrict__ bsr64,
-                                    const int32_t* __restrict__ diag_pos, int64_t nn, double lambda, int scaling,
-                                    S* __restrict__ damp, double* __restrict__ damp64, S* __restrict__ minv,
-                                    unsigned* __restrict__ barrier) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void precondition_node(const S* __restrict__ bsr, const double* __restrict__ bsr64,
+                                         const int32_t* __restrict__ diag_pos, int64_t nn, double lambda, int scaling,
+                                         S* __restrict__ damp, double* __restrict__ damp64, S* __restrict__ minv,
+                                         unsigned* __restrict__ barrier, int64_t i) {
     // The solve's grid-barrier counter starts at zero (the solve runs next on
     // the same stream).
     if (i == 0) barrier[0] = barrier[1] = 0u;
@@ -135,10 +136,33 @@ __global__ void precondition_kernel(const S* __restrict__ bsr, const double* __r
             minv[36 * i + 6 * r + c] = static_cast<S>(ok ? inv[6 * r + c] : (r == c ? 1.0 : 0.0));
 }
 
-template <bool Grid>
-__device__ inline void sync_all(unsigned* barrier) {
+template <typename S>
+__global__ void precondition_kernel(const S* __restrict__ bsr, const double* __restrict__ bsr64,
+                                    const int32_t* __restrict__ diag_pos, int64_t nn, double lambda, int scaling,
+                                    S* __restrict__ damp, double* __restrict__ damp64, S* __restrict__ minv,
+                                    unsigned* __restrict__ barrier) {
+    precondition_node<S>(bsr, bsr64, diag_pos, nn, lambda, scaling, damp, damp64, minv, barrier,
+                         static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// The solve's groups: a whole launch (group blockIdx.x of gridDim.x), or one
+// job's share of a packed launch, whose group index and count the packed
+// kernel keeps in LDS (read where used, so the long solve holds no register
+// for them).
+struct LaunchGroups {
+    __device__ static unsigned group() { return blockIdx.x; }
+    __device__ static unsigned groups() { return gridDim.x; }
+};
+__shared__ unsigned packed_group_set[2];
+struct PackedGroups {
+    __device__ static unsigned group() { return packed_group_set[0]; }
+    __device__ static unsigned groups() { return packed_group_set[1]; }
+};
+
+template <bool Grid, typename GS>
+__device__ inline void sync_all(unsigned* barrier, GS gs) {
     if (Grid)
-        grid_barrier(barrier, gridDim.x);
+        grid_barrier(barrier, gs.groups());
     else
         __syncthreads();
 }
@@ -151,9 +175,9 @@ __device__ inline void sync_all(unsigned* barrier) {
 // obtains the same bits.
 constexpr int kPcgWaves = kPcgThreads / kWave;
 
-template <int Count>
+template <int Count, typename GS>
 __device__ inline void publish_and_wait(const double (&v)[Count], double* lds /* Count x waves */, double* slots,
-                                        unsigned* barrier) {
+                                        unsigned* barrier, GS gs) {
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
     for (int k = 0; k < Count; ++k) {
@@ -166,18 +190,18 @@ __device__ inline void publish_and_wait(const double (&v)[Count], double* lds /*
         for (int k = 0; k < Count; ++k) {
             double s = lds[k * kPcgWaves];
             for (int w = 1; w < kPcgWaves; ++w) s += lds[k * kPcgWaves + w];
-            slots[Count * blockIdx.x + k] = s;
+            slots[Count * gs.group() + k] = s;
         }
-        grid_arrive_wait(barrier, gridDim.x);
+        grid_arrive_wait(barrier, gs.groups());
     }
     __syncthreads();
 }
-template <int Count>
-__device__ inline void fold(const double* slots, double (&out)[Count]) {
+template <int Count, typename GS>
+__device__ inline void fold(const double* slots, double (&out)[Count], GS gs) {
     double acc[Count];
 #pragma unroll
     for (int k = 0; k < Count; ++k) acc[k] = 0.0;
-    for (unsigned t = threadIdx.x % kWave; t < gridDim.x; t += kWave)
+    for (unsigned t = threadIdx.x % kWave; t < gs.groups(); t += kWave)
 #pragma unroll
         for (int k = 0; k < Count; ++k) acc[k] += slots[Count * t + k];
 #pragma unroll
@@ -186,15 +210,15 @@ __device__ inline void fold(const double* slots, double (&out)[Count]) {
 
 // One launch-wide sum with its own synchronisation (used where no vector
 // exchange shares the barrier).
-template <int Count, bool Grid>
+template <int Count, bool Grid, typename GS>
 __device__ inline void launch_sum(const double (&v)[Count], double* slots, double* lds, unsigned* barrier,
-                                  double (&out)[Count]) {
+                                  double (&out)[Count], GS gs) {
     if (!Grid) {
         block_sum<Count, kPcgThreads>(v, lds, out);
         return;
     }
-    publish_and_wait<Count>(v, lds, slots, barrier);
-    fold<Count>(slots, out);
+    publish_and_wait<Count>(v, lds, slots, barrier, gs);
+    fold<Count>(slots, out, gs);
 }
 
 // Row `lane` of the node's preconditioner block times the node's 6-vector,
@@ -372,16 +396,21 @@ constexpr double kResidualGamma = 256.0 * 1.1102230246251565e-16;
 // Exchange buffers are double-buffered by barrier parity: data written before
 // barrier j is read after it, and the same half is next written after barrier
 // j + 1, when every block has finished reading it.
-template <int K, typename S, bool Grid>
-__global__ void __launch_bounds__(kPcgThreads)
-pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
-           const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ bsr64,
-           const double* __restrict__ damp64, const double* __restrict__ rhs, int64_t nn,
-           double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
-           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
-           unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
-    __shared__ double lds[3 * (kPcgThreads / kWave)];
-    __shared__ float4 matrix_lds[kMatrixLdsBytes / 16];
+//
+// The body serves the whole launch (group = blockIdx.x of gridDim.x) or one
+// job's groups of a packed launch; `lds` holds 3 x waves doubles and
+// `matrix_lds` kMatrixLdsBytes.
+constexpr int kPcgLdsDoubles = 3 * (kPcgThreads / kWave);
+
+template <int K, typename S, bool Grid, typename GS>
+__device__ __forceinline__ void pcg_body(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols,
+                                const S* __restrict__ bsr, const S* __restrict__ damp, const S* __restrict__ minv,
+                                const double* __restrict__ bsr64, const double* __restrict__ damp64,
+                                const double* __restrict__ rhs, int64_t nn, double tol, double inner_tol, int max_iter,
+                                S* __restrict__ vbuf /* 2 x 6N */, double* __restrict__ xbuf /* 6N */,
+                                double* __restrict__ slots /* 2 x 3 x groups */, unsigned* __restrict__ barrier,
+                                double* __restrict__ solution, double* __restrict__ outcome, GS gs,
+                                double* lds, float4* matrix_lds) {
     const int lane = threadIdx.x & 7;
     const int lane_base = (threadIdx.x & 63) & ~7;
     const bool row_lane = lane < 6;
@@ -397,7 +426,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     double bb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        node[k] = (static_cast<int64_t>(blockIdx.x) * K + k) * kNodesPerPass + (threadIdx.x >> 3);
+        node[k] = (static_cast<int64_t>(gs.group()) * K + k) * kNodesPerPass + (threadIdx.x >> 3);
         live[k] = node[k] < nn;
         own[k] = row_lane && live[k];
         share[k] = row_share(row_off, cols, node[k], live[k], lane);
@@ -414,7 +443,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     // The group's block rows into LDS, as far as they fit.
     MatrixView<S> mat;
     {
-        const int64_t first = static_cast<int64_t>(blockIdx.x) * K * kNodesPerPass;
+        const int64_t first = static_cast<int64_t>(gs.group()) * K * kNodesPerPass;
         const int64_t last = first + K * kNodesPerPass < nn ? first + K * kNodesPerPass : nn;
         mat.global = bsr;
         mat.local = reinterpret_cast<const S*>(matrix_lds);
@@ -432,7 +461,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
     {
         const double v[1] = {bb};
         double o[1];
-        launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+        launch_sum<1, Grid>(v, slots + phase * 3 * gs.groups(), lds, barrier, o, gs);
         phase ^= 1;
         red[0] = o[0];
     }
@@ -453,7 +482,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             if (own[k]) ex[6 * node[k] + lane] = u[k];
             x[k] = p[k] = s[k] = q[k] = z[k] = S(0);
         }
-        sync_all<Grid>(barrier);
+        sync_all<Grid>(barrier, gs);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const S v = node_product<S, S, S>(share[k], cols, mat, dampk[k], u[k], ex, live[k], lane);
@@ -478,8 +507,8 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
                 part[2] += rk * rk;
             }
             if (Grid) {
-                publish_and_wait<3>(part, lds, slots + phase * 3 * gridDim.x, barrier);
-                fold<3>(slots + phase * 3 * gridDim.x, red);
+                publish_and_wait<3>(part, lds, slots + phase * 3 * gs.groups(), barrier, gs);
+                fold<3>(slots + phase * 3 * gs.groups(), red, gs);
             } else {
                 block_sum<3, kPcgThreads>(part, lds, red);
             }
@@ -541,7 +570,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             x[k] = S(0);
             if (own[k]) xbuf[6 * node[k] + lane] = x64[k];
         }
-        sync_all<Grid>(barrier);
+        sync_all<Grid>(barrier, gs);
         double rr = 0.0;
         double r64[K];
         // The residual of the FP64 system when its matrix is given (the mixed
@@ -560,7 +589,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         {
             const double v[1] = {rr};
             double o[1];
-            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            launch_sum<1, Grid>(v, slots + phase * 3 * gs.groups(), lds, barrier, o, gs);
             phase ^= 1;
             red[0] = o[0];
         }
@@ -589,7 +618,7 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
             }
             const double v[1] = {ee};
             double o[1];
-            launch_sum<1, Grid>(v, slots + phase * 3 * gridDim.x, lds, barrier, o);
+            launch_sum<1, Grid>(v, slots + phase * 3 * gs.groups(), lds, barrier, o, gs);
             phase ^= 1;
             converged = tnorm <= kResidualGamma * sqrt(o[0]);
             break;
@@ -603,12 +632,83 @@ pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols
         if (!own[k]) continue;
         solution[6 * node[k] + lane] = x64[k] + static_cast<double>(x[k]);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (gs.group() == 0 && threadIdx.x == 0) {
         outcome[0] = it;
         outcome[1] = rel;
         outcome[2] = converged ? 1.0 : 0.0;
         outcome[3] = breakdown ? 1.0 : 0.0;
     }
+}
+
+template <int K, typename S, bool Grid>
+__global__ void __launch_bounds__(kPcgThreads)
+pcg_kernel(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
+           const S* __restrict__ damp, const S* __restrict__ minv, const double* __restrict__ bsr64,
+           const double* __restrict__ damp64, const double* __restrict__ rhs, int64_t nn,
+           double tol, double inner_tol, int max_iter, S* __restrict__ vbuf /* 2 x 6N */,
+           double* __restrict__ xbuf /* 6N */, double* __restrict__ slots /* 2 x 3 x groups */,
+           unsigned* __restrict__ barrier, double* __restrict__ solution, double* __restrict__ outcome) {
+    __shared__ double lds[kPcgLdsDoubles];
+    __shared__ float4 matrix_lds[kMatrixLdsBytes / 16];
+    pcg_body<K, S, Grid>(row_off, cols, bsr, damp, minv, bsr64, damp64, rhs, nn, tol, inner_tol, max_iter, vbuf, xbuf,
+                         slots, barrier, solution, outcome, LaunchGroups{}, lds, matrix_lds);
+}
+
+// ---------------------------------------------------------------- packed ---
+// The solves of a batch of jobs (packed.hpp) in one launch. Each job has its
+// own preconditioner, damping (its control record's), barrier counter and
+// partial-sum slots, and its groups meet only each other. Groups take their
+// job and group index by ticket, in the order they start: a job's groups are
+// consecutive tickets, so every job whose tickets have all been taken has all
+// its groups running and finishes, and the one job still taking tickets
+// waits only for groups that start as others finish. With no job holding
+// more groups than the device runs at once, every wait ends, whatever the
+// launch's total.
+
+template <typename S, bool Mixed>
+__global__ void __launch_bounds__(128)
+precondition_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+                    const PackedBlock* __restrict__ map, unsigned* __restrict__ ticket) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
+    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    const BatchControl& c = ctl[m.job];
+    if (!(c.flags & kJobActive)) return;
+    const PackedJob& J = jobs[m.job];
+    precondition_node<S>(static_cast<const S*>(J.bsr), Mixed ? J.bsr64 : nullptr, J.diag_position, J.in.node_count,
+                         c.damping, c.scaling, static_cast<S*>(J.damp), Mixed ? J.damp64 : nullptr,
+                         static_cast<S*>(J.minv), J.barrier, static_cast<int64_t>(m.block) * 128 + threadIdx.x);
+}
+
+template <int K, typename S, bool Mixed>
+__global__ void __launch_bounds__(kPcgThreads)
+pcg_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict__ ctl,
+           const PackedBlock* __restrict__ tickets, unsigned* __restrict__ ticket) {
+    __shared__ double lds[kPcgLdsDoubles];
+    __shared__ float4 matrix_lds[kMatrixLdsBytes / 16];
+    __shared__ int32_t job_lds;
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const PackedBlock m = tickets[t];
+        job_lds = m.job;
+        packed_group_set[0] = static_cast<unsigned>(m.block);
+        packed_group_set[1] = static_cast<unsigned>(jobs[m.job].pcg_groups);
+    }
+    __syncthreads();
+    const int32_t job = job_lds;
+    if (!(ctl[job].flags & kJobActive)) return;
+    // The descriptor read through the constant address space: its loads are
+    // invariant, so the compiler reloads a pointer where it is used instead
+    // of holding every one in registers for the whole solve.
+    ConstJob* J = constant_job(jobs, job);
+    const S* bsr = static_cast<const S*>(J->bsr);
+    const double* bsr64 = Mixed ? J->bsr64 : nullptr;
+    const double* damp64 = Mixed ? J->damp64 : nullptr;
+    // One form of the body for every job (a job of one group meets itself at
+    // its barriers): two inlined forms would not fit the registers.
+    pcg_body<K, S, true>(J->row_offsets, J->columns, bsr, static_cast<const S*>(J->damp),
+                         static_cast<const S*>(J->minv), bsr64, damp64, J->rhs, J->in.node_count, J->tolerance,
+                         J->inner_tolerance, J->max_iterations, static_cast<S*>(J->vbuf), J->xbuf, J->slots,
+                         J->barrier, J->delta, J->outcome, PackedGroups{}, lds, matrix_lds);
 }
 
 // Node slots per lane group K: at most 2 (register budget: every CG vector
@@ -725,6 +825,48 @@ void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64
     ws.minv.resize(36 * nn);
     launch_pcg<double>(L, bsr, nullptr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
                        nullptr, ws.minv.get(), ws.u, stream);
+}
+
+int64_t packed_pcg_groups(int64_t nodes, int k) {
+    const int64_t per_group = static_cast<int64_t>(k) * kNodesPerPass;
+    return std::max<int64_t>(1, (nodes + per_group - 1) / per_group);
+}
+
+void packed_precondition(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
+                         unsigned* ticket, hipStream_t stream) {
+    if (maps.node_blocks <= 0) return;
+    const unsigned blocks = static_cast<unsigned>(maps.node_blocks);
+    if (precision == NRICP_HIP_PRECISION_DOUBLE)
+        precondition_packed<double, false><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
+    else if (precision == NRICP_HIP_PRECISION_MIXED)
+        precondition_packed<float, true><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
+    else
+        precondition_packed<float, false><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
+    NRICP_LAUNCH_CHECK("precondition_packed");
+}
+
+void packed_pcg(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision, int k,
+                unsigned* ticket, hipStream_t stream) {
+    if (maps.pcg_groups <= 0) return;
+    const unsigned groups = static_cast<unsigned>(maps.pcg_groups);
+    auto go = [&](auto kernel) {
+        kernel<<<groups, kPcgThreads, 0, stream>>>(jobs, ctl, maps.pcg, ticket);
+        NRICP_LAUNCH_CHECK("pcg_packed");
+    };
+    if (precision == NRICP_HIP_PRECISION_DOUBLE) {
+        if (k == 1) go(pcg_packed<1, double, false>); else go(pcg_packed<2, double, false>);
+    } else if (precision == NRICP_HIP_PRECISION_MIXED) {
+        if (k == 1) go(pcg_packed<1, float, true>); else go(pcg_packed<2, float, true>);
+    } else {
+        if (k == 1) go(pcg_packed<1, float, false>); else go(pcg_packed<2, float, false>);
+    }
+}
+
+// The most groups one job may hold in a packed launch (the groups the device
+// keeps resident at once; packed_pcg's tickets rely on it).
+int64_t packed_pcg_group_limit() {
+    return std::min(persistent_group_budget(reinterpret_cast<const void*>(pcg_packed<2, double, false>), kPcgThreads),
+                    kPcgThreads);
 }
 
 int64_t pcg_max_nodes() {

@@ -453,10 +453,31 @@ int persistent_group_budget(const void* kernel, int threads) {
     return std::max(1, cus / 2);
 }
 
+namespace {
+struct KernelLaunch {
+    const void* kernel;
+    int groups, threads;
+    void** args;
+    hipStream_t stream;
+    void (*prelude)(void*);
+    void* prelude_arg;
+    static hipError_t run(void* p) {
+        const KernelLaunch& k = *static_cast<const KernelLaunch*>(p);
+        if (k.prelude) k.prelude(k.prelude_arg);
+        return hipLaunchKernel(k.kernel, dim3(k.groups), dim3(k.threads), k.args, 0, k.stream);
+    }
+};
+}  // namespace
+
 void launch_persistent(const void* kernel, int groups, int threads, void** args, hipStream_t stream,
                        void (*prelude)(void*), void* prelude_arg) {
     if (groups < 1 || groups > persistent_group_budget(kernel, threads))
         throw InvalidArgument("persistent launch of " + std::to_string(groups) + " groups exceeds the residency budget");
+    KernelLaunch k{kernel, groups, threads, args, stream, prelude, prelude_arg};
+    admit_persistent(groups, stream, &KernelLaunch::run, &k);
+}
+
+void admit_persistent(int groups, hipStream_t stream, hipError_t (*enqueue)(void*), void* arg) {
     int device = 0;
     const DeviceShape shape = device_shape(&device);
     const int cus = shape.cus, xcds = shape.xcds;
@@ -480,17 +501,15 @@ void launch_persistent(const void* kernel, int groups, int threads, void** args,
         lock.lock();
     }
     const hipEvent_t done = take_event(r, device);
-    if (prelude) {
-        try {
-            prelude(prelude_arg);
-        } catch (...) {
-            r.spare.push_back(Spare{device, done, stream});
-            throw;
-        }
-    }
-    // Launch and record under the lock, so the ledger never holds a charge
+    // Enqueue and record under the lock, so the ledger never holds a charge
     // without its event.
-    hipError_t e = hipLaunchKernel(kernel, dim3(groups), dim3(threads), args, 0, stream);
+    hipError_t e = hipSuccess;
+    try {
+        e = enqueue(arg);
+    } catch (...) {
+        r.spare.push_back(Spare{device, done, stream});
+        throw;
+    }
     if (e == hipSuccess) e = hipEventRecord(done, stream);
     if (e != hipSuccess) {
         r.spare.push_back(Spare{device, done, stream});

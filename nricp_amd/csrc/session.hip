@@ -71,13 +71,6 @@ double relative_change(double before, double after) {
     return (before - after) / std::max(std::abs(before), 1e-12);
 }
 
-int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level) {
-    const double g = std::pow(static_cast<double>(std::max(1, growth)), static_cast<double>(level));
-    const int64_t req = std::max<int64_t>(1, static_cast<int64_t>(std::lround(n0 * g)));
-    (void)levels;
-    return std::min<int64_t>(req, nv);
-}
-
 std::string number(double v) {
     if (!std::isfinite(v)) return "null";
     std::ostringstream o;
@@ -100,6 +93,13 @@ std::string escape(const std::string& t) {
 }
 
 }  // namespace
+
+int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level) {
+    const double g = std::pow(static_cast<double>(std::max(1, growth)), static_cast<double>(level));
+    const int64_t req = std::max<int64_t>(1, static_cast<int64_t>(std::lround(n0 * g)));
+    (void)levels;
+    return std::min<int64_t>(req, nv);
+}
 
 // ------------------------------------------------------------------ graph --
 

@@ -48,6 +48,9 @@ void build_graph_level(const double* points, int64_t vertex_count, const int32_t
                        int max_nodes_per_vertex, int neighbours, GraphLevel& level, SortScratch& scratch,
                        hipStream_t stream);
 
+// Nodes of level `level` of a graph of `levels` levels: min(round(n0 g^l), V).
+int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level);
+
 // Throws InvalidArgument when the finest level of the job's graph (the given
 // one, or the one its settings ask for) has more nodes than one solve takes on
 // this device (pcg_max_nodes), so the job fails before any work, not midway.
@@ -116,7 +119,11 @@ enum Counter : int {
     kCounterCount
 };
 
+class BatchEngine;
+
 class Session {
+    friend class BatchEngine;  // drives the session's levels through packed launches (batch.hip)
+
 public:
     Session(const nricp_hip_problem& problem, const nricp_hip_settings& settings, const DeviceGraph* graph);
     ~Session();
