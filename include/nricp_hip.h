@@ -413,8 +413,8 @@ int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats*
  * job still in the level runs as one packed launch per stage -- evaluation,
  * normals, closest points, assembly, preconditioner, one persistent PCG
  * launch over every job's block rows with per-job convergence, retraction,
- * energies, and the accept/reject decision taken on the device -- captured as
- * a graph and replayed. A job's result is the same bits in any batch.
+ * energies, and the accept/reject decision taken on the device -- built into
+ * a graph once per level and replayed. A job's result is the same bits in any batch.
  * `pcg_slots` (1 or 2) is the packed PCG's node slots per lane group (its
  * tile shape); `use_graphs` 0 launches the stages directly. */
 typedef struct nricp_hip_batch nricp_hip_batch;
@@ -429,8 +429,9 @@ int nricp_hip_batch_run(nricp_hip_batch* batch);
 /* Job `index` as a session handle owned by the batch (results: vertices,
  * report, state, level nodes, counters; not to be destroyed or stepped). */
 int nricp_hip_batch_job(nricp_hip_batch* batch, int32_t index, nricp_hip_session** out_session);
-/* [ticks, graph replays, graphs captured, refresh ticks, assembly ticks,
- * seconds, level set-up seconds, tick-loop seconds]. */
+/* [ticks, graph replays, graphs built, refresh ticks, assembly ticks,
+ * seconds, level set-up seconds, tick-loop seconds, first level's set-up
+ * seconds, level-end seconds, final-statistics seconds]. */
 int nricp_hip_batch_stats(const nricp_hip_batch* batch, double* out, int32_t capacity, int32_t* out_count);
 void nricp_hip_batch_destroy(nricp_hip_batch* batch);
 

@@ -1,22 +1,29 @@
 """Batched uneven instances: a batch of wrap jobs of different sizes run on
-one MI355X, with the number of jobs on the device at once chosen by a
-measured sweep.
+one MI355X.
 
 Jobs (as the 64-job batch configuration): humanoid templates of 2k-50k
 vertices (log-uniform), targets with 2-3x the template's triangles, nodes ~
 vertices / 25, 4 levels x 10 Gauss-Newton steps, FP32 iterations with FP64
-refinement, every solve to a 1e-8 relative residual. The batch runs through nricp_dp.Server, so each job is its own
-resident session on its own streams and the jobs' kernels share the device:
-while one job waits on a grid barrier or a host decision, the others' work
-fills the compute units. The process asks HIP for 16 hardware queues (4 by
-default), so the jobs' streams do not serialise on shared queues.
+refinement, every solve to a 1e-8 relative residual.
 
-Tuning: the whole batch is run `--repeats` times at each candidate slot
-count (a run takes about a second; a subset of the jobs tunes poorly, since
-the mix of sizes decides how many jobs overlap well). The count with the best
-median jobs/s then runs the batch once more for the reported figure. The
-whole batch is also run one job at a time (the serial baseline), and a sample
-of jobs is checked to give the same fitted vertices alone and in the batch.
+Two ways of running the batch are measured:
+
+* packed (nricp_dp.Batch): the jobs in lockstep, one shared launch per stage
+  of a damping attempt for every job, one persistent PCG over all jobs'
+  block rows, the decisions on the device, each level's ticks replayed from
+  graphs built once per level. The autotuner (`tune_packed`) measures the candidate
+  shapes -- the packed PCG's node slots per lane group (its tile: 64 or 128
+  nodes per workgroup) and the jobs per packed batch (the launch's width:
+  the whole batch, or chunks of it run one after another) -- on the batch
+  itself and keeps the fastest;
+* concurrent sessions (nricp_dp.Server): each job its own resident session
+  on its own streams, `--session-slots` at a time, their kernels sharing the
+  device; the process asks HIP for 16 hardware queues (4 by default), so the
+  jobs' streams do not serialise on shared queues.
+
+The whole batch is also run one job at a time (the serial baseline), and a
+sample of jobs is checked to give the same fitted vertices alone and in the
+packed batch (and, for concurrent sessions, alone and in the server).
 
 Run from the repository root:  python -m nricp_amd.batch
 Prints one JSON line.
@@ -71,6 +78,60 @@ def job_args(job):
             dict(KWARGS, graph_levels=job.graph_levels, nodes_initial=job.nodes_initial))
 
 
+def _packed_chunk(dp, jobs, form: str, pcg_slots: int, graphs: bool) -> tuple[list, dict]:
+    batch = dp.Batch(pcg_slots=pcg_slots, use_graphs=graphs)
+    totals = {}
+    t_add = time.perf_counter()
+    for job in jobs:
+        a, kw = job_args(job)
+        batch.add(*a, precision=form, **kw)
+    totals["add_seconds"] = time.perf_counter() - t_add
+    results = batch.run()  # releases the GIL while the device works
+    totals.update(batch.stats())
+    t_close = time.perf_counter()
+    batch.close()
+    totals["close_seconds"] = time.perf_counter() - t_close
+    return results, totals
+
+
+def run_packed(dp, jobs, form: str, pcg_slots: int, chunk: int, concurrent: int = 1,
+               graphs: bool = True) -> tuple[float, list, dict]:
+    """The jobs through packed batches of `chunk` jobs each, `concurrent`
+    batches at a time on host threads (one batch's level set-up then overlaps
+    another's ticks); returns the wall time, the results in job order and the
+    summed engine statistics."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    chunks = [jobs[s:s + max(1, chunk)] for s in range(0, len(jobs), max(1, chunk))]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=max(1, concurrent)) as pool:
+        outs = list(pool.map(lambda c: _packed_chunk(dp, c, form, pcg_slots, graphs), chunks))
+    wall = time.perf_counter() - t0
+    results, totals = [], {}
+    for res, tot in outs:
+        results.extend(res)
+        for k, v in tot.items():
+            totals[k] = totals.get(k, 0.0) + v
+    return wall, results, totals
+
+
+def _key(c) -> str:
+    return f"pcg_slots={c[0]},chunk={c[1]},concurrent={c[2]}"
+
+
+def tune_packed(dp, jobs, form: str, candidates, repeats: int = 1) -> tuple[tuple[int, int, int], dict]:
+    """The autotuner: each candidate shape (pcg_slots, jobs per batch,
+    batches at once) runs the whole batch `repeats` times; returns the
+    fastest and every candidate's median jobs/s."""
+    table = {}
+    for c in candidates:
+        rates = [len(jobs) / run_packed(dp, jobs, form, c[0], c[1], c[2])[0] for _ in range(max(1, repeats))]
+        table[_key(c)] = round(float(np.median(rates)), 3)
+        log(f"packed: {_key(c)} -> {table[_key(c)]} jobs/s")
+    best = max(candidates, key=lambda c: table[_key(c)])
+    return best, table
+
+
 def run_batch(dp, jobs, slots: int, form: str) -> tuple[float, list]:
     """All jobs submitted at once to a server with `slots` slots; returns the
     wall time and the results in job order."""
@@ -88,11 +149,15 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1000)
-    ap.add_argument("--form", default="mixed")
-    ap.add_argument("--candidates", default="1,2,4,6,8,12,16", help="slot counts the tuner measures")
-    ap.add_argument("--repeats", type=int, default=2, help="runs of the whole batch per candidate")
+    ap.add_argument("--form", default="float")
+    ap.add_argument("--candidates", default="1:64:1,2:64:1,1:32:2,2:32:2,1:16:4,2:16:4",
+                    help="packed shapes the autotuner measures, pcg_slots:jobs_per_batch:batches_at_once")
+    ap.add_argument("--repeats", type=int, default=1, help="runs of the whole batch per candidate")
+    ap.add_argument("--session-slots", type=int, default=12,
+                    help="jobs on the device at once for the concurrent-session comparison")
     ap.add_argument("--check", type=int, default=4, help="jobs compared alone vs in the batch")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--packed-only", action="store_true", help="measure the packed batch only (profiling runs)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="hardware queues HIP gives the process (GPU_MAX_HW_QUEUES; unless already set): with the "
                          "default 4, the jobs' streams share queues and their kernels serialise")
@@ -115,41 +180,54 @@ def main(argv=None) -> int:
         return 1
     # Warm-up: code objects loaded, allocator cache filled.
     run_batch(dp, jobs[:8], 4, args.form)
+    run_packed(dp, jobs[:8], args.form, 2, 8)
 
-    tuning = {}
-    for slots in [int(c) for c in args.candidates.split(",")]:
-        rates = [len(jobs) / run_batch(dp, jobs, slots, args.form)[0] for _ in range(max(1, args.repeats))]
-        tuning[slots] = round(float(np.median(rates)), 3)
-        log(f"tuning: {slots} slots -> {tuning[slots]} jobs/s (median of {len(rates)})")
-    best = max(tuning, key=lambda k: tuning[k])
+    candidates = [tuple(int(x) for x in c.split(":")) for c in args.candidates.split(",")]
+    (pcg_slots, chunk, concurrent), tuning = tune_packed(dp, jobs, args.form, candidates, args.repeats)
+    wall, packed, engine = run_packed(dp, jobs, args.form, pcg_slots, chunk, concurrent)
+    log(f"packed: {len(jobs) / wall:.1f} jobs/s ({_key((pcg_slots, chunk, concurrent))})")
+    alone = [run_packed(dp, [jobs[k]], args.form, pcg_slots, 1)[1][0] for k in range(min(args.check, len(jobs)))]
+    same_packed = all(np.array_equal(a["vertices"], packed[k]["vertices"]) for k, a in enumerate(alone))
+    log("packed phases (s): " + ", ".join(f"{k} {v:.4f}" for k, v in engine.items()))
+    if args.packed_only:
+        print(json.dumps({"packed_jobs_per_s": round(len(jobs) / wall, 3), "engine": engine,
+                          "tuning_jobs_per_s": tuning, "same_bits_alone_and_in_batch": bool(same_packed)}), flush=True)
+        return 0 if same_packed else 3
 
-    wall, results = run_batch(dp, jobs, best, args.form)
+    session_wall, sessions = run_batch(dp, jobs, args.session_slots, args.form)
     serial_wall, serial = run_batch(dp, jobs, 1, args.form)
-    latency = np.array([r["queued_seconds"] + r["run_seconds"] for r in results]) * 1e3
+    log(f"sessions: {len(jobs) / session_wall:.1f} jobs/s at {args.session_slots} slots, "
+        f"{len(jobs) / serial_wall:.1f} jobs/s one at a time")
+    same_sessions = all(np.array_equal(sessions[k]["vertices"], serial[k]["vertices"])
+                        for k in range(min(args.check, len(jobs))))
     run_ms = np.array([r["run_seconds"] for r in serial]) * 1e3
-    same = all(np.array_equal(results[k]["vertices"], serial[k]["vertices"]) for k in range(min(args.check, len(jobs))))
+    diags = [float(np.linalg.norm(j.template_points.max(0) - j.template_points.min(0))) for j in jobs]
+    packed_vs_session = max(float(np.sqrt(np.mean(np.sum((p["vertices"] - s["vertices"]) ** 2, axis=1)))) / d
+                            for p, s, d in zip(packed, serial, diags))
     line = {
         "metric": "batched uneven instances: jobs/s on one MI355X (whole batch)",
         "workload": f"{len(jobs)} humanoid jobs, {min(sizes)}-{max(sizes)} template vertices, target 2-3x the "
                     f"template's triangles, nodes ~ V/25, 4 levels x {OUTER} GN steps, seed {args.seed}",
         "form": args.form,
-        "slots": best,
         "jobs_per_s": round(len(jobs) / wall, 3),
-        "batch_seconds": round(wall, 3),
+        "packed": {"pcg_slots": pcg_slots, "jobs_per_batch": chunk, "batches_at_once": concurrent,
+                   "jobs_per_s": round(len(jobs) / wall, 3),
+                   "batch_seconds": round(wall, 3), "gn_solves_per_s": round(len(jobs) * 4 * OUTER / wall, 1),
+                   "engine": {k: round(v, 4) for k, v in engine.items()}, "tuning_jobs_per_s": tuning,
+                   "same_bits_alone_and_in_batch": bool(same_packed),
+                   "vertex_rms_vs_single_job_sessions_max": packed_vs_session},
+        "sessions": {"slots": args.session_slots, "jobs_per_s": round(len(jobs) / session_wall, 3),
+                     "same_bits_alone_and_in_server": bool(same_sessions)},
         "serial_jobs_per_s": round(len(jobs) / serial_wall, 3),
-        "speedup_over_serial": round(serial_wall / wall, 3),
-        "gn_solves_per_s": round(len(jobs) * 4 * OUTER / wall, 1),
-        "latency_p50_ms": round(float(np.percentile(latency, 50)), 2),
-        "latency_p99_ms": round(float(np.percentile(latency, 99)), 2),
+        "packed_over_sessions": round(session_wall / wall, 3),
+        "packed_over_serial": round(serial_wall / wall, 3),
         "alone_run_ms_p50": round(float(np.percentile(run_ms, 50)), 2),
-        "tuning_jobs_per_s": tuning,
-        "same_bits_alone_and_in_batch": bool(same),
     }
     text = json.dumps(line)
     print(text, flush=True)
     if args.json_out:
         Path(args.json_out).write_text(text + "\n")
-    return 0 if same else 3
+    return 0 if same_packed and same_sessions else 3
 
 
 if __name__ == "__main__":

@@ -773,40 +773,36 @@ void translation_rhs(const AssemblyInputs& in, const LevelLayout& L, AssemblyScr
 }
 
 void packed_rows(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, bool records,
-                 hipStream_t stream) {
+                 KernelSink& sink) {
     if (maps.rows_blocks <= 0) return;
     const unsigned blocks = static_cast<unsigned>(maps.rows_blocks);
     if (records)
-        rows_packed<true><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.rows);
+        sink_launch(sink, rows_packed<true>, blocks, kThreads, jobs, ctl, maps.rows);
     else
-        rows_packed<false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.rows);
-    NRICP_LAUNCH_CHECK("rows_packed");
+        sink_launch(sink, rows_packed<false>, blocks, kThreads, jobs, ctl, maps.rows);
 }
 
 void packed_system(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
-                   hipStream_t stream) {
+                   KernelSink& sink) {
     if (maps.system_blocks <= 0) return;
     const unsigned blocks = static_cast<unsigned>(maps.system_blocks);
     if (precision == NRICP_HIP_PRECISION_FLOAT)
-        system_packed<float, false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
+        sink_launch(sink, system_packed<float, false>, blocks, kThreads, jobs, ctl, maps.system);
     else if (precision == NRICP_HIP_PRECISION_MIXED)
-        system_packed<double, true><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
+        sink_launch(sink, system_packed<double, true>, blocks, kThreads, jobs, ctl, maps.system);
     else
-        system_packed<double, false><<<blocks, kThreads, 0, stream>>>(jobs, ctl, maps.system);
-    NRICP_LAUNCH_CHECK("system_packed");
+        sink_launch(sink, system_packed<double, false>, blocks, kThreads, jobs, ctl, maps.system);
 }
 
-void packed_reduce_current(const PackedJob* jobs, const BatchControl* ctl, int32_t job_count, hipStream_t stream) {
+void packed_reduce_current(const PackedJob* jobs, const BatchControl* ctl, int32_t job_count, KernelSink& sink) {
     if (job_count <= 0) return;
-    reduce_current_packed<<<static_cast<unsigned>(job_count), kThreads, 0, stream>>>(jobs, ctl);
-    NRICP_LAUNCH_CHECK("reduce_current_packed");
+    sink_launch(sink, reduce_current_packed, static_cast<unsigned>(job_count), kThreads, jobs, ctl);
 }
 
 void packed_decide(const PackedJob* jobs, BatchControl* ctl, int32_t job_count, int32_t* host_flags,
-                   hipStream_t stream) {
+                   KernelSink& sink) {
     if (job_count <= 0) return;
-    decide_packed<<<static_cast<unsigned>(job_count), kThreads, 0, stream>>>(jobs, ctl, host_flags);
-    NRICP_LAUNCH_CHECK("decide_packed");
+    sink_launch(sink, decide_packed, static_cast<unsigned>(job_count), kThreads, jobs, ctl, host_flags);
 }
 
 template void assemble<double>(const AssemblyInputs&, const LevelLayout&, AssemblyScratch&, double*, double*, double*,

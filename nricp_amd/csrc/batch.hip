@@ -1,6 +1,5 @@
 // The batch engine of batch.hpp: jobs in lockstep, one packed launch per
-// stage of a damping attempt, each level's ticks replayed from captured
-// graphs.
+// stage of a damping attempt, each level's ticks replayed from graphs.
 //
 // Reference behaviour: every job follows the single-job session's schedule
 // (route A semantics for routes A and B, src/solver/route_a.cpp:67-209), so a
@@ -11,8 +10,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 
 #include "batch.hpp"
 
@@ -29,6 +32,38 @@ void append(std::vector<PackedBlock>& map, int32_t slot, int64_t blocks) {
 }
 
 }  // namespace
+
+void parallel_jobs(int count, int device, const std::function<void(int)>& f) {
+    if (count <= 0) return;
+    const int workers = std::min(count, kBatchHostThreads);
+    std::atomic<int> next{0};
+    std::mutex mutex;
+    std::exception_ptr error;
+    auto work = [&] {
+        try {
+            NRICP_CHECK(hipSetDevice(device));
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(mutex);
+            if (!error) error = std::current_exception();
+            return;
+        }
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mutex);
+                if (!error) error = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
+}
 
 BatchEngine::BatchEngine(std::vector<Session*> jobs, const BatchConfig& config)
     : jobs_(std::move(jobs)), config_(config), host_flags_(static_cast<int64_t>(std::max<size_t>(jobs_.size(), 1))) {
@@ -51,6 +86,7 @@ BatchEngine::BatchEngine(std::vector<Session*> jobs, const BatchConfig& config)
             throw InvalidArgument("a batch job's finest level has " + std::to_string(finest) +
                                   " nodes, more than one packed solve takes on this device");
     }
+    NRICP_CHECK(hipGetDevice(&device_));
     NRICP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_flags_device_), host_flags_.get(), 0));
     NRICP_CHECK(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
     NRICP_CHECK(hipEventRecord(done_, stream_.get()));
@@ -100,10 +136,14 @@ void BatchEngine::begin_level(int level) {
     const int k = config_.pcg_slots;
     std::vector<PackedJob> packed(slots_.size());
     control_host_.assign(slots_.size(), BatchControl{});
-    std::vector<PackedBlock> vertex, query, rows, system, node, pcg;
-    int query_stack = 64;
-    charge_groups_ = 1;
-    for (size_t i = 0; i < slots_.size(); ++i) {
+    struct Sizes {
+        int64_t nv, nn, vblocks, gblocks, groups;
+        int32_t system_blocks, query_stack;
+    };
+    std::vector<Sizes> sizes(slots_.size());
+    // Each job's set-up is a chain of small launches and readbacks on its
+    // own streams: the jobs run theirs side by side on host threads.
+    parallel_jobs(static_cast<int>(slots_.size()), device_, [&](int i) {
         Session& s = *jobs_[slots_[i]];
         hipStream_t st = s.stream_.get();
         StreamScope scope(st);
@@ -243,15 +283,23 @@ void BatchEngine::begin_level(int level) {
         c.inner_tolerance = ls.inner_tolerance;
         c.outer_tolerance = ls.outer_tolerance;
 
+        sizes[i] = Sizes{nv, nn, vblocks, gblocks, groups, J.node_blocks + J.pair_blocks,
+                         packed_query_stack(s.bvh_.depth_max)};
+    });
+    std::vector<PackedBlock> vertex, query, rows, system, node, pcg;
+    int query_stack = 64;
+    charge_groups_ = 1;
+    for (size_t i = 0; i < slots_.size(); ++i) {
+        const Sizes& z = sizes[i];
         const int32_t slot = static_cast<int32_t>(i);
-        append(vertex, slot, blocks_for(nv, 256));
-        append(query, slot, blocks_for(nv * 8, 64));
-        append(rows, slot, vblocks + gblocks);
-        append(system, slot, J.node_blocks + J.pair_blocks);
-        append(node, slot, blocks_for(nn, 128));
-        append(pcg, slot, groups);
-        query_stack = std::max(query_stack, packed_query_stack(s.bvh_.depth_max));
-        charge_groups_ = std::max(charge_groups_, static_cast<int>(groups));
+        append(vertex, slot, blocks_for(z.nv, 256));
+        append(query, slot, blocks_for(z.nv * 8, 64));
+        append(rows, slot, z.vblocks + z.gblocks);
+        append(system, slot, z.system_blocks);
+        append(node, slot, blocks_for(z.nn, 128));
+        append(pcg, slot, z.groups);
+        query_stack = std::max(query_stack, z.query_stack);
+        charge_groups_ = std::max(charge_groups_, static_cast<int>(z.groups));
     }
     for (int j : slots_) NRICP_CHECK(hipStreamWaitEvent(es, ready_[j], 0));
     // The tables, in one block array: the stage maps back to back.
@@ -292,52 +340,52 @@ void BatchEngine::begin_level(int level) {
     NRICP_CHECK(hipStreamSynchronize(es));
     drop_graphs();
     stats_.level_setup_seconds += seconds_since(t0);
+    if (level == 0) stats_.first_setup_seconds += seconds_since(t0);
 }
 
 // One tick's launches, in stage order. Refresh: new correspondences of the
 // jobs that start an outer iteration; assembly: the systems of the jobs that
 // refreshed or accepted a step; then every active job's solve, candidate,
 // energies and decision, and the accepted candidates become the states.
-void BatchEngine::enqueue_tick(TickForm form) {
-    hipStream_t es = stream_.get();
+void BatchEngine::enqueue_tick(TickForm form, KernelSink& sink) {
     const PackedJob* jobs = jobs_dev_.get();
     BatchControl* ctl = control_.get();
     const int32_t count = maps_.jobs;
     if (form == kTickFull) {
-        packed_evaluate(jobs, ctl, maps_, es);
-        packed_normals(jobs, ctl, maps_, es);
-        packed_query(jobs, ctl, maps_, es);
+        packed_evaluate(jobs, ctl, maps_, sink);
+        packed_normals(jobs, ctl, maps_, sink);
+        packed_query(jobs, ctl, maps_, sink);
     }
     if (form >= kTickAssemble) {
-        packed_rows(jobs, ctl, maps_, true, es);
-        packed_system(jobs, ctl, maps_, precision_, es);
-        packed_reduce_current(jobs, ctl, count, es);
+        packed_rows(jobs, ctl, maps_, true, sink);
+        packed_system(jobs, ctl, maps_, precision_, sink);
+        packed_reduce_current(jobs, ctl, count, sink);
     }
-    packed_precondition(jobs, ctl, maps_, precision_, ticket_.get(), es);
-    packed_pcg(jobs, ctl, maps_, precision_, config_.pcg_slots, ticket_.get(), es);
-    packed_retract(jobs, ctl, maps_, es);
-    packed_rows(jobs, ctl, maps_, false, es);
-    packed_decide(jobs, ctl, count, host_flags_device_, es);
-    packed_commit(jobs, ctl, maps_, es);
+    packed_precondition(jobs, ctl, maps_, precision_, ticket_.get(), sink);
+    packed_pcg(jobs, ctl, maps_, precision_, config_.pcg_slots, ticket_.get(), sink);
+    packed_retract(jobs, ctl, maps_, sink);
+    packed_rows(jobs, ctl, maps_, false, sink);
+    packed_decide(jobs, ctl, count, host_flags_device_, sink);
+    packed_commit(jobs, ctl, maps_, sink);
 }
 
 // A tick on the stream, admitted into the device's persistent-launch ledger
 // (the packed PCG keeps at most charge_groups_ groups of one job waiting on
-// each other). With graphs, the tick form is captured on first use in the
-// level and replayed after.
+// each other). With graphs, the tick form's graph is built on first use in
+// the level (node by node: a stream capture would be open to what other
+// threads do meanwhile) and replayed after.
 void BatchEngine::launch_tick(TickForm form) {
     hipStream_t es = stream_.get();
     if (config_.graphs && !exec_[form]) {
         hipGraph_t graph = nullptr;
-        NRICP_CHECK(hipStreamBeginCapture(es, hipStreamCaptureModeThreadLocal));
+        NRICP_CHECK(hipGraphCreate(&graph, 0));
+        KernelSink sink{es, graph, nullptr};
         try {
-            enqueue_tick(form);
+            enqueue_tick(form, sink);
         } catch (...) {
-            (void)hipStreamEndCapture(es, &graph);
-            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             throw;
         }
-        NRICP_CHECK(hipStreamEndCapture(es, &graph));
         hipGraphExec_t exec = nullptr;
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -351,7 +399,8 @@ void BatchEngine::launch_tick(TickForm form) {
         static hipError_t run(void* p) {
             const Enqueue& q = *static_cast<const Enqueue*>(p);
             if (q.self->config_.graphs) return hipGraphLaunch(q.self->exec_[q.form], q.self->stream_.get());
-            q.self->enqueue_tick(q.form);
+            KernelSink sink{q.self->stream_.get(), nullptr, nullptr};
+            q.self->enqueue_tick(q.form, sink);
             return hipSuccess;
         }
     } q{this, form};
@@ -380,23 +429,24 @@ void BatchEngine::run_level(int) {
 // correspondence mean on its own stream, the schedule's counts and energies
 // from its control record.
 void BatchEngine::end_level(int level) {
+    const auto t0 = std::chrono::steady_clock::now();
     hipStream_t es = stream_.get();
     NRICP_CHECK(hipEventRecord(done_, es));
     control_host_.resize(slots_.size());
     control_.download(control_host_.data(), static_cast<int64_t>(slots_.size()), es);
-    for (int j : slots_) {
-        Session& s = *jobs_[j];
+    parallel_jobs(static_cast<int>(slots_.size()), device_, [&](int i) {
+        Session& s = *jobs_[slots_[i]];
         hipStream_t st = s.stream_.get();
         StreamScope scope(st);
         NRICP_CHECK(hipStreamWaitEvent(st, done_, 0));
         summarize_correspondences(s.rejection_.get(), s.corr_dist_.get(), s.nv_, s.summary_partials_, s.summary_.get(),
                                   st);
         s.to_host(st, 16, s.summary_.get(), 6);
-    }
+        s.sync();
+    });
     NRICP_CHECK(hipStreamSynchronize(es));
     for (size_t i = 0; i < slots_.size(); ++i) {
         Session& s = *jobs_[slots_[i]];
-        s.sync();
         const BatchControl& c = control_host_[i];
         const double accepted = s.staging_[16];
         s.corr_mean_ = accepted > 0.0 ? s.staging_[21] / accepted : 0.0;
@@ -427,51 +477,24 @@ void BatchEngine::end_level(int level) {
         s.counters_[kQueryRows] += static_cast<double>(c.view_outer) * static_cast<double>(s.nv_);
     }
     drop_graphs();
+    stats_.end_seconds += seconds_since(t0);
 }
 
 // The final correspondence pass and statistics of every job (the single-job
-// session's final_statistics), overlapped across the jobs' streams.
+// session's final_statistics), the jobs side by side on host threads.
 void BatchEngine::finish() {
-    for (Session* sp : jobs_) {
-        Session& s = *sp;
+    const auto t0 = std::chrono::steady_clock::now();
+    parallel_jobs(static_cast<int>(jobs_.size()), device_, [&](int j) {
+        Session& s = *jobs_[j];
         hipStream_t st = s.stream_.get();
         StreamScope scope(st);
         NRICP_CHECK(hipStreamWaitEvent(st, done_, 0));
         s.level_ = s.graph_->levels.size() - 1;
-        const GraphLevel& L = s.graph_->levels[s.level_];
-        evaluate_deformation(s.points_.get(), L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), s.nv_,
-                             s.rot_.get(), s.trans_.get(), L.node_positions.get(), L.node_count, s.deformed_.get(), st);
-        vertex_normals(s.deformed_.get(), s.faces_.get(), s.template_vf_, s.nv_, s.qnormals_.get(), st);
-        QuerySettings q{s.settings_.distance_threshold, s.settings_.min_normal_cosine, s.settings_.min_slope, 0, 0.0};
-        closest_points(s.bvh_, s.deformed_.get(), s.qnormals_.get(), s.has_mask_ ? s.mask_.get() : nullptr, s.nv_, q,
-                       s.corr_points_.get(), s.corr_normals_.get(), s.corr_dist_.get(), s.corr_tri_.get(),
-                       s.rejection_.get(), nullptr, st, s.corr_tri_.get(), s.query_order());
-        summarize_correspondences(s.rejection_.get(), s.corr_dist_.get(), s.nv_, s.summary_partials_, s.summary_.get(),
-                                  st);
-        s.to_host(st, 16, s.summary_.get(), 6);
-    }
-    std::vector<int64_t> accepted(jobs_.size());
-    for (size_t j = 0; j < jobs_.size(); ++j) {
-        Session& s = *jobs_[j];
-        hipStream_t st = s.stream_.get();
-        StreamScope scope(st);
-        s.sync();
-        accepted[j] = static_cast<int64_t>(s.staging_[16]);
-        for (int i = 0; i < 6; ++i) s.staging_[26 + i] = s.staging_[16 + i];
-        distance_quantiles(s.rejection_.get(), s.corr_dist_.get(), s.nv_, s.quantile_keys_, s.sort_,
-                           s.quantiles_.get(), accepted[j], st);
-        s.to_host(st, 24, s.quantiles_.get(), 2);
-    }
-    for (size_t j = 0; j < jobs_.size(); ++j) {
-        Session& s = *jobs_[j];
-        s.sync();
-        const double* h = s.staging_.get();
-        s.report_.counts = {s.nv_, accepted[j], static_cast<int64_t>(h[27]), static_cast<int64_t>(h[28]),
-                            static_cast<int64_t>(h[29]), static_cast<int64_t>(h[30])};
-        s.report_.stats = {accepted[j] > 0 ? h[31] / static_cast<double>(accepted[j]) : 0.0, h[24], h[25]};
         for (int k = 0; k < 5; ++k) s.report_.energy[k] = s.last_energy_[k];
+        s.final_statistics();
         s.phase_ = Session::Phase::Done;
-    }
+    });
+    stats_.finish_seconds += seconds_since(t0);
 }
 
 }  // namespace nricp_hip

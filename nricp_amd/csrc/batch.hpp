@@ -6,7 +6,7 @@
 // ticks: one tick is one damping attempt of every job still in the level --
 // refresh and assembly where a job's schedule asks for them, then the solve,
 // the candidate's energies and the decision -- as one fixed sequence of
-// packed launches (packed.hpp), captured as a graph once per level and
+// packed launches (packed.hpp), built into a graph once per level and
 // replayed. The decisions are taken on the device, from each job's control
 // record; the host reads one flag word per job after a tick to pick the next
 // tick's form (with or without the refresh and assembly stages) and to see
@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <functional>
 #include <vector>
 
 #include "packed.hpp"
@@ -23,20 +24,31 @@
 
 namespace nricp_hip {
 
+// Host threads that set up the jobs of a batch side by side (each job's
+// level set-up is a chain of small launches and readbacks on its own
+// streams).
+constexpr int kBatchHostThreads = 16;
+// f(0 .. count) on up to kBatchHostThreads threads, each with `device`
+// current; rethrows the first exception once all have finished.
+void parallel_jobs(int count, int device, const std::function<void(int)>& f);
+
 struct BatchConfig {
     int pcg_slots = 2;    // node slots per lane group of the packed PCG: 1 or 2
-    bool graphs = true;   // replay each level's ticks as captured graphs
+    bool graphs = true;   // replay each level's ticks from graphs
 };
 
 struct BatchStats {
     int64_t ticks = 0;          // packed ticks over all levels
     int64_t replays = 0;        // of them, graph replays
-    int64_t graphs = 0;         // graphs captured
+    int64_t graphs = 0;         // graphs built
     int64_t full_ticks = 0;     // ticks that refreshed correspondences
     int64_t assemble_ticks = 0; // ticks that assembled (without a refresh)
     double seconds = 0.0;       // run(): the whole batch
     double level_setup_seconds = 0.0;  // host time in the jobs' level set-ups
     double tick_seconds = 0.0;  // host time in the tick loops
+    double first_setup_seconds = 0.0;  // of the set-ups, the first level's (the jobs' preparation)
+    double end_seconds = 0.0;   // host time in the level ends
+    double finish_seconds = 0.0;  // host time in the final statistics
 };
 
 class BatchEngine {
@@ -57,13 +69,14 @@ private:
     void run_level(int level);
     void end_level(int level);
     void finish();
-    void enqueue_tick(TickForm form);
+    void enqueue_tick(TickForm form, KernelSink& sink);
     void launch_tick(TickForm form);
     void drop_graphs();
 
     std::vector<Session*> jobs_;
     BatchConfig config_;
     int precision_ = NRICP_HIP_PRECISION_DOUBLE;
+    int device_ = 0;
     Stream stream_;
     hipEvent_t done_ = nullptr;          // recorded after the engine's work of a level
     std::vector<hipEvent_t> ready_;      // per job: its level set-up is written

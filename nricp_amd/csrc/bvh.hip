@@ -1011,30 +1011,28 @@ int32_t packed_query_stack(int32_t depth_max) {
     return need <= 64 ? 64 : need <= 96 ? 96 : need <= 128 ? 128 : 296;
 }
 
-void packed_normals(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+void packed_normals(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink) {
     if (maps.vertex_blocks <= 0) return;
-    normals_packed<<<static_cast<unsigned>(maps.vertex_blocks), 256, 0, stream>>>(jobs, ctl, maps.vertex);
-    NRICP_LAUNCH_CHECK("normals_packed");
+    sink_launch(sink, normals_packed, static_cast<unsigned>(maps.vertex_blocks), 256, jobs, ctl, maps.vertex);
 }
 
-void packed_query(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+void packed_query(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink) {
     if (maps.query_blocks <= 0) return;
     const unsigned blocks = static_cast<unsigned>(maps.query_blocks);
     switch (maps.query_stack) {
         case 64:
-            query_packed<64><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            sink_launch(sink, query_packed<64>, blocks, kQueryThreads, jobs, ctl, maps.query);
             break;
         case 96:
-            query_packed<96><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            sink_launch(sink, query_packed<96>, blocks, kQueryThreads, jobs, ctl, maps.query);
             break;
         case 128:
-            query_packed<128><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            sink_launch(sink, query_packed<128>, blocks, kQueryThreads, jobs, ctl, maps.query);
             break;
         default:
-            query_packed<296><<<blocks, kQueryThreads, 0, stream>>>(jobs, ctl, maps.query);
+            sink_launch(sink, query_packed<296>, blocks, kQueryThreads, jobs, ctl, maps.query);
             break;
     }
-    NRICP_LAUNCH_CHECK("query_packed");
 }
 
 void fold_query_stats(QueryStats& stats, int64_t count, hipStream_t stream) {

@@ -892,7 +892,19 @@ int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats*
 struct nricp_hip_batch {
     std::vector<nricp_hip_session> sessions;  // each owns its job's session
     std::unique_ptr<BatchEngine> engine;      // released first (it drives the sessions)
-    ~nricp_hip_batch() { engine.reset(); }
+    ~nricp_hip_batch() {
+        engine.reset();
+        // Each session joins its threads and drains and destroys its streams:
+        // side by side on host threads.
+        int device = 0;
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+        try {
+            parallel_jobs(static_cast<int>(sessions.size()), device,
+                          [&](int i) { sessions[static_cast<size_t>(i)].session.reset(); });
+        } catch (...) {
+        }
+        sessions.clear();
+    }
 };
 
 int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_settings* settings, int32_t count,
@@ -901,13 +913,15 @@ int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_se
         require(problems && settings && out && count > 0, "batch_create: null argument or empty batch");
         ensure_device();
         auto b = std::make_unique<nricp_hip_batch>();
-        b->sessions.reserve(static_cast<size_t>(count));
+        b->sessions.resize(static_cast<size_t>(count));
+        // The sessions upload their meshes side by side.
+        int device = 0;
+        NRICP_CHECK(hipGetDevice(&device));
+        parallel_jobs(count, device, [&](int i) {
+            b->sessions[static_cast<size_t>(i)].session = std::make_unique<Session>(problems[i], settings[i], nullptr);
+        });
         std::vector<Session*> jobs;
-        for (int32_t i = 0; i < count; ++i) {
-            b->sessions.emplace_back();
-            b->sessions.back().session = std::make_unique<Session>(problems[i], settings[i], nullptr);
-            jobs.push_back(b->sessions.back().session.get());
-        }
+        for (auto& s : b->sessions) jobs.push_back(s.session.get());
         BatchConfig c;
         if (config) {
             c.pcg_slots = config->pcg_slots;
@@ -940,7 +954,9 @@ int nricp_hip_batch_stats(const nricp_hip_batch* batch, double* out, int32_t cap
         const double v[] = {static_cast<double>(s.ticks),       static_cast<double>(s.replays),
                             static_cast<double>(s.graphs),      static_cast<double>(s.full_ticks),
                             static_cast<double>(s.assemble_ticks), s.seconds,
-                            s.level_setup_seconds,              s.tick_seconds};
+                            s.level_setup_seconds,              s.tick_seconds,
+                            s.first_setup_seconds,              s.end_seconds,
+                            s.finish_seconds};
         *count = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
         for (int32_t i = 0; i < std::min(capacity, *count); ++i) out[i] = v[i];
     });

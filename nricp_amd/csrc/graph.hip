@@ -818,22 +818,19 @@ void retract(const double* R, const double* T, const double* delta, int64_t nn, 
     NRICP_LAUNCH_CHECK("retract_kernel");
 }
 
-void packed_evaluate(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+void packed_evaluate(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink) {
     if (maps.vertex_blocks <= 0) return;
-    evaluate_packed<<<static_cast<unsigned>(maps.vertex_blocks), 256, 0, stream>>>(jobs, ctl, maps.vertex);
-    NRICP_LAUNCH_CHECK("evaluate_packed");
+    sink_launch(sink, evaluate_packed, static_cast<unsigned>(maps.vertex_blocks), 256, jobs, ctl, maps.vertex);
 }
 
-void packed_retract(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+void packed_retract(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink) {
     if (maps.node_blocks <= 0) return;
-    retract_packed<<<static_cast<unsigned>(maps.node_blocks), 128, 0, stream>>>(jobs, ctl, maps.node);
-    NRICP_LAUNCH_CHECK("retract_packed");
+    sink_launch(sink, retract_packed, static_cast<unsigned>(maps.node_blocks), 128, jobs, ctl, maps.node);
 }
 
-void packed_commit(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream) {
+void packed_commit(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink) {
     if (maps.node_blocks <= 0) return;
-    commit_packed<<<static_cast<unsigned>(maps.node_blocks), 128, 0, stream>>>(jobs, ctl, maps.node);
-    NRICP_LAUNCH_CHECK("commit_packed");
+    sink_launch(sink, commit_packed, static_cast<unsigned>(maps.node_blocks), 128, jobs, ctl, maps.node);
 }
 
 namespace {

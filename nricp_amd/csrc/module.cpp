@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <iomanip>
 #include <limits>
@@ -993,37 +994,45 @@ public:
         }
         nricp_hip_batch* h = nullptr;
         int st;
+        const auto t0 = std::chrono::steady_clock::now();
         {
             py::gil_scoped_release release;
             st = nricp_hip_batch_create(problems.data(), settings.data(), static_cast<int32_t>(jobs_.size()), &config_,
                                         &h);
+            create_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (st == 0) {
                 handle_ = h;
                 st = nricp_hip_batch_run(h);
             }
         }
         check(st);
+        const auto t1 = std::chrono::steady_clock::now();
         py::list out;
         for (size_t i = 0; i < jobs_.size(); ++i) {
             nricp_hip_session* s = nullptr;
             check(nricp_hip_batch_job(handle_, static_cast<int32_t>(i), &s));
             out.append(session_result(s, *jobs_[i], false));
         }
+        results_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         return out;
     }
     py::dict stats() const {
         if (!handle_) throw py::value_error("the batch has not run");
-        static const char* names[] = {"ticks", "graph_replays", "graphs_captured", "refresh_ticks",
-                                      "assemble_ticks", "seconds", "level_setup_seconds", "tick_seconds"};
-        double v[8];
+        static const char* names[] = {"ticks", "graph_replays", "graphs_built", "refresh_ticks",
+                                      "assemble_ticks", "seconds", "level_setup_seconds", "tick_seconds",
+                                      "first_setup_seconds", "end_seconds", "finish_seconds"};
+        double v[11];
         int32_t n = 0;
-        check(nricp_hip_batch_stats(handle_, v, 8, &n));
+        check(nricp_hip_batch_stats(handle_, v, 11, &n));
         py::dict d;
-        for (int32_t i = 0; i < n && i < 8; ++i) d[names[i]] = v[i];
+        for (int32_t i = 0; i < n && i < 11; ++i) d[names[i]] = v[i];
+        d["create_seconds"] = create_seconds_;
+        d["results_seconds"] = results_seconds_;
         return d;
     }
 
 private:
+    double create_seconds_ = 0.0, results_seconds_ = 0.0;
     nricp_hip_batch_config config_{2, 1};
     std::vector<std::unique_ptr<Prepared>> jobs_;
     nricp_hip_batch* handle_ = nullptr;
@@ -1823,7 +1832,7 @@ PYBIND11_MODULE(nricp_dp, m) {
                       "A lockstep batch of wrap jobs: add() the jobs (route_a or route_b arguments, one precision "
                       "for all), then run() solves them together -- per graph level, every damping attempt of "
                       "every job is one shared launch per stage (closest points, assembly, one persistent PCG over "
-                      "all jobs' block rows, energies, the accept/reject decision on the device), captured as a "
+                      "all jobs' block rows, energies, the accept/reject decision on the device), built into a "
                       "graph and replayed -- and returns the result dicts in order. pcg_slots (1 or 2) is the "
                       "packed PCG's node slots per lane group. A job's result does not depend on the batch.")
         .def(py::init<int, bool>(), py::arg("pcg_slots") = 2, py::arg("use_graphs") = true)

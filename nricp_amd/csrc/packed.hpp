@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <tuple>
+#include <utility>
 
 #include "../../include/nricp_hip.h"
 #include "kernels.hpp"
@@ -155,33 +157,76 @@ struct PackedMaps {
     int32_t query_stack;         // the deepest job's query stack entries
 };
 
+// Where a stage's kernels go: launched on `stream`, or, with `graph` set,
+// appended to that graph as kernel nodes, each depending on the one before
+// (a tick's graph is built node by node, not captured from a stream, so what
+// other threads do meanwhile cannot touch it).
+struct KernelSink {
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    void launch(const void* kernel, dim3 grid, dim3 block, void** args) {
+        if (!graph) {
+            NRICP_CHECK(hipLaunchKernel(kernel, grid, block, args, 0, stream));
+            return;
+        }
+        hipKernelNodeParams p{};
+        p.func = const_cast<void*>(kernel);
+        p.gridDim = grid;
+        p.blockDim = block;
+        p.sharedMemBytes = 0;
+        p.kernelParams = args;
+        p.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        NRICP_CHECK(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p));
+        last = node;
+    }
+};
+
+template <typename Values, size_t... I>
+void sink_launch_values(KernelSink& sink, const void* kernel, unsigned grid, unsigned block, Values& values,
+                        std::index_sequence<I...>) {
+    void* args[] = {static_cast<void*>(&std::get<I>(values))..., nullptr};
+    sink.launch(kernel, dim3(grid), dim3(block), args);
+}
+
+// kernel<<<grid, block>>>(args...) through the sink (arguments converted to
+// the kernel's parameter types; a graph node copies them).
+template <typename... P, typename... A>
+void sink_launch(KernelSink& sink, void (*kernel)(P...), unsigned grid, unsigned block, A... args) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<P...> values(static_cast<P>(args)...);
+    sink_launch_values(sink, reinterpret_cast<const void*>(kernel), grid, block, values,
+                       std::index_sequence_for<P...>{});
+}
+
 // Stage launchers (each in the file of its single-job kernels). All take the
-// device arrays of jobs and controls, the maps, and the stream.
-void packed_evaluate(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream);
-void packed_normals(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream);
-void packed_query(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream);
+// device arrays of jobs and controls, the maps, and the sink.
+void packed_evaluate(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink);
+void packed_normals(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink);
+void packed_query(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink);
 // Records = true: the current state's rows, records and energy partials
 // (jobs refreshing or assembling); false: the candidate state's energy
 // partials (active jobs).
 void packed_rows(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, bool records,
-                 hipStream_t stream);
+                 KernelSink& sink);
 // precision: NRICP_HIP_PRECISION_* of the batch.
 void packed_system(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
-                   hipStream_t stream);
+                   KernelSink& sink);
 // Current-state energies (one workgroup per job) of the jobs that assembled.
-void packed_reduce_current(const PackedJob* jobs, const BatchControl* ctl, int32_t job_count, hipStream_t stream);
+void packed_reduce_current(const PackedJob* jobs, const BatchControl* ctl, int32_t job_count, KernelSink& sink);
 void packed_precondition(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
-                         unsigned* ticket, hipStream_t stream);
+                         unsigned* ticket, KernelSink& sink);
 // `k`: node slots per lane group (1 or 2), the PCG's tile shape.
 void packed_pcg(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision, int k,
-                unsigned* ticket, hipStream_t stream);
-void packed_retract(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream);
+                unsigned* ticket, KernelSink& sink);
+void packed_retract(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink);
 // Candidate energies and the schedule decision, one workgroup per job;
 // writes each job's next flags to `host_flags` (pinned host memory as the
 // device addresses it) as well.
 void packed_decide(const PackedJob* jobs, BatchControl* ctl, int32_t job_count, int32_t* host_flags,
-                   hipStream_t stream);
-void packed_commit(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, hipStream_t stream);
+                   KernelSink& sink);
+void packed_commit(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, KernelSink& sink);
 
 // PCG groups of a job of `nodes` nodes at `k` node slots per lane group.
 int64_t packed_pcg_groups(int64_t nodes, int k);

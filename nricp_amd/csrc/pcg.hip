@@ -833,26 +833,22 @@ int64_t packed_pcg_groups(int64_t nodes, int k) {
 }
 
 void packed_precondition(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision,
-                         unsigned* ticket, hipStream_t stream) {
+                         unsigned* ticket, KernelSink& sink) {
     if (maps.node_blocks <= 0) return;
     const unsigned blocks = static_cast<unsigned>(maps.node_blocks);
     if (precision == NRICP_HIP_PRECISION_DOUBLE)
-        precondition_packed<double, false><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
+        sink_launch(sink, precondition_packed<double, false>, blocks, 128, jobs, ctl, maps.node, ticket);
     else if (precision == NRICP_HIP_PRECISION_MIXED)
-        precondition_packed<float, true><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
+        sink_launch(sink, precondition_packed<float, true>, blocks, 128, jobs, ctl, maps.node, ticket);
     else
-        precondition_packed<float, false><<<blocks, 128, 0, stream>>>(jobs, ctl, maps.node, ticket);
-    NRICP_LAUNCH_CHECK("precondition_packed");
+        sink_launch(sink, precondition_packed<float, false>, blocks, 128, jobs, ctl, maps.node, ticket);
 }
 
 void packed_pcg(const PackedJob* jobs, const BatchControl* ctl, const PackedMaps& maps, int precision, int k,
-                unsigned* ticket, hipStream_t stream) {
+                unsigned* ticket, KernelSink& sink) {
     if (maps.pcg_groups <= 0) return;
     const unsigned groups = static_cast<unsigned>(maps.pcg_groups);
-    auto go = [&](auto kernel) {
-        kernel<<<groups, kPcgThreads, 0, stream>>>(jobs, ctl, maps.pcg, ticket);
-        NRICP_LAUNCH_CHECK("pcg_packed");
-    };
+    auto go = [&](auto kernel) { sink_launch(sink, kernel, groups, kPcgThreads, jobs, ctl, maps.pcg, ticket); };
     if (precision == NRICP_HIP_PRECISION_DOUBLE) {
         if (k == 1) go(pcg_packed<1, double, false>); else go(pcg_packed<2, double, false>);
     } else if (precision == NRICP_HIP_PRECISION_MIXED) {
