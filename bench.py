@@ -43,9 +43,10 @@ HIP-event stage timing and traversal counters on; the stage breakdown, the
 roofline figures and the query statistics come from those.
 
 Beside the headline (unless --no-extras), in the headline's form: the 64-job
-batch of uneven humanoid jobs (nricp_amd.batch: jobs/s at a fixed slot count,
-the same jobs one at a time, and whether a sample of jobs gives the same bits
-alone and in the batch), and the serving loop (nricp_amd.serve_bench: p50 /
+batch of uneven humanoid jobs (nricp_amd.batch: jobs/s as packed lockstep
+batches at the autotuner's shape and as concurrent sessions at a fixed slot
+count, the same jobs one at a time, and whether a sample of jobs gives the
+same bits alone and in the batch), and the serving loop (nricp_amd.serve_bench: p50 /
 p95 / p99 latency of Poisson arrivals at two offered loads below saturation).
 Their jobs are generated before the GPU is touched; they run while the
 one-core oracle wrap is still going.
@@ -293,6 +294,7 @@ CPU_ARRAYS = ("vertices", "node_positions", "level_nodes", "correspondence")
 
 
 BATCH_JOBS, BATCH_SEED = 64, 1000        # nricp_amd.batch's defaults
+PACKED_SHAPE = (2, 16, 4)                 # PCG node slots per lane group, jobs per packed batch, batches at once
 SERVING_POOL, SERVING_SEED = 16, 3000     # nricp_amd.serve_bench's defaults
 
 
@@ -317,25 +319,38 @@ def run_extras(dp, form: str, gen: dict, slots: int, rates: str) -> dict:
 
     jobs = gen["batch"]
     batch_mod.run_batch(dp, jobs[:8], 4, form)  # warm-up: code objects, allocator cache
+    batch_mod.run_packed(dp, jobs[:8], form, 2, 8)
+    # Packed lockstep batches at the shape nricp_amd.batch's autotuner picks
+    # on this workload (128-node PCG tile, 4 batches of 16 jobs at once).
+    pslots, pchunk, pconc = PACKED_SHAPE
+    packed_wall, packed, engine = batch_mod.run_packed(dp, jobs, form, pslots, pchunk, pconc)
+    check = min(4, len(jobs))
+    packed_alone = [batch_mod.run_packed(dp, [jobs[k]], form, pslots, 1)[1][0] for k in range(check)]
+    packed_same = all(np.array_equal(packed[k]["vertices"], packed_alone[k]["vertices"]) for k in range(check))
     wall, together = batch_mod.run_batch(dp, jobs, slots, form)
     serial_wall, alone = batch_mod.run_batch(dp, jobs, 1, form)
-    check = min(4, len(jobs))
     same = all(np.array_equal(together[k]["vertices"], alone[k]["vertices"]) for k in range(check))
     sizes = [len(j.template_points) for j in jobs]
     latency = np.array([r["queued_seconds"] + r["run_seconds"] for r in together]) * 1e3
-    log(f"batch: {len(jobs) / wall:.1f} jobs/s at {slots} slots, {len(jobs) / serial_wall:.1f} jobs/s one at a time")
+    log(f"batch: packed {len(jobs) / packed_wall:.1f} jobs/s, {len(jobs) / wall:.1f} jobs/s as concurrent sessions "
+        f"at {slots} slots, {len(jobs) / serial_wall:.1f} jobs/s one at a time")
     batch = {
         "metric": "batched uneven instances: jobs/s on one MI355X (whole batch)",
         "workload": f"{len(jobs)} humanoid jobs, {min(sizes)}-{max(sizes)} template vertices, target 2-3x the "
                     f"template's triangles, nodes ~ V/25, 4 levels x {OUTER} GN steps, seed {BATCH_SEED}",
         "form": form,
-        "slots": slots,
-        "jobs_per_s": round(len(jobs) / wall, 3),
+        "jobs_per_s": round(len(jobs) / packed_wall, 3),
+        "packed": {"pcg_slots": pslots, "jobs_per_batch": pchunk, "batches_at_once": pconc,
+                   "jobs_per_s": round(len(jobs) / packed_wall, 3),
+                   "ticks": int(engine.get("ticks", 0)), "graph_replays": int(engine.get("graph_replays", 0)),
+                   "same_bits_alone_and_in_batch": {"jobs_checked": check, "same": bool(packed_same)}},
+        "sessions": {"slots": slots, "jobs_per_s": round(len(jobs) / wall, 3),
+                     "latency_p50_ms": round(float(np.percentile(latency, 50)), 2),
+                     "latency_p99_ms": round(float(np.percentile(latency, 99)), 2),
+                     "same_bits_alone_and_in_batch": {"jobs_checked": check, "same": bool(same)}},
         "serial_jobs_per_s": round(len(jobs) / serial_wall, 3),
-        "speedup_over_serial": round(serial_wall / wall, 3),
-        "latency_p50_ms": round(float(np.percentile(latency, 50)), 2),
-        "latency_p99_ms": round(float(np.percentile(latency, 99)), 2),
-        "same_bits_alone_and_in_batch": {"jobs_checked": check, "same": bool(same)},
+        "packed_over_sessions": round(wall / packed_wall, 3),
+        "packed_over_serial": round(serial_wall / packed_wall, 3),
     }
     pool, big = gen["pool"], gen["big"]
     loads = []
