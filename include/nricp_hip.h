@@ -416,16 +416,21 @@ int nricp_hip_server_stats_get(nricp_hip_server* server, nricp_hip_server_stats*
  * energies, and the accept/reject decision taken on the device -- built into
  * a graph once per level and replayed. A job's result is the same bits in any batch.
  * `pcg_slots` (1 or 2) is the packed PCG's node slots per lane group (its
- * tile shape); `use_graphs` 0 launches the stages directly. */
+ * tile shape), 0 picks it per level as the single-job session does;
+ * `use_graphs` 0 launches the stages directly. */
 typedef struct nricp_hip_batch nricp_hip_batch;
 typedef struct nricp_hip_batch_config {
     int32_t pcg_slots;
     int32_t use_graphs;
+    int32_t lookahead; /* queue each tick one ahead of the flags' readback, every tick in its full form */
 } nricp_hip_batch_config;
 /* `settings` holds `count` entries, one per problem. */
 int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_settings* settings, int32_t count,
                            const nricp_hip_batch_config* config, nricp_hip_batch** out_batch);
 int nricp_hip_batch_run(nricp_hip_batch* batch);
+/* Rewinds every job to the start of its wrap, inputs kept on the device; the
+ * next nricp_hip_batch_run solves them again. */
+int nricp_hip_batch_reset(nricp_hip_batch* batch);
 /* Job `index` as a session handle owned by the batch (results: vertices,
  * report, state, level nodes, counters; not to be destroyed or stepped). */
 int nricp_hip_batch_job(nricp_hip_batch* batch, int32_t index, nricp_hip_session** out_session);

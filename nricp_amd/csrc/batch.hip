@@ -68,7 +68,7 @@ void parallel_jobs(int count, int device, const std::function<void(int)>& f) {
 BatchEngine::BatchEngine(std::vector<Session*> jobs, const BatchConfig& config)
     : jobs_(std::move(jobs)), config_(config), host_flags_(static_cast<int64_t>(std::max<size_t>(jobs_.size(), 1))) {
     if (jobs_.empty()) throw InvalidArgument("a batch needs at least one job");
-    if (config_.pcg_slots != 1 && config_.pcg_slots != 2) throw InvalidArgument("pcg_slots must be 1 or 2");
+    if (config_.pcg_slots < 0 || config_.pcg_slots > 2) throw InvalidArgument("pcg_slots must be 0, 1 or 2");
     precision_ = jobs_[0]->settings_.precision;
     const int64_t group_limit = packed_pcg_group_limit();
     for (Session* s : jobs_) {
@@ -82,7 +82,7 @@ BatchEngine::BatchEngine(std::vector<Session*> jobs, const BatchConfig& config)
             finest = std::max(finest, level_count_for(static_cast<int>(s->graph_->levels.size()),
                                                       s->settings_.nodes_initial, s->settings_.growth, s->nv_,
                                                       static_cast<int>(l)));
-        if (packed_pcg_groups(finest, config_.pcg_slots) > group_limit)
+        if (packed_pcg_groups(finest, config_.pcg_slots == 1 ? 1 : 2) > group_limit)
             throw InvalidArgument("a batch job's finest level has " + std::to_string(finest) +
                                   " nodes, more than one packed solve takes on this device");
     }
@@ -92,6 +92,7 @@ BatchEngine::BatchEngine(std::vector<Session*> jobs, const BatchConfig& config)
     NRICP_CHECK(hipEventRecord(done_, stream_.get()));
     ready_.resize(jobs_.size(), nullptr);
     for (auto& e : ready_) NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : ticked_) NRICP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     StreamScope scope(stream_.get());
     ticket_.resize(1);
 }
@@ -101,7 +102,16 @@ BatchEngine::~BatchEngine() {
     drop_graphs();
     for (hipEvent_t e : ready_)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ticked_)
+        if (e) (void)hipEventDestroy(e);
     if (done_) (void)hipEventDestroy(done_);
+}
+
+void BatchEngine::reset() {
+    NRICP_CHECK(hipStreamSynchronize(stream_.get()));
+    drop_graphs();
+    parallel_jobs(static_cast<int>(jobs_.size()), device_, [&](int j) { jobs_[j]->reset(); });
+    NRICP_CHECK(hipEventRecord(done_, stream_.get()));
 }
 
 void BatchEngine::drop_graphs() {
@@ -133,7 +143,19 @@ void BatchEngine::begin_level(int level) {
     slots_.clear();
     for (size_t j = 0; j < jobs_.size(); ++j)
         if (static_cast<size_t>(level) < jobs_[j]->graph_->levels.size()) slots_.push_back(static_cast<int>(j));
-    const int k = config_.pcg_slots;
+    int k = config_.pcg_slots;
+    if (k == 0) {
+        // The single-job session's tile rule (pcg.hip pcg_shape): one slot per
+        // lane group up to 32 groups, then two.
+        int64_t most = 0;
+        for (int j : slots_) most = std::max(most, jobs_[j]->graph_->levels.empty()
+                                                       ? int64_t{0}
+                                                       : level_count_for(static_cast<int>(jobs_[j]->graph_->levels.size()),
+                                                                         jobs_[j]->settings_.nodes_initial,
+                                                                         jobs_[j]->settings_.growth, jobs_[j]->nv_, level));
+        k = packed_pcg_groups(most, 1) <= 32 ? 1 : 2;
+    }
+    level_slots_ = k;
     std::vector<PackedJob> packed(slots_.size());
     control_host_.assign(slots_.size(), BatchControl{});
     struct Sizes {
@@ -362,7 +384,7 @@ void BatchEngine::enqueue_tick(TickForm form, KernelSink& sink) {
         packed_reduce_current(jobs, ctl, count, sink);
     }
     packed_precondition(jobs, ctl, maps_, precision_, ticket_.get(), sink);
-    packed_pcg(jobs, ctl, maps_, precision_, config_.pcg_slots, ticket_.get(), sink);
+    packed_pcg(jobs, ctl, maps_, precision_, level_slots_, ticket_.get(), sink);
     packed_retract(jobs, ctl, maps_, sink);
     packed_rows(jobs, ctl, maps_, false, sink);
     packed_decide(jobs, ctl, count, host_flags_device_, sink);
@@ -413,6 +435,26 @@ void BatchEngine::launch_tick(TickForm form) {
 
 void BatchEngine::run_level(int) {
     const auto t0 = std::chrono::steady_clock::now();
+    if (config_.lookahead) {
+        // Tick k + 1 is queued before the host reads tick k's flags. A flag
+        // read while a later tick runs is only newer, and a job's level, once
+        // ended, stays ended: the first read that shows no job active ends
+        // the level, and the tick queued ahead of it finds every job idle.
+        hipStream_t es = stream_.get();
+        launch_tick(kTickFull);
+        NRICP_CHECK(hipEventRecord(ticked_[0], es));
+        for (int64_t k = 0;; ++k) {
+            launch_tick(kTickFull);
+            NRICP_CHECK(hipEventRecord(ticked_[(k + 1) % 2], es));
+            NRICP_CHECK(hipEventSynchronize(ticked_[k % 2]));
+            int32_t any = 0;
+            for (int32_t i = 0; i < maps_.jobs; ++i) any |= __atomic_load_n(&host_flags_[i], __ATOMIC_ACQUIRE);
+            if (!(any & kJobActive)) break;
+        }
+        NRICP_CHECK(hipStreamSynchronize(es));
+        stats_.tick_seconds += seconds_since(t0);
+        return;
+    }
     TickForm form = kTickFull;  // every job starts its level with a refresh
     for (;;) {
         launch_tick(form);

@@ -33,8 +33,15 @@ constexpr int kBatchHostThreads = 16;
 void parallel_jobs(int count, int device, const std::function<void(int)>& f);
 
 struct BatchConfig {
-    int pcg_slots = 2;    // node slots per lane group of the packed PCG: 1 or 2
+    // Node slots per lane group of the packed PCG: 1 or 2, or 0 for the
+    // single-job session's rule per level (one slot while the level's largest
+    // job needs at most 32 groups at one slot, else two).
+    int pcg_slots = 2;
     bool graphs = true;   // replay each level's ticks from graphs
+    // Queue each tick one ahead of the one whose flags the host reads, every
+    // tick in its full form (the stages a job does not need this tick return
+    // at once): the device never waits on the host's read of the flags.
+    bool lookahead = false;
 };
 
 struct BatchStats {
@@ -61,6 +68,9 @@ public:
     BatchEngine& operator=(const BatchEngine&) = delete;
     // Runs every job to completion (all levels, final statistics).
     void run();
+    // Rewinds every job to the start of its wrap (inputs kept resident); the
+    // next run() solves them again.
+    void reset();
     const BatchStats& stats() const { return stats_; }
 
 private:
@@ -79,6 +89,7 @@ private:
     int device_ = 0;
     Stream stream_;
     hipEvent_t done_ = nullptr;          // recorded after the engine's work of a level
+    std::array<hipEvent_t, 2> ticked_{}; // recorded after alternate ticks (lookahead)
     std::vector<hipEvent_t> ready_;      // per job: its level set-up is written
     // the level's packed tables
     std::vector<int> slots_;             // jobs in the level, in slot order
@@ -88,6 +99,7 @@ private:
     DeviceBuffer<unsigned> ticket_;
     PackedMaps maps_{};
     int charge_groups_ = 1;              // the most PCG groups of one job
+    int level_slots_ = 2;                // the level's PCG node slots per lane group
     PinnedBuffer<int32_t> host_flags_;
     int32_t* host_flags_device_ = nullptr;
     std::vector<BatchControl> control_host_;

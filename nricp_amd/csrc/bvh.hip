@@ -743,7 +743,9 @@ query_packed(const PackedJob* __restrict__ jobs, const BatchControl* __restrict_
              const PackedBlock* __restrict__ map) {
     __shared__ int32_t stack_node[kGroupsPerBlock][StackDepth];
     __shared__ float stack_dist[kGroupsPerBlock][StackDepth];
-    const PackedBlock m = map[xcd_contiguous(blockIdx.x, gridDim.x)];
+    // Blocks in launch order, as the single-job query takes them (an
+    // XCD-contiguous order measured slower for the traversal).
+    const PackedBlock m = map[blockIdx.x];
     if (!(ctl[m.job].flags & kJobRefresh)) return;
     const PackedJob& J = jobs[m.job];
     // Every refresh after the rest-template pass starts from the last

@@ -926,6 +926,7 @@ int nricp_hip_batch_create(const nricp_hip_problem* problems, const nricp_hip_se
         if (config) {
             c.pcg_slots = config->pcg_slots;
             c.graphs = config->use_graphs != 0;
+            c.lookahead = config->lookahead != 0;
         }
         b->engine = std::make_unique<BatchEngine>(std::move(jobs), c);
         *out = b.release();
@@ -936,6 +937,13 @@ int nricp_hip_batch_run(nricp_hip_batch* batch) {
     return guarded([&] {
         require(batch && batch->engine, "batch_run: null handle");
         batch->engine->run();
+    });
+}
+
+int nricp_hip_batch_reset(nricp_hip_batch* batch) {
+    return guarded([&] {
+        require(batch && batch->engine, "batch_reset: null handle");
+        batch->engine->reset();
     });
 }
 

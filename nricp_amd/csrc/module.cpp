@@ -964,9 +964,10 @@ private:
 // layout of solve_route_*, without the energy trace).
 class Batch {
 public:
-    Batch(int pcg_slots, bool use_graphs) {
+    Batch(int pcg_slots, bool use_graphs, bool lookahead) {
         config_.pcg_slots = pcg_slots;
         config_.use_graphs = use_graphs ? 1 : 0;
+        config_.lookahead = lookahead ? 1 : 0;
     }
     ~Batch() { close(); }
     Batch(const Batch&) = delete;
@@ -984,7 +985,7 @@ public:
         return static_cast<int>(jobs_.size() - 1);
     }
     py::list run() {
-        if (handle_) throw py::value_error("the batch has already run");
+        if (handle_) return rerun();
         if (jobs_.empty()) throw py::value_error("the batch has no jobs");
         std::vector<nricp_hip_problem> problems;
         std::vector<nricp_hip_settings> settings;
@@ -1016,6 +1017,23 @@ public:
         results_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         return out;
     }
+    // The same jobs again, their inputs still on the device.
+    py::list rerun() {
+        int st;
+        {
+            py::gil_scoped_release release;
+            st = nricp_hip_batch_reset(handle_);
+            if (st == 0) st = nricp_hip_batch_run(handle_);
+        }
+        check(st);
+        py::list out;
+        for (size_t i = 0; i < jobs_.size(); ++i) {
+            nricp_hip_session* s = nullptr;
+            check(nricp_hip_batch_job(handle_, static_cast<int32_t>(i), &s));
+            out.append(session_result(s, *jobs_[i], false));
+        }
+        return out;
+    }
     py::dict stats() const {
         if (!handle_) throw py::value_error("the batch has not run");
         static const char* names[] = {"ticks", "graph_replays", "graphs_built", "refresh_ticks",
@@ -1033,7 +1051,7 @@ public:
 
 private:
     double create_seconds_ = 0.0, results_seconds_ = 0.0;
-    nricp_hip_batch_config config_{2, 1};
+    nricp_hip_batch_config config_{2, 1, 0};
     std::vector<std::unique_ptr<Prepared>> jobs_;
     nricp_hip_batch* handle_ = nullptr;
 };
@@ -1834,8 +1852,12 @@ PYBIND11_MODULE(nricp_dp, m) {
                       "every job is one shared launch per stage (closest points, assembly, one persistent PCG over "
                       "all jobs' block rows, energies, the accept/reject decision on the device), built into a "
                       "graph and replayed -- and returns the result dicts in order. pcg_slots (1 or 2) is the "
-                      "packed PCG's node slots per lane group. A job's result does not depend on the batch.")
-        .def(py::init<int, bool>(), py::arg("pcg_slots") = 2, py::arg("use_graphs") = true)
+                      "packed PCG's node slots per lane group (0: per level, as the single-job session picks them); "
+                      "lookahead queues each tick ahead of the host's "
+                      "read of the decisions. run() again solves the same jobs again (inputs kept on the device). "
+                      "A job's result does not depend on the batch.")
+        .def(py::init<int, bool, bool>(), py::arg("pcg_slots") = 2, py::arg("use_graphs") = true,
+             py::arg("lookahead") = false)
         .def("add",
              [](Batch& self, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                 const py::object& mk, const py::object& li, const py::object& lt, double dist, double mincos,
