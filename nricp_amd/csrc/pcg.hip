@@ -817,12 +817,251 @@ void launch_pcg(const LevelLayout& L, const S* bsr, const double* bsr64, const d
     ws.last_groups = sh.groups;
 }
 
+// ------------------------------------------------------- sweeps (large) ---
+// Levels above the persistent launch's node limit: the same block-Jacobi PCG
+// (src/cuda/pcg_solver.cu:63-241: x0 = 0, stop when |r|/|b| <= tol on the
+// recurrence residual, ok <=> converged) as a sequence of ordinary launches,
+// three per iteration, with FP64 vectors and the matrix in its own precision.
+// Each launch folds the previous launch's per-block partial sums in one fixed
+// order (every block obtains the same bits), so a solve is reproducible and
+// needs no grid barrier and no atomics. A solve that has converged or broken
+// down turns the launches that remain into no-ops; the host reads the done
+// flag every kSweepCheck iterations and stops enqueueing.
+constexpr int kSweepThreads = 192;  // 32 nodes: a node's six rows never straddle two blocks
+constexpr int kSweepNodes = kSweepThreads / 6;
+constexpr int kSweepCheck = 8;
+// The state after the partial sums: r.z by iteration parity, |b|, done, rel.
+constexpr int kSweepRz = 0, kSweepBnorm = 2, kSweepDone = 3, kSweepRel = 4, kSweepStateDoubles = 8;
+
+template <int C>
+__device__ inline void fold_blocks(const double* __restrict__ part, int64_t blocks, double* lds, double (&out)[C]) {
+    double v[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) v[k] = 0.0;
+    for (int64_t b = threadIdx.x; b < blocks; b += kSweepThreads)
+#pragma unroll
+        for (int k = 0; k < C; ++k) v[k] += part[C * b + k];
+    block_sum<C, kSweepThreads>(v, lds, out);
+}
+
+__device__ inline void sweep_finish(double* state, double* outcome, int it, double rel, bool converged,
+                                    bool breakdown) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        outcome[0] = it;
+        outcome[1] = rel;
+        outcome[2] = converged ? 1.0 : 0.0;
+        outcome[3] = breakdown ? 1.0 : 0.0;
+        state[kSweepDone] = 1.0;
+    }
+}
+
+// r = b, x = 0, z = M^-1 b; partials (r.z, r.r) per block.
+template <typename S>
+__global__ void __launch_bounds__(kSweepThreads)
+sweep_init(const S* __restrict__ minv, const double* __restrict__ rhs, int64_t n6, double* __restrict__ x,
+           double* __restrict__ r, double* __restrict__ z, double* __restrict__ part_a, double* __restrict__ state) {
+    __shared__ double lds[2 * (kSweepThreads / kWave)];
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kSweepThreads + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (t < n6) {
+        const int64_t node = t / 6;
+        const int row = static_cast<int>(t % 6);
+        const S* m = minv + 36 * node + 6 * row;
+        double zt = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) zt += static_cast<double>(m[c]) * rhs[6 * node + c];
+        const double b = rhs[t];
+        x[t] = 0.0;
+        r[t] = b;
+        z[t] = zt;
+        v[0] = b * zt;
+        v[1] = b * b;
+    }
+    double s[2];
+    block_sum<2, kSweepThreads>(v, lds, s);
+    if (threadIdx.x == 0) {
+        part_a[2 * blockIdx.x] = s[0];
+        part_a[2 * blockIdx.x + 1] = s[1];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) state[kSweepDone] = 0.0;
+}
+
+// Iteration `it` begins: (r.z, r.r) of the last update, the stopping tests,
+// then the search direction p = z + beta p (p = z at it = 0). With `last`
+// (it = the iteration cap) only the tests run.
+__global__ void __launch_bounds__(kSweepThreads)
+sweep_direction(int it, int last, const double* __restrict__ part_a, int64_t blocks, double tol, int64_t n6,
+                const double* __restrict__ z, double* __restrict__ p, double* __restrict__ state,
+                double* __restrict__ outcome) {
+    __shared__ double lds[2 * (kSweepThreads / kWave)];
+    if (state[kSweepDone] != 0.0) return;
+    double s[2];
+    fold_blocks<2>(part_a, blocks, lds, s);
+    const double rz = s[0], rr = s[1];
+    const double bnorm = it == 0 ? sqrt(rr) : state[kSweepBnorm];
+    const double rnorm = sqrt(rr);
+    const double rel = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+    if (rnorm <= tol * bnorm) {
+        sweep_finish(state, outcome, it, rel, true, false);
+        return;
+    }
+    if (last) {
+        sweep_finish(state, outcome, it, rel, false, false);
+        return;
+    }
+    if (!(rz > 0.0) || !isfinite(rz)) {
+        sweep_finish(state, outcome, it, rel, false, true);
+        return;
+    }
+    const double beta = it == 0 ? 0.0 : rz / state[kSweepRz + ((it - 1) & 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state[kSweepRz + (it & 1)] = rz;
+        state[kSweepRel] = rel;
+        if (it == 0) state[kSweepBnorm] = bnorm;
+    }
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kSweepThreads + threadIdx.x;
+    if (t < n6) p[t] = it == 0 ? z[t] : z[t] + beta * p[t];
+}
+
+// q = (A + D) p over the block rows; partials p.q per block.
+template <typename S>
+__global__ void __launch_bounds__(kSweepThreads)
+sweep_product(const int32_t* __restrict__ row_off, const int32_t* __restrict__ cols, const S* __restrict__ bsr,
+              const S* __restrict__ damp, int64_t n6, const double* __restrict__ p, double* __restrict__ q,
+              double* __restrict__ part_b, const double* __restrict__ state) {
+    __shared__ double lds[kSweepThreads / kWave];
+    if (state[kSweepDone] != 0.0) return;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kSweepThreads + threadIdx.x;
+    double v[1] = {0.0};
+    if (t < n6) {
+        const int64_t node = t / 6;
+        const int row = static_cast<int>(t % 6);
+        double acc = static_cast<double>(damp[t]) * p[t];
+        for (int32_t b = row_off[node]; b < row_off[node + 1]; ++b) {
+            const S* blk = bsr + 36 * static_cast<int64_t>(b) + 6 * row;
+            const double* pc = p + 6 * static_cast<int64_t>(cols[b]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += static_cast<double>(blk[c]) * pc[c];
+        }
+        q[t] = acc;
+        v[0] = p[t] * acc;
+    }
+    double s[1];
+    block_sum<1, kSweepThreads>(v, lds, s);
+    if (threadIdx.x == 0) part_b[blockIdx.x] = s[0];
+}
+
+// alpha = r.z / p.q; x += alpha p, r -= alpha q, z = M^-1 r; partials
+// (r.z, r.r) per block.
+template <typename S>
+__global__ void __launch_bounds__(kSweepThreads)
+sweep_update(int it, const double* __restrict__ part_b, int64_t blocks, const S* __restrict__ minv, int64_t n6,
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ p,
+             const double* __restrict__ q, double* __restrict__ part_a, double* __restrict__ state,
+             double* __restrict__ outcome) {
+    __shared__ double lds[2 * (kSweepThreads / kWave)];
+    __shared__ double rows[kSweepThreads];
+    if (state[kSweepDone] != 0.0) return;
+    double s1[1];
+    fold_blocks<1>(part_b, blocks, lds, s1);
+    const double pq = s1[0];
+    if (!(pq > 0.0) || !isfinite(pq)) {
+        sweep_finish(state, outcome, it, state[kSweepRel], false, true);
+        return;
+    }
+    const double alpha = state[kSweepRz + (it & 1)] / pq;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kSweepThreads + threadIdx.x;
+    double rt = 0.0;
+    if (t < n6) {
+        x[t] += alpha * p[t];
+        rt = r[t] - alpha * q[t];
+        r[t] = rt;
+    }
+    rows[threadIdx.x] = rt;
+    __syncthreads();
+    double v[2] = {0.0, 0.0};
+    if (t < n6) {
+        const int64_t node = t / 6;
+        const int row = static_cast<int>(t % 6);
+        const S* m = minv + 36 * node + 6 * row;
+        const double* rn = rows + 6 * (threadIdx.x / 6);
+        double zt = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) zt += static_cast<double>(m[c]) * rn[c];
+        z[t] = zt;
+        v[0] = rt * zt;
+        v[1] = rt * rt;
+    }
+    double s[2];
+    block_sum<2, kSweepThreads>(v, lds, s);
+    if (threadIdx.x == 0) {
+        part_a[2 * blockIdx.x] = s[0];
+        part_a[2 * blockIdx.x + 1] = s[1];
+    }
+}
+
+// The multi-launch solve on the session's workspace: damp (6N, S) and minv
+// (36N, S) come in; ws.r, ws.u, ws.p, ws.w (6N each) hold r, z, p and q.
+template <typename S>
+void pcg_sweeps(const LevelLayout& L, const S* bsr, const double* rhs, int64_t nn, double lambda, bool scaling,
+                double tol, int max_iter, double* solution, PcgWorkspace& ws, S* damp, S* minv, hipStream_t stream) {
+    const int64_t n6 = 6 * nn;
+    const int64_t blocks = (nn + kSweepNodes - 1) / kSweepNodes;
+    if (blocks > static_cast<int64_t>(INT32_MAX)) throw InvalidArgument("linear system too large");
+    ws.barrier.resize(2);
+    ws.outcome.resize(4);
+    ws.reduce.resize(3 * blocks + kSweepStateDoubles);
+    ws.r.resize(n6);
+    ws.u.resize(n6);
+    ws.p.resize(n6);
+    ws.w.resize(n6);
+    double* part_a = ws.reduce.get();
+    double* part_b = part_a + 2 * blocks;
+    double* state = part_b + blocks;
+    const unsigned grid = static_cast<unsigned>(blocks);
+    precondition_kernel<S><<<blocks_for(nn, 128), 128, 0, stream>>>(bsr, nullptr, L.diag_position.get(), nn, lambda,
+                                                                     scaling ? 1 : 0, damp, nullptr, minv,
+                                                                     ws.barrier.get());
+    NRICP_LAUNCH_CHECK("precondition_kernel");
+    if (ws.kernel_start) NRICP_CHECK(hipEventRecord(ws.kernel_start, stream));
+    sweep_init<S><<<grid, kSweepThreads, 0, stream>>>(minv, rhs, n6, solution, ws.r.get(), ws.u.get(), part_a, state);
+    NRICP_LAUNCH_CHECK("sweep_init");
+    const int cap = std::max(0, max_iter);
+    for (int it = 0;; ++it) {
+        const int last = it == cap ? 1 : 0;
+        sweep_direction<<<grid, kSweepThreads, 0, stream>>>(it, last, part_a, blocks, tol, n6, ws.u.get(),
+                                                            ws.p.get(), state, ws.outcome.get());
+        NRICP_LAUNCH_CHECK("sweep_direction");
+        if (last) break;
+        sweep_product<S><<<grid, kSweepThreads, 0, stream>>>(L.row_offsets.get(), L.columns.get(), bsr, damp, n6,
+                                                             ws.p.get(), ws.w.get(), part_b, state);
+        NRICP_LAUNCH_CHECK("sweep_product");
+        sweep_update<S><<<grid, kSweepThreads, 0, stream>>>(it, part_b, blocks, minv, n6, solution, ws.r.get(),
+                                                            ws.u.get(), ws.p.get(), ws.w.get(), part_a, state,
+                                                            ws.outcome.get());
+        NRICP_LAUNCH_CHECK("sweep_update");
+        if ((it + 1) % kSweepCheck == 0) {
+            double done = 0.0;
+            NRICP_CHECK(hipMemcpyAsync(&done, state + kSweepDone, sizeof(double), hipMemcpyDeviceToHost, stream));
+            NRICP_CHECK(hipStreamSynchronize(stream));
+            if (done != 0.0) break;
+        }
+    }
+    if (ws.kernel_stop) NRICP_CHECK(hipEventRecord(ws.kernel_stop, stream));
+    ws.last_groups = 0;
+}
+
 }  // namespace
 
 void pcg_solve(const LevelLayout& L, const double* bsr, const double* rhs, int64_t nn, double lambda, bool scaling,
                double tol, int max_iter, double* solution, PcgWorkspace& ws, hipStream_t stream) {
     ws.damp.resize(6 * nn);
     ws.minv.resize(36 * nn);
+    if (nn > pcg_max_nodes()) {
+        pcg_sweeps<double>(L, bsr, rhs, nn, lambda, scaling, tol, max_iter, solution, ws, ws.damp.get(),
+                           ws.minv.get(), stream);
+        return;
+    }
     launch_pcg<double>(L, bsr, nullptr, rhs, nn, lambda, scaling, tol, tol, max_iter, solution, ws, ws.damp.get(),
                        nullptr, ws.minv.get(), ws.u, stream);
 }
@@ -901,6 +1140,23 @@ void pcg_solve_on(const int32_t* row_off, const int32_t* cols, const int32_t* di
 void pcg_solve_mixed(const LevelLayout& L, const float* bsr, const double* bsr64, const double* rhs, int64_t nn,
                      double lambda, bool scaling, double tol, int max_iter, double* solution, PcgWorkspace& ws,
                      hipStream_t stream) {
+    if (nn > pcg_max_nodes()) {
+        // Above one persistent launch: the multi-launch solve in FP64
+        // arithmetic, on the FP64 blocks when the form has them (the mixed
+        // form's target system), else on the FP32 ones.
+        if (bsr64) {
+            ws.damp.resize(6 * nn);
+            ws.minv.resize(36 * nn);
+            pcg_sweeps<double>(L, bsr64, rhs, nn, lambda, scaling, tol, max_iter, solution, ws, ws.damp.get(),
+                               ws.minv.get(), stream);
+        } else {
+            ws.w32.resize(6 * nn);
+            ws.minv32.resize(36 * nn);
+            pcg_sweeps<float>(L, bsr, rhs, nn, lambda, scaling, tol, max_iter, solution, ws, ws.w32.get(),
+                              ws.minv32.get(), stream);
+        }
+        return;
+    }
     ws.w32.resize(6 * nn);  // damping vector (FP32)
     ws.minv32.resize(36 * nn);
     if (bsr64) ws.damp.resize(6 * nn);  // FP64 damping of the refinement residual

@@ -265,10 +265,9 @@ void check_level_sizes(const nricp_hip_problem& pb, const nricp_hip_settings& s,
         for (int l = 0; l < lv; ++l)
             finest = std::max(finest, level_count_for(lv, s.nodes_initial, s.growth, pb.template_vertex_count, l));
     }
-    const int64_t limit = pcg_max_nodes();
-    if (finest > limit)
-        throw InvalidArgument("the finest graph level has " + std::to_string(finest) + " nodes; one solve on this "
-                              "device takes at most " + std::to_string(limit) + " (lower nodes_initial, growth or "
+    if (finest > kMaxLevelNodes)
+        throw InvalidArgument("the finest graph level has " + std::to_string(finest) + " nodes; a level takes at "
+                              "most " + std::to_string(kMaxLevelNodes) + " (lower nodes_initial, growth or "
                               "graph_levels)");
 }
 

@@ -52,8 +52,11 @@ void build_graph_level(const double* points, int64_t vertex_count, const int32_t
 int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level);
 
 // Throws InvalidArgument when the finest level of the job's graph (the given
-// one, or the one its settings ask for) has more nodes than one solve takes on
-// this device (pcg_max_nodes), so the job fails before any work, not midway.
+// one, or the one its settings ask for) has more nodes than the level's 32-bit
+// block indices allow (kMaxLevelNodes), so the job fails before any work, not
+// midway. Levels above one persistent solve's size (pcg_max_nodes) are solved
+// by the multi-launch PCG.
+constexpr int64_t kMaxLevelNodes = int64_t{1} << 25;
 void check_level_sizes(const nricp_hip_problem& problem, const nricp_hip_settings& settings,
                        const DeviceGraph* graph);
 

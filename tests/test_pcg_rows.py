@@ -51,7 +51,7 @@ def random_system(n: int, seed: int):
     return diag, edges, blocks, rhs, degree
 
 
-def direct_solve(diag, edges, blocks, rhs, damping):
+def system_matrix(diag, edges, blocks, damping):
     n = len(diag)
     rows, cols, vals = [], [], []
     r6, c6 = np.meshgrid(np.arange(6), np.arange(6), indexing="ij")
@@ -66,15 +66,42 @@ def direct_solve(diag, edges, blocks, rhs, damping):
     for (a, b), blk in zip(edges, blocks):
         put(a, b, blk)
         put(b, a, blk.T)
-    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(6 * n, 6 * n))
-    return spla.spsolve(A.tocsc(), rhs.reshape(-1))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(6 * n, 6 * n))
+
+
+def direct_solve(diag, edges, blocks, rhs, damping):
+    return spla.spsolve(system_matrix(diag, edges, blocks, damping).tocsc(), rhs.reshape(-1))
+
+
+def block_jacobi_pcg_iterations(diag, edges, blocks, rhs, damping, tol):
+    """Iterations of the reference's PCG (block-Jacobi, x0 = 0, stop when the
+    recurrence residual reaches tol |b|) in numpy, FP64."""
+    A = system_matrix(diag, edges, blocks, damping)
+    minv = np.linalg.inv(diag + damping * np.eye(6))
+    precondition = lambda r: np.einsum("nij,nj->ni", minv, r.reshape(-1, 6)).reshape(-1)  # noqa: E731
+    b = rhs.reshape(-1)
+    r = b.copy()
+    z = precondition(r)
+    p = z.copy()
+    rz = r @ z
+    for it in range(10_000):
+        if np.linalg.norm(r) <= tol * np.linalg.norm(b):
+            return it
+        q = A @ p
+        alpha = rz / (p @ q)
+        r = r - alpha * q
+        z = precondition(r)
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+    return 10_000
 
 
 # 40 and 100 nodes run in one workgroup (one and two node slots per lane
 # group); 1,500 and 3,000 take the grid kernels (24 groups of one slot, 24
 # groups of two); 8,000 is the 200k-vertex job's finest level (63 groups of
-# two slots).
-@pytest.mark.parametrize("n", [40, 100, 1500, 3000, 8000])
+# two slots); 20,000 is above what one persistent launch holds (16,384) and
+# runs as the multi-launch PCG.
+@pytest.mark.parametrize("n", [40, 100, 1500, 3000, 8000, 20_000])
 @pytest.mark.parametrize("precision", ["double", "mixed", "float"])
 def test_rows_of_every_length_match_a_direct_solve(device, n, precision):
     diag, edges, blocks, rhs, degree = random_system(n, seed=n)
@@ -93,3 +120,9 @@ def test_rows_of_every_length_match_a_direct_solve(device, n, precision):
     # it is the FP32 rounding of the matrix (about 6e-8 relative) times |x|.
     bound = 1e-6 if precision == "float" else 1e-11
     assert r["residual_norm"] <= bound * np.linalg.norm(rhs)
+    if n > 16_384 and precision != "float":
+        # The multi-launch PCG is the reference's recurrence in FP64 (the
+        # mixed form solves its FP64 system there): the same iteration count
+        # as a numpy block-Jacobi PCG, up to rounding at the stopping test.
+        expected_it = block_jacobi_pcg_iterations(diag, edges, blocks, rhs, DAMPING, 1e-13)
+        assert abs(r["iterations"] - expected_it) <= 1, (r["iterations"], expected_it)

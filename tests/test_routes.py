@@ -9,11 +9,14 @@ from __future__ import annotations
 import json
 import re
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
 
 from meshes import bulge, route_settings, uv_sphere
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 def run(mod, name, template, faces, target, levels=1, target_faces=None, nodes_initial=60, overrides=None,
@@ -287,21 +290,38 @@ def test_the_device_graph_builder_is_the_oracles_bit_for_bit(device, oracle):
 
 
 @pytest.mark.gpu
-def test_a_level_too_large_for_one_solve_fails_at_construction(device):
-    """One solve takes at most 16,384 nodes on this device (a persistent
-    launch of two node slots per lane group); a schedule whose finest level
-    is larger is refused when the session or the server job is created, not
-    after the coarse levels have run."""
-    from nricp_amd import workloads
+@pytest.mark.parametrize("form", ["route_a", "double", "mixed", "float"])
+def test_a_level_above_one_persistent_solve_matches_the_oracle(device, form):
+    """A level of 16,500 nodes is more than one persistent PCG launch holds
+    on this device (16,384), so its solves run as the multi-launch PCG. One
+    Gauss-Newton step of a sphere onto a bumped sphere against the oracle's
+    route A (its result is the fixture tests/golden/large_level_route_a.npz,
+    written by tests/golden/make_large_level.py: the oracle takes a minute at
+    this size): vertices within 1e-5 RMS and 1e-4 max of the box diagonal,
+    the same level size, correspondence counts and outer iterations. The
+    same job submitted to a server gives the same bits."""
+    sys.path.insert(0, str(GOLDEN))
+    import make_large_level as large
 
-    template, faces = workloads.geodesic_sphere(41)  # 16,812 vertices
-    assert len(template) > 16_384
-    levels = workloads.level_settings(1, outer=1)
-    with pytest.raises(ValueError, match="at most 16384"):
-        device.RouteBSession(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_500)
-    with device.Server(max_active=1) as server:
-        with pytest.raises(ValueError, match="at most 16384"):
-            server.submit(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_500)
-    # Just inside the limit the session is created.
-    session = device.RouteBSession(template, faces, template, faces, levels, graph_levels=1, nodes_initial=16_384)
-    assert not session.finished()
+    fixture = np.load(GOLDEN / "large_level_route_a.npz")
+    job = large.job()
+    args, kwargs = large.call_args(job)
+    if form == "route_a":
+        d = device.solve_route_a(*args, **kwargs)
+    else:
+        kwargs.update(precision=form, pcg_tolerance=1e-10, pcg_max_iterations=5000)
+        d = device.solve_route_b(*args, **kwargs)
+    expected = job.template_points + fixture["displacement"].astype(np.float64)
+    diag = float(np.linalg.norm(job.template_points.max(0) - job.template_points.min(0)))
+    e = np.linalg.norm(np.asarray(d["vertices"]) - expected, axis=1) / diag
+    rms, worst = float(np.sqrt(np.mean(e ** 2))), float(e.max())
+    print(f"16,500-node level, {form}: rms {rms:.3e} max {worst:.3e} of the diagonal")
+    assert rms <= 1e-5 and worst <= 1e-4, (rms, worst)
+    assert list(d["level_nodes"]) == list(fixture["level_nodes"]) == [large.NODES]
+    report = json.loads(d["report"])
+    assert [report["correspondence"][k] for k in large.COUNTS] == list(fixture["correspondence"])
+    assert [l["outer_iterations"] for l in report["levels"]] == list(fixture["outer_iterations"])
+    if form == "double":
+        with device.Server(max_active=1) as server:
+            served = server.result(server.submit(*args, route="route_b", **kwargs))
+        assert np.array_equal(served["vertices"], d["vertices"])
