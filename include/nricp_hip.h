@@ -285,6 +285,13 @@ typedef struct nricp_hip_settings {
     int32_t pcg_max_iterations;
     int32_t precision;
     int32_t route;
+    /* Nonzero: each damping attempt's successor (the damping times
+     * damping_increase) is solved beside it on a stream of its own, from the
+     * same system, so a rejected attempt's successor is ready when the
+     * rejection is known. The same decisions and results either way; more
+     * device work per accepted attempt (no reference counterpart). Routes A
+     * and B only. */
+    int32_t speculative_attempts;
 } nricp_hip_settings;
 
 /* The wrap problem, both meshes already in the template graph's normalized

@@ -740,7 +740,7 @@ std::unique_ptr<Prepared> prepare(const Dbl& tp, const Idx& tf, const Dbl& gp, c
                                   const py::object& mask, const py::object& lmi, const py::object& lmt, double dist,
                                   double mincos, double slope, double d0, double ddec, double dinc, int attempts,
                                   bool diag, int graph_levels, int n0, int growth, double pcg_tol, int pcg_max,
-                                  int neighbours, int route, const std::string& precision) {
+                                  int neighbours, int route, const std::string& precision, bool speculative) {
     auto p = std::make_unique<Prepared>();
     p->tp = pts(tp, "template_points");
     p->tf = tris(tf, "template_faces");
@@ -792,6 +792,7 @@ std::unique_ptr<Prepared> prepare(const Dbl& tp, const Idx& tf, const Dbl& gp, c
     s.pcg_max_iterations = pcg_max;
     s.precision = precision_form;
     s.route = route;
+    s.speculative_attempts = speculative ? 1 : 0;
     return p;
 }
 
@@ -1162,12 +1163,12 @@ py::dict solve(const std::string& route, const Dbl& tp, const Idx& tf, const Dbl
                const py::list& levels, const py::object& mask, const py::object& lmi, const py::object& lmt,
                double dist, double mincos, double slope, double d0, double ddec, double dinc, int attempts, bool diag,
                int graph_levels, int n0, int growth, double pcg_tol, int pcg_max, int neighbours,
-               const std::string& precision) {
+               const std::string& precision, bool speculative) {
     const int r = route_of(route);
     const bool exact = r == NRICP_HIP_ROUTE_A;
     auto p = prepare(tp, tf, gp, gf, levels, mask, lmi, lmt, dist, mincos, slope, d0, ddec, dinc, attempts, diag,
                      graph_levels, n0, growth, exact ? kRouteATolerance : pcg_tol,
-                     exact ? kRouteAMaxIterations : pcg_max, neighbours, r, exact ? "double" : precision);
+                     exact ? kRouteAMaxIterations : pcg_max, neighbours, r, exact ? "double" : precision, speculative);
     Session s(std::move(p));
     s.run();
     return s.result();
@@ -1786,27 +1787,31 @@ PYBIND11_MODULE(nricp_dp, m) {
         "Fit the template onto the target and return the result plus the run report. Distances are in the "
         "caller's units; template and target must already be aligned. precision (device routes B and C): "
         "'double' (FP64 throughout), 'mixed' (FP32 PCG iterations, FP64 refinement against the FP64 system) or "
-        "'float' (FP32 assembly blocks and PCG iterations, FP64 refinement of that FP32 system).";
+        "'float' (FP32 assembly blocks and PCG iterations, FP64 refinement of that FP32 system). "
+        "speculative_attempts (routes A and B): on levels of 1,024 nodes or more, solve each damping attempt's "
+        "successor beside it on a stream of its own, so a rejected attempt's successor is ready at once; the "
+        "same decisions and results, more device work per accepted attempt.";
     m.def("solve_route_a", [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                               const py::object& mk, const py::object& li, const py::object& lt, double dist,
                               double mincos, double slope, double d0, double ddec, double dinc, int att, bool diag,
-                              int gl, int n0, int g, double pt, int pm, int nb) {
+                              int gl, int n0, int g, double pt, int pm, int nb, bool spec) {
         return solve("route_a", tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag, gl, n0, g,
-                     pt, pm, nb, "double");
-    }, ROUTE_ARGS, route_doc);
+                     pt, pm, nb, "double", spec);
+    }, ROUTE_ARGS, py::arg("speculative_attempts") = true, route_doc);
     m.def("solve_route_b", [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                               const py::object& mk, const py::object& li, const py::object& lt, double dist,
                               double mincos, double slope, double d0, double ddec, double dinc, int att, bool diag,
-                              int gl, int n0, int g, double pt, int pm, int nb, const std::string& precision) {
+                              int gl, int n0, int g, double pt, int pm, int nb, const std::string& precision,
+                              bool spec) {
         return solve("route_b", tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag, gl, n0, g,
-                     pt, pm, nb, precision);
-    }, ROUTE_ARGS, py::arg("precision") = "mixed", route_doc);
+                     pt, pm, nb, precision, spec);
+    }, ROUTE_ARGS, py::arg("precision") = "mixed", py::arg("speculative_attempts") = true, route_doc);
     m.def("solve_route_c", [](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                               const py::object& mk, const py::object& li, const py::object& lt, double dist,
                               double mincos, double slope, double d0, double ddec, double dinc, int att, bool diag,
                               int gl, int n0, int g, double pt, int pm, int nb, const std::string& precision) {
         return solve("route_c", tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag, gl, n0, g,
-                     pt, pm, nb, precision);
+                     pt, pm, nb, precision, false);
     }, ROUTE_ARGS, py::arg("precision") = "mixed", route_doc);
 
     py::class_<Session>(m, "RouteBSession",
@@ -1815,7 +1820,8 @@ PYBIND11_MODULE(nricp_dp, m) {
         .def(py::init([](const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                          const py::object& mk, const py::object& li, const py::object& lt, double dist, double mincos,
                          double slope, double d0, double ddec, double dinc, int att, bool diag, int gl, int n0, int g,
-                         double pt, int pm, const std::string& route, int nb, const std::string& precision) {
+                         double pt, int pm, const std::string& route, int nb, const std::string& precision,
+                         bool spec) {
                  // 'route_a' is accepted as well: the FP64 route, solved to
                  // convergence, driven step by step.
                  const int r = route_of(route);
@@ -1823,7 +1829,7 @@ PYBIND11_MODULE(nricp_dp, m) {
                  return std::make_unique<Session>(prepare(tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec,
                                                           dinc, att, diag, gl, n0, g, exact ? kRouteATolerance : pt,
                                                           exact ? kRouteAMaxIterations : pm, nb, r,
-                                                          exact ? "double" : precision));
+                                                          exact ? "double" : precision, spec));
              }),
              py::arg("template_points"), py::arg("template_faces"), py::arg("target_points"), py::arg("target_faces"),
              py::arg("level_settings"), py::arg("mask") = py::none(), py::arg("landmark_indices") = py::none(),
@@ -1834,7 +1840,7 @@ PYBIND11_MODULE(nricp_dp, m) {
              py::arg("max_damping_attempts") = 6, py::arg("diagonal_damping") = true, py::arg("graph_levels") = 4,
              py::arg("nodes_initial") = 100, py::arg("growth") = 2, py::arg("pcg_tolerance") = 1e-5,
              py::arg("pcg_max_iterations") = 60, py::arg("route") = "route_b", py::arg("graph_neighbours") = 4,
-             py::arg("precision") = "mixed")
+             py::arg("precision") = "mixed", py::arg("speculative_attempts") = true)
         .def("step", &Session::step)
         .def("finished", &Session::finished)
         .def("vertices", &Session::vertices)
@@ -1866,9 +1872,10 @@ PYBIND11_MODULE(nricp_dp, m) {
                  const int r = route_of(route);
                  if (r == NRICP_HIP_ROUTE_C) throw py::value_error("the local/global route does not run in a batch");
                  const bool exact = r == NRICP_HIP_ROUTE_A;
+                 // The packed engine decides the attempts on the device, one at a time.
                  return self.add(prepare(tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att, diag,
                                          gl, n0, g, exact ? kRouteATolerance : pt, exact ? kRouteAMaxIterations : pm, nb,
-                                         r, exact ? "double" : precision));
+                                         r, exact ? "double" : precision, false));
              },
              ROUTE_ARGS, py::arg("route") = "route_b", py::arg("precision") = "mixed")
         .def("run", &Batch::run)
@@ -1888,14 +1895,18 @@ PYBIND11_MODULE(nricp_dp, m) {
              [](Server& self, const Dbl& tp, const Idx& tf, const Dbl& gp, const Idx& gf, const py::list& l,
                 const py::object& mk, const py::object& li, const py::object& lt, double dist, double mincos,
                 double slope, double d0, double ddec, double dinc, int att, bool diag, int gl, int n0, int g, double pt,
-                int pm, int nb, const std::string& route, const std::string& precision) {
+                int pm, int nb, const std::string& route, const std::string& precision, bool spec) {
                  const int r = route_of(route);
                  const bool exact = r == NRICP_HIP_ROUTE_A;
                  return self.submit(prepare(tp, tf, gp, gf, l, mk, li, lt, dist, mincos, slope, d0, ddec, dinc, att,
                                             diag, gl, n0, g, exact ? kRouteATolerance : pt,
-                                            exact ? kRouteAMaxIterations : pm, nb, r, exact ? "double" : precision));
+                                            exact ? kRouteAMaxIterations : pm, nb, r, exact ? "double" : precision,
+                                            spec));
              },
-             ROUTE_ARGS, py::arg("route") = "route_b", py::arg("precision") = "mixed")
+             // Off by default here: the server's jobs share the device, and the
+             // speculative attempts' extra work then costs the other jobs.
+             ROUTE_ARGS, py::arg("route") = "route_b", py::arg("precision") = "mixed",
+             py::arg("speculative_attempts") = false)
         .def("result", &Server::result, py::arg("ticket"), py::arg("timeout") = py::none())
         .def("stats", &Server::stats)
         .def("close", &Server::close)

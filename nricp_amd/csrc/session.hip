@@ -207,13 +207,27 @@ void KernelClock::begin(const char* name, hipStream_t s) {
     pending_.push_back(p);
 }
 
-bool KernelClock::external(const char* name, hipEvent_t* start, hipEvent_t* stop) {
+bool KernelClock::external(const char* name, hipEvent_t* start, hipEvent_t* stop, bool held) {
     if (!on_) return false;
-    Pending p{name, take(), take()};
+    Pending p{name, take(), take(), held};
     *start = p.start;
     *stop = p.stop;
     pending_.push_back(p);
     return true;
+}
+
+void KernelClock::resolve(hipEvent_t start, bool keep) {
+    for (auto it = pending_.begin(); it != pending_.end(); ++it) {
+        if (it->start != start) continue;
+        if (keep) {
+            it->held = false;
+        } else {
+            pool_.push_back(it->start);
+            pool_.push_back(it->stop);
+            pending_.erase(it);
+        }
+        return;
+    }
 }
 
 void KernelClock::end(hipStream_t s) {
@@ -226,6 +240,10 @@ void KernelClock::collect(bool wait) {
     // another stream) stay pending unless the caller waits for them.
     std::vector<Pending> still;
     for (Pending& p : pending_) {
+        if (p.held) {  // not yet known whether its time counts
+            still.push_back(p);
+            continue;
+        }
         if (!wait) {
             const hipError_t q = hipEventQuery(p.stop);
             if (q == hipErrorNotReady) {
@@ -359,6 +377,17 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     rejection_.resize(nv_);
     energy_cur_.resize(5);
     energy_cand_.resize(5);
+    speculate_ = s.speculative_attempts != 0 && s.route != NRICP_HIP_ROUTE_C && s.max_damping_attempts > 1;
+    if (speculate_) {
+        spec_stream_ = std::make_unique<Stream>();
+        StreamScope spec_scope(spec_stream_->get());
+        spec_rot_.resize(9 * nmax);
+        spec_trans_.resize(3 * nmax);
+        spec_delta_.resize(6 * nmax);
+        spec_energy_.resize(5);
+        NRICP_CHECK(hipEventCreateWithFlags(&system_ready_, hipEventDisableTiming));
+        NRICP_CHECK(hipEventCreateWithFlags(&spec_done_, hipEventDisableTiming));
+    }
     energy_fit_.resize(5);
     summary_.resize(6);
     quantiles_.resize(2);
@@ -371,10 +400,13 @@ Session::~Session() {
     (void)hipStreamSynchronize(prep_stream_.get());
     (void)hipStreamSynchronize(fps_stream_.get());
     (void)hipStreamSynchronize(stream_.get());
+    if (spec_stream_) (void)hipStreamSynchronize(spec_stream_->get());
     for (hipEvent_t e : prep_ready_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : fps_ready_) (void)hipEventDestroy(e);
     if (order_ready_) (void)hipEventDestroy(order_ready_);
+    if (system_ready_) (void)hipEventDestroy(system_ready_);
+    if (spec_done_) (void)hipEventDestroy(spec_done_);
 }
 
 KernelClock& Session::clock() {
@@ -384,6 +416,7 @@ KernelClock& Session::clock() {
     NRICP_CHECK(hipStreamSynchronize(prep_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(fps_stream_.get()));
     NRICP_CHECK(hipStreamSynchronize(stream_.get()));
+    if (spec_stream_) NRICP_CHECK(hipStreamSynchronize(spec_stream_->get()));
     clock_.collect(true);
     prep_clock_.collect(true);
     for (auto& kv : prep_clock_.totals) {
@@ -673,23 +706,24 @@ void Session::refresh() {
     assembled_ = false;
 }
 
-void Session::solve(double damping) {
-    hipStream_t st = stream_.get();
+hipEvent_t Session::solve(double damping, PcgWorkspace& ws, double* delta, hipStream_t st, bool held) {
     const int64_t nn = graph_->levels[level_].node_count;
     const int max_it = settings_.pcg_max_iterations > 0 ? settings_.pcg_max_iterations : 1;
     // The clock brackets exactly the persistent kernel (the preconditioner
     // set-up before it is a separate, much shorter launch).
-    pcg_.kernel_start = pcg_.kernel_stop = nullptr;
-    clock_.external("pcg", &pcg_.kernel_start, &pcg_.kernel_stop);
+    ws.kernel_start = ws.kernel_stop = nullptr;
+    clock_.external("pcg", &ws.kernel_start, &ws.kernel_stop, held);
+    const hipEvent_t clocked = ws.kernel_start;
     if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE)
         pcg_solve_mixed(layout(), bsr32_.get(),
                         settings_.precision == NRICP_HIP_PRECISION_MIXED ? bsr_.get() : nullptr, rhs_.get(), nn,
                         damping, settings_.diagonal_damping != 0,
-                        settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
+                        settings_.pcg_tolerance, max_it, delta, ws, st);
     else
         pcg_solve(layout(), bsr_.get(), rhs_.get(), nn, damping, settings_.diagonal_damping != 0,
-                  settings_.pcg_tolerance, max_it, delta_.get(), pcg_, st);
-    pcg_.kernel_start = pcg_.kernel_stop = nullptr;
+                  settings_.pcg_tolerance, max_it, delta, ws, st);
+    ws.kernel_start = ws.kernel_stop = nullptr;
+    return clocked;
 }
 
 // One inner iteration with all its damping attempts (src/solver/route_a.cpp:131-169).
@@ -728,6 +762,25 @@ bool Session::inner_step() {
     const int attempts = std::max(1, settings_.max_damping_attempts);
     bool accepted = false;
     double cur = 0.0, cand = 0.0;
+    // Work and algorithmic traffic of one decided attempt. Per iteration the
+    // symmetric matrix once (N diagonal and P pair blocks of 36 values) and
+    // the exchanged 6-vector written and read once; per solve the
+    // preconditioner written and read once, the rhs read and the solution
+    // written (FP64); the mixed form's refinement residual reads the FP64
+    // matrix at least once more. Candidate energy pass: its inputs once.
+    auto count_attempt = [&](double iterations) {
+        const double s = settings_.precision == NRICP_HIP_PRECISION_DOUBLE ? 8.0 : 4.0;
+        const double P = static_cast<double>(layout().group_count), N = static_cast<double>(nn);
+        counters_[kPcgLaunches] += 1.0;
+        counters_[kPcgIterations] += iterations;
+        counters_[kPcgMaxIterations] = std::max(counters_[kPcgMaxIterations], iterations);
+        counters_[kPcgBytes] += iterations * s * (36.0 * (N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
+        if (settings_.precision == NRICP_HIP_PRECISION_MIXED) counters_[kPcgBytes] += 8.0 * 36.0 * (N + P);
+        counters_[kEnergyLaunches] += 1.0;
+        counters_[kEnergyBytes] += 134.0 * static_cast<double>(nv_) + 120.0 * N +
+                                   8.0 * static_cast<double>(graph_->levels[level_].edge_count) + 40.0;
+        counters_[kGaussNewtonSteps] += 1.0;
+    };
     for (int a = 0; a < attempts; ++a) {
         if (local_global) {
             // Local step in place, then the translation right-hand side of the
@@ -744,7 +797,15 @@ bool Session::inner_step() {
             translation_rhs(inputs(rot_.get(), trans_.get()), layout(), assembly_, rhs_.get(), energy_fit_.get(), st);
             clock_.end(st);
         }
-        solve(damping_);
+        // Attempt a + 1 (the damping times damping_increase) beside attempt a,
+        // from the same system and state, on its own stream.
+        const bool spec = speculate_ && a + 1 < attempts && nn >= kSpeculateMinNodes;
+        hipStream_t ss = spec ? spec_stream_->get() : nullptr;
+        if (spec) {
+            NRICP_CHECK(hipEventRecord(system_ready_, st));
+            NRICP_CHECK(hipStreamWaitEvent(ss, system_ready_, 0));
+        }
+        solve(damping_, pcg_, delta_.get(), st);
         retract(rot_.get(), trans_.get(), delta_.get(), nn, rot_cand_.get(), trans_cand_.get(), st);
         clock_.begin("energy", st);
         // The candidate's energies, and with them the current energies and
@@ -753,6 +814,17 @@ bool Session::inner_step() {
         const HostReadback readback{staging_device_, energy_cur_.get(), 5, pcg_.outcome.get(), 4};
         energies(inputs(rot_cand_.get(), trans_cand_.get()), assembly_, energy_cand_.get(), st, &readback);
         clock_.end(st);
+        hipEvent_t spec_clock = nullptr;
+        if (spec) {
+            StreamScope spec_scope(ss);
+            spec_clock = solve(damping_ * settings_.damping_increase, spec_pcg_, spec_delta_.get(), ss, true);
+            retract(rot_.get(), trans_.get(), spec_delta_.get(), nn, spec_rot_.get(), spec_trans_.get(), ss);
+            // [32, 37) its candidate's energies, [37, 41) its outcome.
+            const HostReadback spec_readback{staging_device_ + kSpecStaging, nullptr, 0, spec_pcg_.outcome.get(), 4};
+            energies(inputs(spec_rot_.get(), spec_trans_.get()), spec_scratch_, spec_energy_.get(), ss,
+                     &spec_readback);
+            NRICP_CHECK(hipEventRecord(spec_done_, ss));
+        }
         sync();
         const double* h = staging_.get();
         cur = (((h[0] + h[1]) + h[2]) + h[3]) + h[4];
@@ -762,41 +834,46 @@ bool Session::inner_step() {
             trace_.push_back(cur);
             if (outer_ == 0 && inner_ == 0) view_.energy_start = cur;
         }
-        // Algorithmic traffic of the solve: per iteration the symmetric
-        // matrix once (N diagonal and P pair blocks of 36 values) and the
-        // exchanged 6-vector written and read once; per solve the
-        // preconditioner written and read once, the rhs read and the solution
-        // written (FP64); the mixed form's refinement residual reads the FP64
-        // matrix at least once more. Candidate energy pass: its inputs once.
-        {
-            const double s = settings_.precision == NRICP_HIP_PRECISION_DOUBLE ? 8.0 : 4.0;
-            const double P = static_cast<double>(layout().group_count), N = static_cast<double>(nn);
-            counters_[kPcgLaunches] += 1.0;
-            counters_[kPcgIterations] += h[10];
-            counters_[kPcgMaxIterations] = std::max(counters_[kPcgMaxIterations], h[10]);
-            counters_[kPcgBytes] += h[10] * s * (36.0 * (N + P) + 12.0 * N) + s * 72.0 * N + 96.0 * N;
-            if (settings_.precision == NRICP_HIP_PRECISION_MIXED) counters_[kPcgBytes] += 8.0 * 36.0 * (N + P);
-            counters_[kEnergyLaunches] += 1.0;
-            counters_[kEnergyBytes] += 134.0 * static_cast<double>(nv_) + 120.0 * N +
-                                       8.0 * static_cast<double>(graph_->levels[level_].edge_count) + 40.0;
-            counters_[kGaussNewtonSteps] += 1.0;
-        }
+        count_attempt(h[10]);
         const bool ok = h[12] != 0.0;
-        if (!ok) {
-            damping_ *= settings_.damping_increase;
-            continue;
-        }
         cand = (((h[5] + h[6]) + h[7]) + h[8]) + h[9];
-        if (cand < cur) {
+        if (ok && cand < cur) {
             accepted = true;
             for (int k = 0; k < 5; ++k) last_energy_[k] = h[5 + k];
+            if (spec) {
+                // Not needed: the main stream orders everything after it (the
+                // next writes of the system, state and correspondences) behind
+                // its work, and its time leaves the clock.
+                NRICP_CHECK(hipStreamWaitEvent(st, spec_done_, 0));
+                if (spec_clock) clock_.resolve(spec_clock, false);
+            }
             break;
         }
-        if (local_global) {
+        if (ok && local_global) {
             NRICP_CHECK(hipMemcpyAsync(rot_.get(), rot_prev_.get(), 9 * nn * sizeof(double), hipMemcpyDeviceToDevice,
                                        st));
             NRICP_CHECK(hipMemcpyAsync(trans_.get(), trans_prev_.get(), 3 * nn * sizeof(double),
                                        hipMemcpyDeviceToDevice, st));
+        }
+        damping_ *= settings_.damping_increase;
+        if (!spec) continue;
+        // Attempt a + 1 is the one solved beside it: decided as the loop
+        // would decide it, with the same current energy.
+        ++a;
+        NRICP_CHECK(hipStreamSynchronize(ss));
+        if (spec_clock) clock_.resolve(spec_clock, true);
+        const double* g = staging_.get() + kSpecStaging;
+        count_attempt(g[5]);
+        const bool spec_ok = g[7] != 0.0;
+        const double spec_cand = (((g[0] + g[1]) + g[2]) + g[3]) + g[4];
+        if (spec_ok && spec_cand < cur) {
+            accepted = true;
+            cand = spec_cand;
+            for (int k = 0; k < 5; ++k) last_energy_[k] = g[k];
+            std::swap(rot_cand_, spec_rot_);
+            std::swap(trans_cand_, spec_trans_);
+            std::swap(energy_cand_, spec_energy_);
+            break;
         }
         damping_ *= settings_.damping_increase;
     }

@@ -8,6 +8,7 @@
 #include <condition_variable>
 #include <exception>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -57,6 +58,11 @@ int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level);
 // midway. Levels above one persistent solve's size (pcg_max_nodes) are solved
 // by the multi-launch PCG.
 constexpr int64_t kMaxLevelNodes = int64_t{1} << 25;
+// Levels below this many nodes solve each attempt in one or a few
+// workgroups, and a rejected attempt costs little; there the speculative
+// attempt's launches cost more than they save, so it runs only on larger
+// levels.
+constexpr int64_t kSpeculateMinNodes = 1024;
 void check_level_sizes(const nricp_hip_problem& problem, const nricp_hip_settings& settings,
                        const DeviceGraph* graph);
 
@@ -86,7 +92,11 @@ public:
     void end(hipStream_t s);
     // A pending entry whose two events the caller records itself; returns
     // false (and no events) when the clock is off.
-    bool external(const char* name, hipEvent_t* start, hipEvent_t* stop);
+    // With `held`, the entry is not collected until resolve() keeps it.
+    bool external(const char* name, hipEvent_t* start, hipEvent_t* stop, bool held = false);
+    // Keeps (counted when collected) or drops the held entry whose start
+    // event is `start`.
+    void resolve(hipEvent_t start, bool keep);
     void collect(bool wait);  // wait: block on entries not yet reached
     std::map<std::string, std::pair<double, int64_t>> totals;  // name -> (ms, launches)
     ~KernelClock();
@@ -95,6 +105,7 @@ private:
     struct Pending {
         std::string name;
         hipEvent_t start, stop;
+        bool held = false;
     };
     bool on_ = false;
     std::vector<Pending> pending_;
@@ -161,7 +172,9 @@ private:
     bool inner_step();
     bool close_outer();
     bool end_level();
-    void solve(double damping);
+    // One damping attempt's solve on `st` with workspace `ws` into `delta`;
+    // returns the clock's start event of it (null when the clock is off).
+    hipEvent_t solve(double damping, PcgWorkspace& ws, double* delta, hipStream_t st, bool held = false);
     void final_statistics();
     void sync();
 
@@ -207,6 +220,15 @@ private:
     std::exception_ptr prep_error_;
     AssemblyScratch assembly_;
     PcgWorkspace pcg_;
+    // The speculative attempt (settings_.speculative_attempts): its own
+    // stream, solve workspace, candidate state and energy scratch.
+    bool speculate_ = false;
+    std::unique_ptr<Stream> spec_stream_;  // only when speculating
+    PcgWorkspace spec_pcg_;
+    AssemblyScratch spec_scratch_;
+    DeviceBuffer<double> spec_delta_, spec_rot_, spec_trans_, spec_energy_;
+    hipEvent_t system_ready_ = nullptr;  // the attempt's system is assembled (main stream)
+    hipEvent_t spec_done_ = nullptr;     // the speculative attempt's work is done (its stream)
     // state
     DeviceBuffer<double> rot_, trans_, rot_cand_, trans_cand_, rot_prev_, trans_prev_, delta_;
     DeviceBuffer<double> deformed_, qnormals_, corr_points_, corr_normals_, corr_dist_, robust_;
@@ -224,7 +246,10 @@ private:
     DeviceBuffer<double> rhs_;
     DeviceBuffer<double> energy_cur_, energy_cand_, energy_fit_, summary_, summary_partials_, quantiles_;
     DeviceBuffer<uint64_t> quantile_keys_;
-    static constexpr int kStaging = 32;
+    // [0, 14) an attempt's readback, [16, 26) the final statistics, [32, 41)
+    // the speculative attempt's readback.
+    static constexpr int kStaging = 48;
+    static constexpr int kSpecStaging = 32;
     PinnedBuffer<double> staging_{kStaging};
     double* staging_device_ = nullptr;  // staging_ as the device addresses it
     // Writes a (na values), then b, then c into staging_[at ...) on the

@@ -325,3 +325,37 @@ def test_a_level_above_one_persistent_solve_matches_the_oracle(device, form):
         with device.Server(max_active=1) as server:
             served = server.result(server.submit(*args, route="route_b", **kwargs))
         assert np.array_equal(served["vertices"], d["vertices"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["route_a", "mixed", "float"])
+def test_speculative_attempts_change_nothing(device, form):
+    """Each damping attempt's successor solved beside it (levels of 1,024
+    nodes or more) gives the same bits, the same report and the same solve
+    counts as the attempts one after another: two levels of 2,000 and 4,000
+    nodes, five correspondence updates each."""
+    from nricp_amd import workloads
+
+    job = workloads.sphere_job(41, 60, 5, nodes_final=4_000, graph_levels=2)
+    settings = workloads.level_settings(2, outer=5, inner=1)
+    args = (job.template_points, job.template_faces, job.target_points, job.target_faces, settings)
+    # A small initial damping: the first steps overshoot and are rejected.
+    kwargs = dict(graph_levels=2, nodes_initial=2_000, growth=2, graph_neighbours=8, initial_damping=1e-7)
+    if form == "route_a":
+        kwargs.update(route="route_a")
+    else:
+        kwargs.update(precision=form, pcg_tolerance=1e-8, pcg_max_iterations=50)
+
+    def run(speculative):
+        session = device.RouteBSession(*args, speculative_attempts=speculative, **kwargs)
+        session.run()
+        return session.result(), session.counters()
+
+    (on, on_counts), (off, off_counts) = run(True), run(False)
+    assert np.array_equal(on["vertices"], off["vertices"])
+    assert json.loads(on["report"])["levels"] == json.loads(off["report"])["levels"]
+    for key in ("pcg_launches", "pcg_iterations", "gauss_newton_solves"):
+        assert on_counts[key] == off_counts[key], key
+    # Rejected attempts are what the speculation hides: this job has some.
+    outer = sum(l["outer_iterations"] for l in json.loads(on["report"])["levels"])
+    assert on_counts["gauss_newton_solves"] > outer
