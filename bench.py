@@ -81,7 +81,7 @@ OUTER = 10
 # averaged over a wrap's launches: measured on the 200k-vertex job by the
 # counter passes recorded in profiles/pcg_kernel_traffic_geodesic-200k.md, per
 # (config, form, PCG tolerance). Other jobs and tolerances report null.
-PCG_TRAFFIC = {(2, "float", 1e-8): 3.297e7}
+PCG_TRAFFIC = {(2, "float", 1e-8): 3.185e7}
 # Device-timed stages of a wrap (HIP events around each stage's launches).
 STAGES = ("fps", "graph", "target_bvh", "layout", "deform", "closest_points", "assemble", "pcg", "energy",
           "local_step")
