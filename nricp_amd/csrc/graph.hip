@@ -680,6 +680,86 @@ __device__ inline void distance_weights(const double* d2, int support, double* w
         if (k < support) w[k] = total > 0.0 ? w[k] / total : 1.0 / support;
 }
 
+// ------------------------------------------------------ brute force ----
+//
+// The k nearest of a few thousand points, for a few thousand queries: one
+// wave per query, kBruteWaves queries per block sharing each tile of
+// candidate points in LDS. Each lane keeps its own sorted KMAX best among the
+// candidates it visits; the wave then merges the lanes' lists in KMAX rounds
+// of a butterfly arg-min. The order is the grid search's (the key, then the
+// index), so the answer is the same bits: Root orders by sqrt(d^2) (the
+// Euclidean distance, taken only when a candidate's d^2 is within rounding of
+// the lane's current KMAX-th), otherwise by d^2.
+constexpr int kBruteWaves = 16;
+constexpr int kBruteThreads = kBruteWaves * kWave;
+constexpr int kBruteTile = kBruteThreads;  // candidate points per LDS tile
+// Query-candidate pairs up to which the brute-force searches replace the
+// grid's. Measured on the 200k template's levels (kNN of 2,000 / 4,000 /
+// 8,000 nodes: 45 / 88 / 438 us by brute force against 165 / 154 / 183 us on
+// the grid; the level transfer onto 4,000 / 8,000 nodes: 60 / 95 against
+// 91 / 80 us): the grid's dependent cell visits leave a small level's
+// launch latency-bound, and every pair costs more beyond a few million.
+constexpr int64_t kBruteMaxPairs = int64_t{1} << 24;
+
+template <int KMAX, bool Root>
+__device__ inline void brute_nearest(d3 x, int64_t self, const double* __restrict__ pts, int64_t np, bool active,
+                                     double* tile /* 3 x kBruteTile */, double (&best)[KMAX],
+                                     int32_t (&best_id)[KMAX]) {
+    const int lane = threadIdx.x % kWave;
+    double dist[KMAX];
+    int32_t id[KMAX];
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        dist[m] = __builtin_inf();
+        id[m] = 0x7fffffff;
+    }
+    for (int64_t base = 0; base < np; base += kBruteTile) {
+        const int count = static_cast<int>(np - base < kBruteTile ? np - base : kBruteTile);
+        __syncthreads();
+        for (int t = threadIdx.x; t < 3 * count; t += kBruteThreads) tile[t] = pts[3 * base + t];
+        __syncthreads();
+        if (!active) continue;
+        for (int t = lane; t < count; t += kWave) {
+            const int64_t j = base + t;
+            if (j == self) continue;
+            const double d2 = norm2(x - d3{tile[3 * t], tile[3 * t + 1], tile[3 * t + 2]});
+            const double last = dist[KMAX - 1];
+            // sqrt is monotone and correctly rounded: a d^2 this far above
+            // the KMAX-th one's square cannot tie or beat it.
+            if (Root ? !(d2 <= last * last * (1.0 + 1e-15)) : !(d2 <= last)) continue;
+            const double key = Root ? sqrt(d2) : d2;
+            const int32_t jj = static_cast<int32_t>(j);
+            if (key < last || (key == last && jj < id[KMAX - 1])) insert_sorted<KMAX>(key, jj, dist, id);
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        double bd = dist[0];
+        int32_t bi = id[0];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double od = __shfl_xor(bd, off);
+            const int32_t oi = __shfl_xor(bi, off);
+            if (od < bd || (od == bd && oi < bi)) {
+                bd = od;
+                bi = oi;
+            }
+        }
+        best[m] = bd;
+        best_id[m] = bi;
+        if (dist[0] == bd && id[0] == bi) {
+#pragma unroll
+            for (int q = 0; q + 1 < KMAX; ++q) {
+                dist[q] = dist[q + 1];
+                id[q] = id[q + 1];
+            }
+            dist[KMAX - 1] = __builtin_inf();
+            id[KMAX - 1] = 0x7fffffff;
+        }
+    }
+}
+
 constexpr int kBindThreads = 128;
 
 __global__ void __launch_bounds__(kBindThreads)
@@ -699,6 +779,25 @@ bind_kernel(const double* __restrict__ p, int64_t n, GridView grid, int support,
         ow[4 * v + k] = k < support ? w[k] : 0.0;
     }
     oc[v] = support;
+}
+
+// Node i's first k neighbours as sortable (min, max) pair keys; entries past
+// the nn - 1 other nodes are all ones.
+template <int KMAX>
+__device__ inline void emit_knn_keys(int64_t i, int64_t nn, int k, const int32_t (&id)[KMAX],
+                                     uint64_t* __restrict__ keys) {
+    const int held = static_cast<int>(nn - 1 < k ? nn - 1 : k);
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m) {
+        if (m >= k) break;
+        uint64_t key = ~0ull;
+        if (m < held) {
+            const uint64_t a = static_cast<uint64_t>(i < id[m] ? i : id[m]);
+            const uint64_t b = static_cast<uint64_t>(i < id[m] ? id[m] : i);
+            key = a * static_cast<uint64_t>(nn) + b;
+        }
+        keys[static_cast<int64_t>(k) * i + m] = key;
+    }
 }
 
 // k nearest other nodes by (Euclidean distance, index); emits the (min, max)
@@ -732,17 +831,21 @@ knn_kernel(const double* __restrict__ nodes, int64_t nn, GridView grid, int k, u
             for (int m = 1; m < KMAX; ++m) kth = m == held - 1 ? dist[m] : kth;
             return kth < bound;
         });
-#pragma unroll
-    for (int m = 0; m < KMAX; ++m) {
-        if (m >= k) break;
-        uint64_t key = ~0ull;
-        if (m < held) {
-            const uint64_t a = static_cast<uint64_t>(i < id[m] ? i : id[m]);
-            const uint64_t b = static_cast<uint64_t>(i < id[m] ? id[m] : i);
-            key = a * static_cast<uint64_t>(nn) + b;
-        }
-        keys[static_cast<int64_t>(k) * i + m] = key;
-    }
+    emit_knn_keys<KMAX>(i, nn, k, id, keys);
+}
+
+// The same k nearest, by brute force (few nodes): one wave per node.
+template <int KMAX>
+__global__ void __launch_bounds__(kBruteThreads)
+knn_brute(const double* __restrict__ nodes, int64_t nn, int k, uint64_t* __restrict__ keys) {
+    __shared__ double tile[3 * kBruteTile];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBruteWaves + threadIdx.x / kWave;
+    const bool active = i < nn;
+    const d3 x = active ? ld3(nodes, i) : d3{0.0, 0.0, 0.0};
+    double dist[KMAX];
+    int32_t id[KMAX];
+    brute_nearest<KMAX, true>(x, i, nodes, nn, active, tile, dist, id);
+    if (active && threadIdx.x % kWave == 0) emit_knn_keys<KMAX>(i, nn, k, id, keys);
 }
 
 __global__ void unique_edges(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flags,
@@ -753,6 +856,10 @@ __global__ void unique_edges(const uint64_t* __restrict__ keys, const int32_t* _
     edges[2 * e] = static_cast<int32_t>(keys[i] / static_cast<uint64_t>(nn));
     edges[2 * e + 1] = static_cast<int32_t>(keys[i] % static_cast<uint64_t>(nn));
 }
+
+__device__ inline void transfer_node(const double* __restrict__ from, const double* __restrict__ R,
+                                     const double* __restrict__ T, int64_t i, d3 x, int support, const double* d2,
+                                     const int32_t* id, double* __restrict__ Ro, double* __restrict__ To);
 
 // Blend of the bound source rotations in quaternion space, signs aligned to
 // the heaviest contributor, heaviest-neighbour fallback when the blend nearly
@@ -767,6 +874,31 @@ transfer_kernel(const double* __restrict__ from, GridView grid, const double* __
     double d2[4];
     int32_t id[4];
     nearest4(x, grid, support, d2, id);
+    transfer_node(from, R, T, i, x, support, d2, id, Ro, To);
+}
+
+// The same transfer with the 4 nearest coarse nodes by brute force (few
+// coarse nodes): one wave per fine node.
+__global__ void __launch_bounds__(kBruteThreads)
+transfer_brute(const double* __restrict__ from, int64_t nf, const double* __restrict__ R,
+               const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
+               double* __restrict__ Ro, double* __restrict__ To) {
+    __shared__ double tile[3 * kBruteTile];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBruteWaves + threadIdx.x / kWave;
+    const bool active = i < nt;
+    const d3 x = active ? ld3(to, i) : d3{0.0, 0.0, 0.0};
+    double d2[4];
+    int32_t id[4];
+    brute_nearest<4, false>(x, -1, from, nf, active, tile, d2, id);
+    if (active && threadIdx.x % kWave == 0) transfer_node(from, R, T, i, x, support, d2, id, Ro, To);
+}
+
+// A fine node's rotation and translation from its (up to) 4 nearest coarse
+// nodes: distance weights, rotations blended as quaternions in the
+// hemisphere of the heaviest one, positions moved by each and averaged.
+__device__ inline void transfer_node(const double* __restrict__ from, const double* __restrict__ R,
+                                     const double* __restrict__ T, int64_t i, d3 x, int support, const double* d2,
+                                     const int32_t* id, double* __restrict__ Ro, double* __restrict__ To) {
     double w[4];
     distance_weights(d2, support, w);
     int32_t heavy = id[0];
@@ -965,15 +1097,27 @@ int64_t build_node_edges(const double* nodes, int64_t nn, const NodeGrid& grid, 
     const int64_t m = nn * kk;
     keys.resize(m);
     if (grid.count != nn) throw InvalidArgument("build_node_edges: the grid is not over these nodes");
-    const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
-    const GridView view = view_of(grid);
-    if (kk <= 4)
-        knn_kernel<4><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
-    else if (kk <= 8)
-        knn_kernel<8><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
-    else
-        knn_kernel<16><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
-    NRICP_LAUNCH_CHECK("knn_kernel");
+    if (nn * nn <= kBruteMaxPairs) {
+        // Few nodes: every pair, one wave per node.
+        const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kBruteWaves)));
+        if (kk <= 4)
+            knn_brute<4><<<blocks, kBruteThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        else if (kk <= 8)
+            knn_brute<8><<<blocks, kBruteThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        else
+            knn_brute<16><<<blocks, kBruteThreads, 0, stream>>>(nodes, nn, kk, keys.get());
+        NRICP_LAUNCH_CHECK("knn_brute");
+    } else {
+        const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
+        const GridView view = view_of(grid);
+        if (kk <= 4)
+            knn_kernel<4><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+        else if (kk <= 8)
+            knn_kernel<8><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+        else
+            knn_kernel<16><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+        NRICP_LAUNCH_CHECK("knn_kernel");
+    }
     radix_sort_pairs(keys.get(), nullptr, m, bits_for(static_cast<uint64_t>(nn) * static_cast<uint64_t>(nn)), scratch,
                      stream);
     flags.resize(m + 1);
@@ -997,6 +1141,12 @@ void transfer_level(const double* from, int64_t nf, const NodeGrid& from_grid, c
     if (nf <= 0) throw InvalidArgument("transfer_level: the coarser level has no nodes");
     if (from_grid.count != nf) throw InvalidArgument("transfer_level: the grid is not over the coarser nodes");
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nf, 4)), 4}));
+    if (nf * nt <= kBruteMaxPairs) {
+        transfer_brute<<<static_cast<unsigned>(blocks_for(nt, kBruteWaves)), kBruteThreads, 0, stream>>>(
+            from, nf, R, T, to, nt, support, Ro, To);
+        NRICP_LAUNCH_CHECK("transfer_brute");
+        return;
+    }
     transfer_kernel<<<blocks_for(nt, kBindThreads), kBindThreads, 0, stream>>>(from, view_of(from_grid), R, T, to, nt,
                                                                                 support, Ro, To);
     NRICP_LAUNCH_CHECK("transfer_kernel");

@@ -16,6 +16,8 @@
 #include <iomanip>
 #include <sstream>
 
+#include <sys/prctl.h>
+
 #include "session.hpp"
 
 namespace nricp_hip {
@@ -547,6 +549,10 @@ void Session::prepare() {
 void Session::build_levels_ahead() {
     try {
         NRICP_CHECK(hipSetDevice(device_));
+        // The polls below sleep 20 us at a time; with the kernel's default
+        // 50 us timer slack each would last 70 us or more, and the finest
+        // level's build, which the wrap waits for, would start that late.
+        (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
         hipStream_t ps = prep_stream_.get();
         StreamScope scope(ps);  // the levels' buffers are ordered on the building stream
         prep_clock_.enable(clock_.enabled());
@@ -560,7 +566,7 @@ void Session::build_levels_ahead() {
                     const hipError_t q = hipEventQuery(fps_ready_[l]);
                     if (q == hipSuccess) break;
                     if (q != hipErrorNotReady) NRICP_CHECK(q);
-                    std::this_thread::sleep_for(std::chrono::microseconds(100));
+                    std::this_thread::sleep_for(std::chrono::microseconds(20));
                 }
                 NRICP_CHECK(hipStreamWaitEvent(ps, fps_ready_[l], 0));
                 prep_clock_.begin("graph", ps);
