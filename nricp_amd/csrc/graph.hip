@@ -800,15 +800,27 @@ __device__ inline void emit_knn_keys(int64_t i, int64_t nn, int k, const int32_t
     }
 }
 
-// k nearest other nodes by (Euclidean distance, index); emits the (min, max)
-// pair of each as a sortable key. KMAX >= k entries are kept (the first k are
-// the answer).
-constexpr int kKnnThreads = 128;
-template <int KMAX>
-__global__ void __launch_bounds__(kKnnThreads)
-knn_kernel(const double* __restrict__ nodes, int64_t nn, GridView grid, int k, uint64_t* __restrict__ keys) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nn) return;
+// The grid search of one query spread over a wave: each shell's cells are
+// dealt to the lanes, each lane keeps its own sorted KMAX best, and after
+// each shell the wave merges the lanes' lists (without consuming them) to test
+// the stop condition on the need-th best, which is grid_search's test on the
+// same candidates. Root orders by sqrt(d^2) and stops when the need-th best
+// is closer than the bound; otherwise by d^2, stopping when it is below the
+// bound's square. Every lane of the wave returns the merged list.
+constexpr int kWaveQueryThreads = 256;
+constexpr int kWaveQueries = kWaveQueryThreads / kWave;
+
+template <int KMAX, bool Root>
+__device__ inline void wave_grid_nearest(const GridView& g, d3 x, int64_t self, int need, double (&best)[KMAX],
+                                         int32_t (&best_id)[KMAX]) {
+    const int lane = threadIdx.x % kWave;
+    const double o[3] = {g.frame[0], g.frame[1], g.frame[2]};
+    const double h = g.frame[3];
+    const double xv[3] = {x.x, x.y, x.z};
+    int c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = grid_axis_cell(xv[k], o[k], h, g.cells);
+    const double margin = 1e-9 * h;
     double dist[KMAX];
     int32_t id[KMAX];
 #pragma unroll
@@ -816,22 +828,79 @@ knn_kernel(const double* __restrict__ nodes, int64_t nn, GridView grid, int k, u
         dist[m] = __builtin_inf();
         id[m] = 0x7fffffff;
     }
-    const d3 x = ld3(nodes, i);
-    const int held = static_cast<int>(nn - 1 < k ? nn - 1 : k);
-    grid_search(
-        grid, x,
-        [&](int32_t j, d3 g) {
-            if (j == i) return;
-            const double s = sqrt(norm2(x - g));
-            if (s < dist[KMAX - 1] || (s == dist[KMAX - 1] && j < id[KMAX - 1])) insert_sorted<KMAX>(s, j, dist, id);
-        },
-        [&](double bound) {
-            double kth = dist[0];
+    for (int r = 0;; ++r) {
+        const int side = 2 * r + 1;
+        const int cube = side * side * side;
+        for (int q = lane; q < cube; q += kWave) {
+            const int dz = q / (side * side) - r, dy = (q / side) % side - r, dx = q % side - r;
+            if (abs(dx) != r && abs(dy) != r && abs(dz) != r) continue;  // inside: an earlier shell
+            const int cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= g.cells || cy >= g.cells || cz >= g.cells) continue;
+            const int64_t cell = (static_cast<int64_t>(cz) * g.cells + cy) * g.cells + cx;
+            const int32_t e = g.runs[2 * cell + 1];
+            for (int32_t t = g.runs[2 * cell]; t < e; ++t) {
+                const double4 p = g.sorted[t];
+                const int32_t j = static_cast<int32_t>(p.w);
+                if (j == self) continue;
+                const double d2 = norm2(x - d3{p.x, p.y, p.z});
+                const double key = Root ? sqrt(d2) : d2;
+                if (key < dist[KMAX - 1] || (key == dist[KMAX - 1] && j < id[KMAX - 1]))
+                    insert_sorted<KMAX>(key, j, dist, id);
+            }
+        }
+        // The merged best so far, the lanes' lists left intact.
+        int head = 0;
 #pragma unroll
-            for (int m = 1; m < KMAX; ++m) kth = m == held - 1 ? dist[m] : kth;
-            return kth < bound;
-        });
-    emit_knn_keys<KMAX>(i, nn, k, id, keys);
+        for (int m = 0; m < KMAX; ++m) {
+            double hd = __builtin_inf();
+            int32_t hi = 0x7fffffff;
+#pragma unroll
+            for (int q = 0; q < KMAX; ++q)
+                if (q == head) {
+                    hd = dist[q];
+                    hi = id[q];
+                }
+            double bd = hd;
+            int32_t bi = hi;
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const double od = __shfl_xor(bd, off);
+                const int32_t oi = __shfl_xor(bi, off);
+                if (od < bd || (od == bd && oi < bi)) {
+                    bd = od;
+                    bi = oi;
+                }
+            }
+            best[m] = bd;
+            best_id[m] = bi;
+            if (hd == bd && hi == bi && head < KMAX) ++head;
+        }
+        double bound = __builtin_inf();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (c[k] - r > 0) bound = fmin(bound, xv[k] - (o[k] + static_cast<double>(c[k] - r) * h));
+            if (c[k] + r < g.cells - 1) bound = fmin(bound, (o[k] + static_cast<double>(c[k] + r + 1) * h) - xv[k]);
+        }
+        if (bound == __builtin_inf()) return;
+        const double b = bound - margin;
+        double kth = best[0];
+#pragma unroll
+        for (int m = 1; m < KMAX; ++m) kth = m == need - 1 ? best[m] : kth;
+        if (Root ? kth < b : (b > 0.0 && kth < b * b)) return;
+    }
+}
+
+// kNN edges of a large level: one wave per node.
+template <int KMAX>
+__global__ void __launch_bounds__(kWaveQueryThreads)
+knn_wave(const double* __restrict__ nodes, int64_t nn, GridView grid, int k, uint64_t* __restrict__ keys) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kWaveQueries + threadIdx.x / kWave;
+    if (i >= nn) return;  // the whole wave
+    const int held = static_cast<int>(nn - 1 < k ? nn - 1 : k);
+    double dist[KMAX];
+    int32_t id[KMAX];
+    wave_grid_nearest<KMAX, true>(grid, ld3(nodes, i), i, held, dist, id);
+    if (threadIdx.x % kWave == 0) emit_knn_keys<KMAX>(i, nn, k, id, keys);
 }
 
 // The same k nearest, by brute force (few nodes): one wave per node.
@@ -864,17 +933,17 @@ __device__ inline void transfer_node(const double* __restrict__ from, const doub
 // Blend of the bound source rotations in quaternion space, signs aligned to
 // the heaviest contributor, heaviest-neighbour fallback when the blend nearly
 // cancels; translation maps the new rest position onto the field's value.
-__global__ void __launch_bounds__(kBindThreads)
-transfer_kernel(const double* __restrict__ from, GridView grid, const double* __restrict__ R,
-                const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
-                double* __restrict__ Ro, double* __restrict__ To) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
+__global__ void __launch_bounds__(kWaveQueryThreads)
+transfer_wave(const double* __restrict__ from, GridView grid, const double* __restrict__ R,
+              const double* __restrict__ T, const double* __restrict__ to, int64_t nt, int support,
+              double* __restrict__ Ro, double* __restrict__ To) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kWaveQueries + threadIdx.x / kWave;
+    if (i >= nt) return;  // the whole wave
     const d3 x = ld3(to, i);
     double d2[4];
     int32_t id[4];
-    nearest4(x, grid, support, d2, id);
-    transfer_node(from, R, T, i, x, support, d2, id, Ro, To);
+    wave_grid_nearest<4, false>(grid, x, -1, support, d2, id);
+    if (threadIdx.x % kWave == 0) transfer_node(from, R, T, i, x, support, d2, id, Ro, To);
 }
 
 // The same transfer with the 4 nearest coarse nodes by brute force (few
@@ -1108,15 +1177,15 @@ int64_t build_node_edges(const double* nodes, int64_t nn, const NodeGrid& grid, 
             knn_brute<16><<<blocks, kBruteThreads, 0, stream>>>(nodes, nn, kk, keys.get());
         NRICP_LAUNCH_CHECK("knn_brute");
     } else {
-        const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kKnnThreads)));
+        const dim3 blocks(static_cast<unsigned>(blocks_for(nn, kWaveQueries)));
         const GridView view = view_of(grid);
         if (kk <= 4)
-            knn_kernel<4><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+            knn_wave<4><<<blocks, kWaveQueryThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
         else if (kk <= 8)
-            knn_kernel<8><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+            knn_wave<8><<<blocks, kWaveQueryThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
         else
-            knn_kernel<16><<<blocks, kKnnThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
-        NRICP_LAUNCH_CHECK("knn_kernel");
+            knn_wave<16><<<blocks, kWaveQueryThreads, 0, stream>>>(nodes, nn, view, kk, keys.get());
+        NRICP_LAUNCH_CHECK("knn_wave");
     }
     radix_sort_pairs(keys.get(), nullptr, m, bits_for(static_cast<uint64_t>(nn) * static_cast<uint64_t>(nn)), scratch,
                      stream);
@@ -1147,9 +1216,9 @@ void transfer_level(const double* from, int64_t nf, const NodeGrid& from_grid, c
         NRICP_LAUNCH_CHECK("transfer_brute");
         return;
     }
-    transfer_kernel<<<blocks_for(nt, kBindThreads), kBindThreads, 0, stream>>>(from, view_of(from_grid), R, T, to, nt,
-                                                                                support, Ro, To);
-    NRICP_LAUNCH_CHECK("transfer_kernel");
+    transfer_wave<<<static_cast<unsigned>(blocks_for(nt, kWaveQueries)), kWaveQueryThreads, 0, stream>>>(
+        from, view_of(from_grid), R, T, to, nt, support, Ro, To);
+    NRICP_LAUNCH_CHECK("transfer_wave");
 }
 
 }  // namespace nricp_hip

@@ -18,6 +18,7 @@ where exact ties have probability zero.
 
 from __future__ import annotations
 
+import functools
 import json
 
 import numpy as np
@@ -185,6 +186,17 @@ def _awkward_node_sets():
     yield "three nodes", rng.normal(size=(3, 3)), rng.normal(size=(100, 3))
     sheet = rng.uniform(size=(5000, 3)) * np.array([10.0, 10.0, 0.01])
     yield "thin sheet", sheet[oracle_free_fps(sheet, 700)], sheet
+    # Above 2^24 query-node pairs the device searches the grid one wave per
+    # query instead of every pair: the same awkward shapes at that size.
+    big = np.stack(np.meshgrid(*(np.arange(17),) * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    yield "large lattice (exact ties)", big, rng.uniform(-1, 17, size=(4000, 3))
+    wide = rng.uniform(size=(20000, 3)) * np.array([30.0, 30.0, 0.01])
+    yield "large thin sheet", wide[oracle_free_fps(wide, 4500)], wide[:5000]
+
+
+@functools.lru_cache(maxsize=None)
+def _awkward_cases():
+    return list(_awkward_node_sets())
 
 
 def oracle_free_fps(points, count):
@@ -198,12 +210,14 @@ def oracle_free_fps(points, count):
     return np.asarray(chosen)
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(8))
 def test_grid_searches_match_brute_force_on_awkward_node_sets(device, oracle, case):
-    # The device searches nodes through a uniform grid; the oracle scans them
-    # all. Lattices give exact distance ties, coincident nodes a degenerate
-    # box, far queries long searches, a thin sheet flat cells.
-    name, nodes, queries = list(_awkward_node_sets())[case]
+    # The device searches nodes through a uniform grid (the kNN edges and the
+    # level transfer: every pair up to 2^24 pairs, a wave per query beyond);
+    # the oracle scans them all. Lattices give exact distance ties, coincident
+    # nodes a degenerate box, far queries long searches, a thin sheet flat
+    # cells.
+    name, nodes, queries = _awkward_cases()[case]
     for k in (1, 4):
         d = device.bind_vertices(queries, nodes, k)
         o = oracle.bind_vertices(queries, nodes, k)
