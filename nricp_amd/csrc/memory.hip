@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "buffers.hpp"
 #include "stream_pool.hpp"
@@ -114,6 +116,65 @@ void stream_retired(hipStream_t s) {
     cache().pool.stream_retired(s);
 }
 
+namespace {
+struct IdleStreams {
+    std::mutex mutex;
+    std::map<int, std::vector<hipStream_t>> idle;  // device -> synchronised streams
+    bool closed = false;
+};
+IdleStreams& idle_streams() {
+    static IdleStreams* s = new IdleStreams;  // never destroyed: used until exit
+    return *s;
+}
+constexpr size_t kIdleStreamsPerDevice = 64;
+}  // namespace
+
+hipStream_t stream_acquire() {
+    int device = 0;
+    NRICP_CHECK(hipGetDevice(&device));
+    hipStream_t s = nullptr;
+    {
+        IdleStreams& pool = idle_streams();
+        std::lock_guard<std::mutex> lock(pool.mutex);
+        auto& v = pool.idle[device];
+        if (!v.empty()) {
+            s = v.back();
+            v.pop_back();
+        }
+    }
+    if (!s) NRICP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    stream_created(s);
+    return s;
+}
+
+void stream_release(hipStream_t s) {
+    stream_retired(s);
+    int device = -1;
+    if (hipStreamGetDevice(s, &device) != hipSuccess) {
+        (void)hipGetLastError();
+        device = -1;
+    }
+    IdleStreams& pool = idle_streams();
+    bool room = false;
+    if (device >= 0) {
+        std::lock_guard<std::mutex> lock(pool.mutex);
+        room = !pool.closed && pool.idle[device].size() < kIdleStreamsPerDevice;
+    }
+    if (room) {
+        // Synchronised: its persistent launches leave the admission ledger
+        // (their events kept for reuse) before another owner can take it and
+        // admit launches of its own on it.
+        launches_completed(s);
+        std::lock_guard<std::mutex> lock(pool.mutex);
+        if (!pool.closed) {
+            pool.idle[device].push_back(s);
+            return;
+        }
+    }
+    launches_retired(s);
+    (void)hipStreamDestroy(s);
+}
+
 void release_all() {
     Cache& c = cache();
     int count = 0;
@@ -132,6 +193,17 @@ void release_all() {
         });
     }
     (void)hipSetDevice(current);
+    {
+        IdleStreams& pool = idle_streams();
+        std::lock_guard<std::mutex> lock(pool.mutex);
+        pool.closed = true;
+        for (auto& kv : pool.idle)
+            for (hipStream_t s : kv.second) {
+                launches_retired(s);
+                (void)hipStreamDestroy(s);
+            }
+        pool.idle.clear();
+    }
     launches_teardown();
 }
 
