@@ -176,6 +176,7 @@ void BatchEngine::begin_level(int level) {
             s.level_ = static_cast<size_t>(level);
         }
         s.begin_level();
+        s.ensure_layout();
         const GraphLevel& L = s.graph_->levels[s.level_];
         const LevelLayout& lay = s.layout();
         const int64_t nv = s.nv_, nn = L.node_count, np = lay.group_count;

@@ -362,8 +362,11 @@ Session::Session(const nricp_hip_problem& pb, const nricp_hip_settings& s, const
     }
     layouts_.resize(graph_->levels.size());
     prep_ready_.resize(graph_->levels.size(), nullptr);
-    for (size_t l = 1; l < prep_ready_.size(); ++l)
+    graph_ready_.resize(graph_->levels.size(), nullptr);
+    for (size_t l = 1; l < prep_ready_.size(); ++l) {
         NRICP_CHECK(hipEventCreateWithFlags(&prep_ready_[l], hipEventDisableTiming));
+        NRICP_CHECK(hipEventCreateWithFlags(&graph_ready_[l], hipEventDisableTiming));
+    }
     if (s.max_nodes_per_vertex < 1) throw InvalidArgument("max_nodes_per_vertex must be at least 1");
     if (s.graph_neighbours < 0 || s.graph_neighbours > 16) throw InvalidArgument("graph_neighbours must be in [0, 16]");
     // Buffers at the finest level's size.
@@ -404,6 +407,8 @@ Session::~Session() {
     (void)hipStreamSynchronize(stream_.get());
     if (spec_stream_) (void)hipStreamSynchronize(spec_stream_->get());
     for (hipEvent_t e : prep_ready_)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : graph_ready_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : fps_ready_) (void)hipEventDestroy(e);
     if (order_ready_) (void)hipEventDestroy(order_ready_);
@@ -541,6 +546,7 @@ void Session::prepare() {
     // The later levels' graphs and layouts, ahead of the solve.
     join_prep();
     prep_recorded_ = 0;
+    graph_recorded_ = 0;
     prep_failed_ = false;
     prep_error_ = nullptr;
     if (graph_->levels.size() > 1) prep_thread_ = std::thread([this] { build_levels_ahead(); });
@@ -576,6 +582,12 @@ void Session::build_levels_ahead() {
                                   prep_sort_, ps);
                 prep_clock_.end(ps);
             }
+            NRICP_CHECK(hipEventRecord(graph_ready_[l], ps));
+            {
+                std::lock_guard<std::mutex> lock(prep_mutex_);
+                graph_recorded_ = static_cast<size_t>(l);
+            }
+            prep_cv_.notify_all();
             const GraphLevel& L = graph_->levels[l];
             prep_clock_.begin("layout", ps);
             build_level_layout(L.bind_nodes.get(), L.bind_weights.get(), L.bind_counts.get(), nv_, L.edges.get(),
@@ -630,13 +642,13 @@ void Session::reset() {
 void Session::begin_level() {
     hipStream_t st = stream_.get();
     if (level_ > 0) {
-        // Built ahead (build_levels_ahead): wait until the level's event is
-        // recorded, then order the main stream after it.
+        // Built ahead (build_levels_ahead): wait until the level's graph
+        // event is recorded, then order the main stream after it.
         std::unique_lock<std::mutex> lock(prep_mutex_);
-        prep_cv_.wait(lock, [this] { return prep_failed_ || prep_recorded_ >= level_; });
-        if (prep_failed_ && prep_recorded_ < level_) std::rethrow_exception(prep_error_);
+        prep_cv_.wait(lock, [this] { return prep_failed_ || graph_recorded_ >= level_; });
+        if (prep_failed_ && graph_recorded_ < level_) std::rethrow_exception(prep_error_);
         lock.unlock();
-        NRICP_CHECK(hipStreamWaitEvent(st, prep_ready_[level_], 0));
+        NRICP_CHECK(hipStreamWaitEvent(st, graph_ready_[level_], 0));
     } else if (!shared_graph_) {
         NRICP_CHECK(hipStreamWaitEvent(st, fps_ready_[level_], 0));
         clock_.begin("graph", st);
@@ -664,10 +676,7 @@ void Session::begin_level() {
                            L.edge_count, L.node_count, L.node_positions.get(), layout(), sort_, st);
         clock_.end(st);
     }
-    // FP64 blocks for the double and mixed forms, FP32 blocks for the mixed
-    // form's iterations and the all-FP32 form.
-    if (settings_.precision != NRICP_HIP_PRECISION_FLOAT) bsr_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
-    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE) bsr32_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
+    layout_pending_ = true;
     view_ = LevelView{};
     view_.level = static_cast<int>(level_);
     view_.nodes = static_cast<int>(L.node_count);
@@ -676,6 +685,23 @@ void Session::begin_level() {
     outer_ = 0;
     inner_ = 0;
     current_energy_ = 0.0;
+}
+
+void Session::ensure_layout() {
+    if (!layout_pending_) return;
+    hipStream_t st = stream_.get();
+    if (level_ > 0) {
+        std::unique_lock<std::mutex> lock(prep_mutex_);
+        prep_cv_.wait(lock, [this] { return prep_failed_ || prep_recorded_ >= level_; });
+        if (prep_failed_ && prep_recorded_ < level_) std::rethrow_exception(prep_error_);
+        lock.unlock();
+        NRICP_CHECK(hipStreamWaitEvent(st, prep_ready_[level_], 0));
+    }
+    // FP64 blocks for the double and mixed forms, FP32 blocks for the mixed
+    // form's iterations and the all-FP32 form.
+    if (settings_.precision != NRICP_HIP_PRECISION_FLOAT) bsr_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
+    if (settings_.precision != NRICP_HIP_PRECISION_DOUBLE) bsr32_.resize(36 * std::max<int64_t>(layout().nnzb, 1));
+    layout_pending_ = false;
 }
 
 void Session::refresh() {
@@ -734,6 +760,7 @@ hipEvent_t Session::solve(double damping, PcgWorkspace& ws, double* delta, hipSt
 
 // One inner iteration with all its damping attempts (src/solver/route_a.cpp:131-169).
 bool Session::inner_step() {
+    ensure_layout();
     hipStream_t st = stream_.get();
     const nricp_hip_level_settings& ls = level_settings();
     const int64_t nn = graph_->levels[level_].node_count;

@@ -167,7 +167,11 @@ private:
     void build_levels_ahead();  // body of prep_thread_
     void join_prep();
     LevelLayout& layout() { return layouts_[level_]; }
+    // A level starts with its graph: the deformation transfer and the first
+    // correspondence pass need nothing else. Its block layout (built ahead
+    // with it) is waited for only before the first assembly.
     void begin_level();
+    void ensure_layout();
     void refresh();
     bool inner_step();
     bool close_outer();
@@ -211,11 +215,14 @@ private:
     Stream prep_stream_;
     SortScratch prep_sort_;
     KernelClock prep_clock_;
-    std::vector<hipEvent_t> prep_ready_;  // per level (from 1): graph level and layout written
+    std::vector<hipEvent_t> graph_ready_;  // per level (from 1): graph level written
+    std::vector<hipEvent_t> prep_ready_;   // per level (from 1): graph level and layout written
     std::thread prep_thread_;
     std::mutex prep_mutex_;
     std::condition_variable prep_cv_;
-    size_t prep_recorded_ = 0;            // levels 1 .. prep_recorded_ have their events recorded
+    size_t graph_recorded_ = 0;           // levels 1 .. graph_recorded_ have graph_ready_ recorded
+    size_t prep_recorded_ = 0;            // levels 1 .. prep_recorded_ have prep_ready_ recorded
+    bool layout_pending_ = false;         // the level's layout not yet waited for (ensure_layout)
     bool prep_failed_ = false;
     std::exception_ptr prep_error_;
     AssemblyScratch assembly_;
