@@ -31,6 +31,17 @@ NRICP_HD inline int64_t xcd_contiguous(int64_t b, int64_t total) {
     return x * q + xcd_min(x, r) + i;
 }
 
+// A bijection of [0, total) that gives each XCD runs of `chunk` consecutive
+// logical indices, the runs dealt to the XCDs in turn: workgroup b (XCD
+// b % 8, its (b / 8)-th there) takes run (b / 8) / chunk of that XCD. The
+// indices past the last whole round of 8 runs map to themselves.
+NRICP_HD inline int64_t xcd_chunked(int64_t b, int64_t total, int64_t chunk) {
+    const int64_t round = kXcds * chunk;
+    if (chunk <= 1 || b >= total / round * round) return b;
+    const int64_t x = b % kXcds, i = b / kXcds;
+    return (i / chunk) * round + x * chunk + i % chunk;
+}
+
 // A launch of `total` workgroups that does two kinds of work: the first
 // `first` logical items of kind 0, the remaining total - first of kind 1.
 // Each XCD takes a contiguous eighth of kind 0 (as xcd_contiguous over

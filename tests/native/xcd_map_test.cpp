@@ -1,6 +1,7 @@
-// Host test of the XCD work mapping (nricp_amd/csrc/xcd_map.hpp): both
-// mappings are bijections, each XCD's logical indices are one contiguous run
-// (per kind), and the runs follow the XCD order. Exits non-zero on failure.
+// Host test of the XCD work mapping (nricp_amd/csrc/xcd_map.hpp): the
+// mappings are bijections; xcd_contiguous and xcd_split give each XCD one
+// contiguous run (per kind), in XCD order; xcd_chunked gives it runs of
+// `chunk` dealt in turn. Exits non-zero on failure.
 #include <cstdio>
 #include <vector>
 
@@ -20,6 +21,32 @@ static int failures = 0;
     } while (0)
 
 int main() {
+    // xcd_chunked: a bijection; within each whole round of 8 runs, XCD x
+    // (workgroups b = x + 8 i) takes the run [round start + x chunk, + chunk)
+    // in order; past the last whole round every index maps to itself.
+    for (int64_t chunk : {1, 2, 3, 16}) {
+        for (int64_t total = 0; total <= 700; total += 7) {
+            std::vector<int> seen(total, 0);
+            const int64_t round = kXcds * chunk, whole = total / round * round;
+            for (int64_t b = 0; b < total; ++b) {
+                const int64_t l = xcd_chunked(b, total, chunk);
+                EXPECT(l >= 0 && l < total, "chunk %lld total %lld b %lld -> %lld", (long long)chunk,
+                       (long long)total, (long long)b, (long long)l);
+                if (l < 0 || l >= total) continue;
+                ++seen[l];
+                const int64_t x = b % kXcds, i = b / kXcds;
+                const int64_t want = b >= whole || chunk <= 1 ? b : (i / chunk) * round + x * chunk + i % chunk;
+                EXPECT(l == want, "chunk %lld total %lld b %lld -> %lld, not %lld", (long long)chunk,
+                       (long long)total, (long long)b, (long long)l, (long long)want);
+                if (b < whole && chunk > 1)
+                    EXPECT(l / chunk % kXcds == x, "chunk %lld total %lld b %lld: run of another XCD",
+                           (long long)chunk, (long long)total, (long long)b);
+            }
+            for (int64_t l = 0; l < total; ++l)
+                EXPECT(seen[l] == 1, "chunk %lld total %lld index %lld seen %d", (long long)chunk, (long long)total,
+                       (long long)l, seen[l]);
+        }
+    }
     for (int64_t total = 0; total <= 300; ++total) {
         std::vector<int> seen(total, 0);
         std::vector<int64_t> lo(kXcds, total), hi(kXcds, -1), cnt(kXcds, 0);
