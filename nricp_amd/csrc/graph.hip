@@ -763,10 +763,11 @@ __device__ inline void brute_nearest(d3 x, int64_t self, const double* __restric
 constexpr int kBindThreads = 128;
 
 __global__ void __launch_bounds__(kBindThreads)
-bind_kernel(const double* __restrict__ p, int64_t n, GridView grid, int support, int32_t* __restrict__ on,
-            double* __restrict__ ow, int32_t* __restrict__ oc) {
-    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
+bind_kernel(const double* __restrict__ p, int64_t n, const int32_t* __restrict__ order, GridView grid, int support,
+            int32_t* __restrict__ on, double* __restrict__ ow, int32_t* __restrict__ oc) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t v = order ? order[t] : t;
     const d3 x = ld3(p, v);
     double d2[4];
     int32_t id[4];
@@ -1144,13 +1145,13 @@ void build_node_grid(const double* nodes, int64_t nn, NodeGrid& g, SortScratch& 
 }
 
 void bind_vertices(const double* points, int64_t n, const double* nodes, int64_t nn, const NodeGrid& grid, int32_t k,
-                   int32_t* on, double* ow, int32_t* oc, hipStream_t stream) {
+                   int32_t* on, double* ow, int32_t* oc, hipStream_t stream, const int32_t* order) {
     if (n <= 0 || nn <= 0) return;
     if (grid.count != nn) throw InvalidArgument("bind_vertices: the grid is not over these nodes");
     (void)nodes;
     const int support = std::max(1, std::min<int>({k, static_cast<int>(std::min<int64_t>(nn, 4)), 4}));
-    bind_kernel<<<blocks_for(n, kBindThreads), kBindThreads, 0, stream>>>(points, n, view_of(grid), support, on, ow,
-                                                                           oc);
+    bind_kernel<<<blocks_for(n, kBindThreads), kBindThreads, 0, stream>>>(points, n, order, view_of(grid), support, on,
+                                                                           ow, oc);
     NRICP_LAUNCH_CHECK("bind_kernel");
 }
 

@@ -193,9 +193,12 @@ struct NodeGrid {
 void build_node_grid(const double* nodes, int64_t node_count, NodeGrid& grid, SortScratch& scratch,
                      hipStream_t stream);
 // Nearest `k` nodes by (d^2, index), weights (1 - d/dmax)^2 normalised.
+// `order` (a permutation of the vertices, or null for index order) is the
+// order the threads take the vertices in: a spatially coherent one keeps a
+// wave's searches in the same grid cells.
 void bind_vertices(const double* points, int64_t count, const double* nodes, int64_t node_count,
                    const NodeGrid& grid, int32_t k, int32_t* out_nodes, double* out_weights, int32_t* out_counts,
-                   hipStream_t stream);
+                   hipStream_t stream, const int32_t* order = nullptr);
 // k nearest by (distance, index) per node, union as sorted unique (min, max)
 // pairs. Returns the edge count (synchronises).
 int64_t build_node_edges(const double* nodes, int64_t node_count, const NodeGrid& grid, int32_t k,

@@ -106,7 +106,7 @@ int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level) {
 // ------------------------------------------------------------------ graph --
 
 void build_graph_level(const double* points, int64_t nv, const int32_t* seeds, int64_t node_count, int k,
-                       int neighbours, GraphLevel& L, SortScratch& scratch, hipStream_t stream) {
+                       int neighbours, GraphLevel& L, SortScratch& scratch, hipStream_t stream, const int32_t* order) {
     L.node_count = node_count;
     L.node_indices.resize(node_count);
     NRICP_CHECK(hipMemcpyAsync(L.node_indices.get(), seeds, node_count * sizeof(int32_t), hipMemcpyDeviceToDevice,
@@ -120,7 +120,7 @@ void build_graph_level(const double* points, int64_t nv, const int32_t* seeds, i
     L.bind_counts.resize(nv);
     build_node_grid(L.node_positions.get(), node_count, L.grid, scratch, stream);
     bind_vertices(points, nv, L.node_positions.get(), node_count, L.grid, k, L.bind_nodes.get(),
-                  L.bind_weights.get(), L.bind_counts.get(), stream);
+                  L.bind_weights.get(), L.bind_counts.get(), stream, order);
     L.edge_count = build_node_edges(L.node_positions.get(), node_count, L.grid, neighbours, L.edges, L.edge_keys,
                                     L.edge_flags, L.edge_pos, scratch, stream);
 }
@@ -579,7 +579,7 @@ void Session::build_levels_ahead() {
                 build_graph_level(points_.get(), nv_, seeds_.get(),
                                   level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, l),
                                   settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_.levels[l],
-                                  prep_sort_, ps);
+                                  prep_sort_, ps, fps_work_.order.get());
                 prep_clock_.end(ps);
             }
             NRICP_CHECK(hipEventRecord(graph_ready_[l], ps));
@@ -656,7 +656,7 @@ void Session::begin_level() {
         build_graph_level(points_.get(), nv_, seeds_.get(),
                           level_count_for(lv, settings_.nodes_initial, settings_.growth, nv_, static_cast<int>(level_)),
                           settings_.max_nodes_per_vertex, settings_.graph_neighbours, own_graph_.levels[level_], sort_,
-                          st);
+                          st, fps_work_.order.get());
         clock_.end(st);
     }
     const GraphLevel& L = graph_->levels[level_];

@@ -44,10 +44,11 @@ void build_device_graph(const double* points, int64_t vertex_count, int levels, 
                         int max_nodes_per_vertex, int neighbours, DeviceGraph& graph, SortScratch& scratch,
                         hipStream_t stream);
 // One level from the first `node_count` seeds: node positions, bindings and
-// kNN edges (synchronises the stream for the edge count).
+// kNN edges (synchronises the stream for the edge count). `order`, if given,
+// is a spatially coherent order of the vertices for the binding's threads.
 void build_graph_level(const double* points, int64_t vertex_count, const int32_t* seeds, int64_t node_count,
                        int max_nodes_per_vertex, int neighbours, GraphLevel& level, SortScratch& scratch,
-                       hipStream_t stream);
+                       hipStream_t stream, const int32_t* order = nullptr);
 
 // Nodes of level `level` of a graph of `levels` levels: min(round(n0 g^l), V).
 int64_t level_count_for(int levels, int n0, int growth, int64_t nv, int level);
