@@ -5,7 +5,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <cstring>
 #include <vector>
 
 #include "device_common.hpp"
@@ -179,42 +178,6 @@ public:
 private:
     T* p_ = nullptr;
     int64_t n_ = 0;
-};
-
-// Host waits on one stream for the host thread that owns it: an event the
-// host spins on (hipEventQuery reads the event's completion without
-// sleeping, where a blocking wait can leave the thread asleep for tens of
-// microseconds after the work is done) and pinned words for small readbacks.
-// Both are made on first use, on the device current then.
-class HostWait {
-public:
-    HostWait() = default;
-    ~HostWait() {
-        if (event_) (void)hipEventDestroy(event_);
-        if (words_) (void)hipHostFree(words_);
-    }
-    HostWait(const HostWait&) = delete;
-    HostWait& operator=(const HostWait&) = delete;
-
-    // Returns once all work enqueued on `s` so far is done.
-    void wait(hipStream_t s);
-
-    // The value at `device`, read back after the work enqueued on `s`.
-    template <typename T>
-    T read(const T* device, hipStream_t s) {
-        static_assert(sizeof(T) <= sizeof(uint64_t), "one word at most");
-        ensure();
-        NRICP_CHECK(hipMemcpyAsync(words_, device, sizeof(T), hipMemcpyDeviceToHost, s));
-        wait(s);
-        T v;
-        std::memcpy(&v, words_, sizeof(T));
-        return v;
-    }
-
-private:
-    void ensure();
-    hipEvent_t event_ = nullptr;
-    uint64_t* words_ = nullptr;
 };
 
 }  // namespace nricp_hip

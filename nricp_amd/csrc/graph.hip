@@ -1194,7 +1194,9 @@ int64_t build_node_edges(const double* nodes, int64_t nn, const NodeGrid& grid, 
     pos.resize(m + 1);
     run_starts(keys.get(), m, flags.get(), stream);
     exclusive_scan32(flags.get(), pos.get(), m, scratch, stream);
-    const int32_t count = scratch.host.read(pos.get() + m, stream);
+    int32_t count = 0;
+    NRICP_CHECK(hipMemcpyAsync(&count, pos.get() + m, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    NRICP_CHECK(hipStreamSynchronize(stream));
     edges.resize(2 * static_cast<int64_t>(count));
     unique_edges<<<blocks_for(m, 256), 256, 0, stream>>>(keys.get(), flags.get(), pos.get(), m, nn, edges.get());
     NRICP_LAUNCH_CHECK("unique_edges");

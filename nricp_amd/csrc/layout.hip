@@ -159,7 +159,12 @@ __global__ void group_rank_keys(const int32_t* __restrict__ pairs, const int32_t
     val[g] = static_cast<uint32_t>(g);
 }
 
-int32_t read_i32(const int32_t* p, SortScratch& scratch, hipStream_t s) { return scratch.host.read(p, s); }
+int32_t read_i32(const int32_t* p, hipStream_t s) {
+    int32_t v = 0;
+    NRICP_CHECK(hipMemcpyAsync(&v, p, sizeof(v), hipMemcpyDeviceToHost, s));
+    NRICP_CHECK(hipStreamSynchronize(s));
+    return v;
+}
 
 }  // namespace
 
@@ -183,8 +188,8 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     }
     exclusive_scan32(gcount, gpos, nv, scratch, stream);
     exclusive_scan32(pcount, ppos, nv, scratch, stream);
-    const int64_t ng = read_i32(gpos + nv, scratch, stream);
-    const int64_t nvp = read_i32(ppos + nv, scratch, stream);
+    const int64_t ng = read_i32(gpos + nv, stream);
+    const int64_t nvp = read_i32(ppos + nv, stream);
     const int64_t np_entries = nvp + ne;
     // 2. gather entries (node keys) and pair entries ((min,max) keys).
     const int64_t total = ng + np_entries + 2 * ne;
@@ -234,7 +239,9 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     int64_t valid_entries = np_entries;
     if (np_entries > 0) {
         // Entries below the sentinel: invalid or self edges sorted last.
-        const uint64_t last = scratch.host.read(pkey + np_entries - 1, stream);
+        uint64_t last = 0;
+        NRICP_CHECK(hipMemcpyAsync(&last, pkey + np_entries - 1, sizeof(last), hipMemcpyDeviceToHost, stream));
+        NRICP_CHECK(hipStreamSynchronize(stream));
         if (last == ~0ull) {
             std::vector<uint64_t> host(np_entries);
             NRICP_CHECK(hipMemcpyAsync(host.data(), pkey, np_entries * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
@@ -244,7 +251,7 @@ void build_level_layout(const int32_t* bn, const double* bw, const int32_t* bc, 
     }
     run_starts(pkey, valid_entries, L.flags.get(), stream);
     exclusive_scan32(L.flags.get(), L.runs.get(), valid_entries, scratch, stream);
-    const int64_t np = valid_entries > 0 ? read_i32(L.runs.get() + valid_entries, scratch, stream) : 0;
+    const int64_t np = valid_entries > 0 ? read_i32(L.runs.get() + valid_entries, stream) : 0;
     L.group_count = np;
     L.group_offsets.resize(np + 1);
     L.group_pairs.resize(std::max<int64_t>(2 * np, 1));

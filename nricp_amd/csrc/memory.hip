@@ -17,10 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
-#include <chrono>
 #include <map>
 #include <mutex>
-#include <thread>
 #include <vector>
 
 #include "buffers.hpp"
@@ -175,29 +173,6 @@ void stream_release(hipStream_t s) {
     }
     launches_retired(s);
     (void)hipStreamDestroy(s);
-}
-
-void HostWait::ensure() {
-    if (!event_) NRICP_CHECK(hipEventCreateWithFlags(&event_, hipEventDisableTiming));
-    if (!words_) NRICP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&words_), sizeof(uint64_t), hipHostMallocDefault));
-}
-
-void HostWait::wait(hipStream_t s) {
-    ensure();
-    NRICP_CHECK(hipEventRecord(event_, s));
-    // Spins for the first millisecond, then yields between queries so that a
-    // long wait leaves the core to other threads.
-    const auto start = std::chrono::steady_clock::now();
-    bool slow = false;
-    for (int polls = 0;; ++polls) {
-        const hipError_t q = hipEventQuery(event_);
-        if (q == hipSuccess) return;
-        if (q != hipErrorNotReady) NRICP_CHECK(q);
-        if (slow)
-            std::this_thread::yield();
-        else if ((polls & 63) == 63)
-            slow = std::chrono::steady_clock::now() - start > std::chrono::milliseconds(1);
-    }
 }
 
 void release_all() {

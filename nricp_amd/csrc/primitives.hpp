@@ -28,7 +28,6 @@ struct SortScratch {
     uint32_t epoch = 0;              // the last digit pass's tag
     int parity = 0;                  // the table the next sort uses
     DeviceBuffer<int64_t> scan_partials;  // tile sums of the scans
-    HostWait host;                        // the owning thread's readbacks and waits
 };
 
 // Sorts keys[0..n) (and vals alongside, when non-null) by the low `key_bits`

@@ -476,7 +476,7 @@ void Session::to_host(hipStream_t st, int at, const double* a, int na, const dou
 }
 
 void Session::sync() {
-    sort_.host.wait(stream_.get());
+    NRICP_CHECK(hipStreamSynchronize(stream_.get()));
     clock_.collect(false);
     if (stats_pending_) {
         const std::vector<uint64_t> h = query_stats_.totals.to_host(stream_.get());
